@@ -1,0 +1,165 @@
+/*
+ * swsph.h -- C ABI of the MI355X SPHENIX neighbour pass (libswsph.so).
+ *
+ * Plain pointers and sizes only.  Every call returns 0 on success and a
+ * non-zero status on error; swsph_last_error() returns the message of the
+ * last failure on the calling thread.  A SWIFT-side shim turns a non-zero
+ * status into error() (src/error.h:60-80), the reference's fail-stop
+ * convention.
+ *
+ * Model: an engine owns a device-resident copy of the gas particles
+ * (structure of arrays in HBM, sorted by top-level cell) and the top-level
+ * cell grid.  Cells are addressed by their index
+ * cid = (cx * ncell[1] + cy) * ncell[2] + cz on that grid.  The per-task
+ * entry points below are the drop-in for SWIFT's task functions; each one
+ * enqueues its work on the engine's stream and returns (the ghost waits for
+ * its own convergence test, as the reference's does).  Calls on one engine
+ * are serialised internally.
+ *
+ * Scope: SPHENIX, Wendland C2, 3D, gamma = 5/3, no cosmology (a = 1, H = 0),
+ * periodic box with >= 3 top-level cells per axis and kernel support
+ * H = gamma_K h <= top-level cell width.
+ */
+#ifndef SWSPH_API_H_
+#define SWSPH_API_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct swsph_engine swsph_engine;
+
+/* Run-time parameters read on this path: struct hydro_props
+ * (src/hydro_properties.h; defaults src/hydro_properties.c:348-390 and
+ * src/hydro/SPHENIX/hydro_parameters.h:59-101), the engine's time_base and
+ * max_active_bin (src/engine.h), and the space geometry (src/space.h). */
+typedef struct swsph_props {
+  double dim[3];          /* periodic box size */
+  int ncell[3];           /* top-level cells per axis */
+  float eta;              /* SPH:resolution_eta */
+  float h_tolerance;      /* SPH:h_tolerance */
+  float h_max, h_min;     /* SPH:h_max, h_min */
+  int max_iter;           /* SPH:max_ghost_iterations */
+  float alpha_max, alpha_min, visc_length;          /* viscosity */
+  float diff_beta, diff_alpha_max, diff_alpha_min;  /* diffusion */
+  double time_base;       /* dt of bin b = 2^(b+1) * time_base */
+  int max_active_bin;     /* part_is_active: time_bin <= max_active_bin */
+} swsph_props;
+
+/* Host particle arrays in the caller's order.  Vectors are [3n] interleaved
+ * like struct part.x[3] / v[3] (src/hydro/SPHENIX/hydro_part.h:105-324).
+ * The (io) fields are the state an inactive particle brings into the pass
+ * and are written back for active particles by swsph_get_field. */
+typedef struct swsph_host_parts {
+  int64_t n;
+  const double *x;
+  const float *v;
+  const float *m;
+  const float *u;
+  const int8_t *time_bin;
+  const float *h;           /* (io) smoothing length */
+  const float *rho;         /* (io) */
+  const float *pressure;    /* (io) force.pressure */
+  const float *soundspeed;  /* (io) force.soundspeed */
+  const float *f;           /* (io) force.f */
+  const float *balsara;     /* (io) force.balsara */
+  const float *alpha_visc;  /* (io) viscosity.alpha */
+  const float *alpha_diff;  /* (io) diffusion.alpha */
+  const float *div_v_prev;  /* (io) viscosity.div_v_previous_step */
+} swsph_host_parts;
+
+/* Field identifiers for swsph_get_field (host buffer in the caller's order;
+ * float unless noted, vectors [3n]). */
+enum swsph_field {
+  SWSPH_H = 0, SWSPH_RHO, SWSPH_RHO_DH, SWSPH_WCOUNT, SWSPH_WCOUNT_DH,
+  SWSPH_DIV_V, SWSPH_ROT_V, SWSPH_PRESSURE, SWSPH_SOUNDSPEED, SWSPH_F,
+  SWSPH_BALSARA, SWSPH_V_SIG, SWSPH_LAPLACE_U, SWSPH_ALPHA_VISC_MAX_NGB,
+  SWSPH_ALPHA_VISC, SWSPH_ALPHA_DIFF, SWSPH_DIV_V_DT, SWSPH_DIV_V_PREV,
+  SWSPH_A_HYDRO, SWSPH_U_DT, SWSPH_H_DT,
+  SWSPH_N_DENSITY,          /* int32: neighbours j != i, r < H_i */
+  SWSPH_N_GRADIENT,         /* int32 */
+  SWSPH_N_FORCE,            /* int32: neighbours j != i, r < max(H_i, H_j) */
+  SWSPH_MIN_NGB_TIME_BIN,   /* int8 */
+  SWSPH_NUM_FIELDS
+};
+
+typedef struct swsph_pass_stats {
+  int ghost_rounds;              /* density evaluations of the slowest particle */
+  int64_t n_density;             /* sum of N_density over active particles */
+  int64_t n_gradient;
+  int64_t n_force;
+  int64_t launches;              /* kernel launches issued by the pass */
+} swsph_pass_stats;
+
+const char *swsph_last_error(void);
+
+/* Engine life cycle. */
+int swsph_engine_create(const swsph_props *props, swsph_engine **out);
+int swsph_engine_destroy(swsph_engine *e);
+int swsph_set_props(swsph_engine *e, const swsph_props *props);
+
+/* Bins the particles into the top-level grid (counting sort by cell) and
+ * uploads them.  Replaces space_rebuild's particle sort for this path. */
+int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p);
+int swsph_get_field(swsph_engine *e, int field, void *host_out);
+int swsph_num_cells(swsph_engine *e, int *ncells);
+int swsph_cell_count(swsph_engine *e, int cid, int *count);
+int swsph_sync(swsph_engine *e);
+
+/* ---- drop-in task entry points ------------------------------------------
+ * Each replaces the named SWIFT function for one top-level cell (pair).  */
+
+/* hydro_init_part on the active particles of the cell (called from the
+ * drift/init in SWIFT, src/hydro/SPHENIX/hydro.h:564). */
+int swsph_init_cell(swsph_engine *e, int ci);
+/* runner_doself1_branch_density (src/runner_doiact_hydro.h:177). */
+int swsph_doself1_density(swsph_engine *e, int ci);
+/* runner_dopair1_branch_density (src/runner_doiact_hydro.h:181). */
+int swsph_dopair1_density(swsph_engine *e, int ci, int cj);
+/* runner_doself_subset_branch_density / runner_dopair_subset_branch_density
+ * (src/runner_doiact_hydro.h:197-210): density for the particles of ci that
+ * the ghost flagged for another round. */
+int swsph_doself_subset_density(swsph_engine *e, int ci);
+int swsph_dopair_subset_density(swsph_engine *e, int ci, int cj);
+/* runner_do_ghost (src/runner.h:94, src/runner_ghost.c:1113): iterates h to
+ * convergence with subset re-runs over the cell's 27 density interactions. */
+int swsph_do_ghost(swsph_engine *e, int ci);
+/* runner_doself1_branch_gradient / runner_dopair1_branch_gradient
+ * (src/runner_doiact_hydro.h:177-185, gradient instance). */
+int swsph_doself1_gradient(swsph_engine *e, int ci);
+int swsph_dopair1_gradient(swsph_engine *e, int ci, int cj);
+/* runner_do_extra_ghost (src/runner.h:96, src/runner_ghost.c:1016). */
+int swsph_do_extra_ghost(swsph_engine *e, int ci);
+/* runner_doself2_branch_force / runner_dopair2_branch_force
+ * (src/runner_doiact_hydro.h:187-195). */
+int swsph_doself2_force(swsph_engine *e, int ci);
+int swsph_dopair2_force(swsph_engine *e, int ci, int cj);
+/* runner_do_end_hydro_force (src/runner.h:110, src/runner_others.c:815). */
+int swsph_do_end_hydro_force(swsph_engine *e, int ci);
+
+/* ---- whole passes -------------------------------------------------------
+ * Plain: the C task loop of the reference's dependency order (all density
+ * tasks, ghosts, gradient tasks, extra ghosts, force tasks, end-force),
+ * one launch per task.  Batched: the same pass as a handful of launches over
+ * all cells.  Both leave identical neighbour sets. */
+int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats);
+int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats);
+
+/* Interaction counters summed over active particles (device reduction). */
+int swsph_count_interactions(swsph_engine *e, int64_t out3[3]);
+
+/* The HIP stream (a hipStream_t) every launch of the engine goes to. */
+int swsph_stream(swsph_engine *e, void **stream);
+/* Kernel timing with HIP events recorded on the engine stream around every
+ * neighbour-loop launch of the batched pass.  swsph_get_timing returns, per
+ * loop (0 density incl. ghost re-runs, 1 gradient, 2 force), the summed
+ * kernel time in ms and the number of launches since timing was enabled. */
+int swsph_set_timing(swsph_engine *e, int on);
+int swsph_get_timing(swsph_engine *e, double ms[3], int64_t launches[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

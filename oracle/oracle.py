@@ -1,0 +1,201 @@
+"""ctypes wrapper of the CPU oracle (liborc.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _lib.orc_kernel_gamma.restype = C.c_float
+        _lib.orc_kernel_gamma2.restype = C.c_float
+        _lib.orc_kernel_root.restype = C.c_float
+        _lib.orc_kernel_norm.restype = C.c_float
+        _lib.orc_kernel_deval32.argtypes = [C.c_float, C.POINTER(C.c_float),
+                                            C.POINTER(C.c_float)]
+        _lib.orc_kernel_deval64.argtypes = [C.c_double, C.POINTER(C.c_double),
+                                            C.POINTER(C.c_double)]
+        for fn in ("orc_iact_density32", "orc_iact_gradient32",
+                   "orc_iact_force32"):
+            getattr(_lib, fn).argtypes = [C.c_float, C.c_void_p, C.c_float,
+                                          C.c_float, C.c_void_p, C.c_void_p,
+                                          C.c_int]
+    return _lib
+
+
+class OrcConfig(C.Structure):
+    _fields_ = [("dim", C.c_double * 3), ("ncell", C.c_int * 3),
+                ("eta", C.c_float), ("h_tolerance", C.c_float),
+                ("h_max", C.c_float), ("h_min", C.c_float),
+                ("max_iter", C.c_int), ("alpha_max", C.c_float),
+                ("alpha_min", C.c_float), ("visc_length", C.c_float),
+                ("diff_beta", C.c_float), ("diff_alpha_max", C.c_float),
+                ("diff_alpha_min", C.c_float), ("time_base", C.c_double),
+                ("max_active_bin", C.c_int), ("nthreads", C.c_int)]
+
+
+_PART_IN = ["x", "v", "m", "u", "time_bin"]
+_PART_IO = ["h", "rho", "pressure", "soundspeed", "f", "balsara",
+            "alpha_visc", "alpha_diff", "div_v_prev"]
+
+
+class OrcParts(C.Structure):
+    _fields_ = [("n", C.c_int)] + [(k, C.c_void_p) for k in _PART_IN + _PART_IO]
+
+
+OUT64 = ["rho", "rho_dh", "wcount", "wcount_dh", "div_v", "rot_v",
+         "pressure", "soundspeed", "f", "balsara",
+         "v_sig", "laplace_u", "alpha_visc_max_ngb",
+         "alpha_visc", "alpha_diff", "div_v_dt",
+         "a_hydro", "u_dt", "h_dt",
+         "n_density", "n_gradient", "n_force", "min_ngb_time_bin",
+         "s_rho_dh", "s_wcount_dh", "s_div_v", "s_rot_v", "s_laplace_u",
+         "s_a_hydro", "s_u_dt", "s_h_dt"]
+OUT32 = ["rho_dh", "wcount", "wcount_dh", "div_v", "rot_v",
+         "v_sig", "laplace_u", "alpha_visc_max_ngb", "div_v_dt",
+         "a_hydro", "u_dt", "h_dt",
+         "n_density", "n_gradient", "n_force", "min_ngb_time_bin",
+         "ghost_iters"]
+VEC = {"rot_v", "a_hydro", "s_rot_v", "s_a_hydro"}
+
+
+class OrcOut64(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in OUT64]
+
+
+class OrcOut32(C.Structure):
+    _fields_ = [(k, C.c_void_p) for k in OUT32]
+
+
+def make_config(props, nthreads: int = 0) -> OrcConfig:
+    c = OrcConfig()
+    c.dim[:] = list(props.dim)
+    c.ncell[:] = list(props.ncell)
+    for k in ("eta", "h_tolerance", "h_max", "h_min", "max_iter", "alpha_max",
+              "alpha_min", "visc_length", "diff_beta", "diff_alpha_max",
+              "diff_alpha_min", "time_base", "max_active_bin"):
+        setattr(c, k, getattr(props, k))
+    c.nthreads = nthreads
+    return c
+
+
+def _parts(st: dict):
+    keep = {}
+    p = OrcParts()
+    p.n = st["h"].shape[0]
+    for k in _PART_IN + _PART_IO:
+        a = st[k]
+        assert a.flags["C_CONTIGUOUS"], k
+        keep[k] = a
+        setattr(p, k, a.ctypes.data)
+    return p, keep
+
+
+def _out_dtype(k: str, fp64: bool):
+    if k.startswith("n_") or k == "ghost_iters":
+        return np.int32
+    if k == "min_ngb_time_bin":
+        return np.int8
+    return np.float64 if fp64 else np.float32
+
+
+def pass_fp64(props, st: dict, nthreads: int = 0) -> dict:
+    """FP64 rerun of density -> gradient -> force at the h in ``st``."""
+    n = st["h"].shape[0]
+    out = {}
+    o = OrcOut64()
+    for k in OUT64:
+        shape = (n, 3) if k in VEC else (n,)
+        out[k] = np.zeros(shape, _out_dtype(k, True))
+        setattr(o, k, out[k].ctypes.data)
+    cfg = make_config(props, nthreads)
+    p, _keep = _parts(st)
+    rc = lib().orc_pass_fp64(C.byref(cfg), C.byref(p), C.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"orc_pass_fp64 failed ({rc})")
+    return out
+
+
+def pass_fp32(props, st: dict, nthreads: int = 0) -> dict:
+    """FP32 restatement of the full pass including the ghost; updates the
+    (io) fields of ``st`` in place (h converged) and returns the outputs."""
+    n = st["h"].shape[0]
+    out = {}
+    o = OrcOut32()
+    for k in OUT32:
+        shape = (n, 3) if k in VEC else ((1,) if k == "ghost_iters" else (n,))
+        out[k] = np.zeros(shape, _out_dtype(k, False))
+        setattr(o, k, out[k].ctypes.data)
+    cfg = make_config(props, nthreads)
+    p, _keep = _parts(st)
+    rc = lib().orc_pass_fp32(C.byref(cfg), C.byref(p), C.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"orc_pass_fp32 failed ({rc})")
+    for k in ("rho", "pressure", "soundspeed", "f", "balsara", "alpha_visc",
+              "alpha_diff", "h"):
+        out[k] = st[k].copy()
+    return out
+
+
+def count_bruteforce(props, st: dict, mode: int) -> np.ndarray:
+    n = st["h"].shape[0]
+    counts = np.zeros(n, np.int32)
+    cfg = make_config(props)
+    p, _keep = _parts(st)
+    lib().orc_count_bruteforce(C.byref(cfg), C.byref(p), mode,
+                               counts.ctypes.data_as(C.c_void_p))
+    return counts
+
+
+def kernel_deval32(u: float):
+    w, dw = C.c_float(), C.c_float()
+    lib().orc_kernel_deval32(u, C.byref(w), C.byref(dw))
+    return w.value, dw.value
+
+
+def kernel_deval64(u: float):
+    w, dw = C.c_double(), C.c_double()
+    lib().orc_kernel_deval64(u, C.byref(w), C.byref(dw))
+    return w.value, dw.value
+
+
+def constants() -> dict:
+    L = lib()
+    return {"kernel_gamma": L.orc_kernel_gamma(),
+            "kernel_gamma2": L.orc_kernel_gamma2(),
+            "kernel_root": L.orc_kernel_root(),
+            "kernel_norm": L.orc_kernel_norm()}
+
+
+PART_FIELDS = ["V0", "V1", "V2", "M", "H", "U", "RHO", "RHO_DH", "WC", "WC_DH",
+               "DIVV", "ROT0", "ROT1", "ROT2", "F", "P", "C", "BAL", "VSIG",
+               "LAPU", "AVMAX", "ALPHA", "ADIFF", "A0", "A1", "A2", "UDT", "HDT",
+               "NDEN", "NGRAD", "NFORCE", "TB", "MINTB"]
+
+
+def iact(kind: str, r2: float, dx, hi: float, hj: float, pi: np.ndarray,
+         pj: np.ndarray, symmetric: bool) -> None:
+    """Single FP32 pair interaction on flat float32 particle records."""
+    dxa = np.ascontiguousarray(dx, dtype=np.float32)
+    fn = getattr(lib(), f"orc_iact_{kind}32")
+    fn(r2, dxa.ctypes.data, hi, hj, pi.ctypes.data, pj.ctypes.data,
+       int(symmetric))

@@ -1,0 +1,90 @@
+// sph_device.h -- device-side SPHENIX physics for gfx950.
+//
+// Constants restate src/kernel_hydro.h:121-146 and 203-241 (Wendland C2, 3D),
+// src/dimension.h:40-41, src/adiabatic_index.h:41-49 (gamma = 5/3) and
+// src/hydro/SPHENIX/hydro_parameters.h:53 (beta_visc = 3).  The constant
+// expressions are evaluated exactly as the reference's macros are (float
+// products, double reciprocals rounded to float) so that kernel_gamma2,
+// kernel_root and the pair test agree bit-for-bit with the host.
+//
+// The per-pair bodies compute the same quantities as
+// src/hydro/SPHENIX/hydro_iact.h (runner_iact_nonsym_density :141,
+// runner_iact_nonsym_gradient :291, runner_iact_nonsym_force :507) but are
+// arranged for the VALU: the Wendland polynomial is evaluated in its factored
+// form (1-q)^4 (1+4q), one reciprocal square root gives both r and 1/r, and
+// every per-particle quotient (P/rho^2, 1/rho, 1/m, h^-4) is hoisted out of
+// the pair loop.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sph {
+
+// --- kernel_hydro.h constants (evaluated as the reference's macros are) ---
+constexpr float kGamma = (float)(1.936492);
+constexpr float kConst = (float)(21. * 0.31830988618379067154 / 2.);  // 21/(2 pi)
+constexpr float kGammaInv = (float)(1. / (double)kGamma);
+constexpr float kGamma2 = kGamma * kGamma;
+constexpr float kGammaInvDim = (float)(1. / (double)(kGamma * kGamma * kGamma));
+constexpr float kGammaInvDim1 =
+    (float)(1. / (double)(kGamma * kGamma * kGamma * kGamma));
+constexpr float kRoot = 1.f * kConst * kGammaInvDim;
+// Factors folded into the factored polynomial: W = cW (1-q)^4 (1+4q),
+// dW/du = cdW q (1-q)^3 with q = u / gamma.
+constexpr float cW = kConst * kGammaInvDim;
+constexpr float cdW = -20.f * kConst * kGammaInvDim1;
+
+constexpr float kDim = 3.f;
+constexpr float kDimInv = 0.3333333333f;
+constexpr float kGasGamma = 1.66666666666666667f;
+constexpr float kGasGm1 = 0.66666666666666667f;
+constexpr float kViscBeta = 3.0f;
+constexpr int kNumTimeBins = 56;
+constexpr int kBinInhibited = kNumTimeBins + 2;
+
+// Wendland C2 W(u) and dW/du for u = r/h (kernel_deval, kernel_hydro.h:257),
+// factored form; q >= 1 gives zero.
+__device__ __forceinline__ void kernel_deval(float u, float &w, float &dw) {
+  const float q = u * kGammaInv;
+  const float om = fmaxf(1.f - q, 0.f);
+  const float om2 = om * om;
+  const float om3 = om2 * om;
+  w = (om2 * om2) * fmaf(4.f * cW, q, cW);
+  dw = (q * om3) * cdW;
+}
+
+// Only the derivative (gradient loop).
+__device__ __forceinline__ float kernel_dw(float u) {
+  const float q = u * kGammaInv;
+  const float om = fmaxf(1.f - q, 0.f);
+  return (q * (om * om * om)) * cdW;
+}
+
+// The FP32 pair contract: dx = (float)((x_i - t) - x_j) per axis, computed
+// from FP64 positions, and r2 as two FMAs.  xs = x_i - t is formed once per
+// (i, neighbour cell) by the caller.
+__device__ __forceinline__ float pair_r2(double xs, double ys, double zs,
+                                         double xj, double yj, double zj,
+                                         float &dx, float &dy, float &dz) {
+  dx = (float)(xs - xj);
+  dy = (float)(ys - yj);
+  dz = (float)(zs - zj);
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
+
+__device__ __forceinline__ float search_radius2(float h) {
+  return (h * h) * kGamma2;
+}
+
+__device__ __forceinline__ float fast_rsq(float x) {
+  return __builtin_amdgcn_rsqf(x);
+}
+__device__ __forceinline__ float fast_rcp(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+__device__ __forceinline__ float fast_sqrt(float x) {
+  return __builtin_amdgcn_sqrtf(x);
+}
+
+}  // namespace sph

@@ -1,0 +1,785 @@
+// sph_engine.cpp -- host side of libswsph.so: the C ABI declared in
+// include/swsph.h.
+//
+// The engine keeps the gas particles resident in HBM as a structure of
+// arrays sorted by top-level cell and issues the kernels of sph_kernels.hip
+// on one HIP stream.  The per-task entry points are the drop-in for SWIFT's
+// runner task functions; swsph_pass_plain drives them in the reference's
+// dependency order (src/engine_maketasks.c:2546-2573: density -> ghost ->
+// gradient -> extra ghost -> force -> end force), swsph_pass_batched runs
+// the same pass as a few launches over all cells.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/swsph.h"
+#include "sph_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                       \
+  do {                                                                     \
+    hipError_t err_ = (call);                                              \
+    if (err_ != hipSuccess)                                                \
+      return fail(2, "%s failed: %s (%s:%d)", #call,                        \
+                  hipGetErrorString(err_), __FILE__, __LINE__);            \
+  } while (0)
+
+constexpr int kNumTimeBins = 56;
+constexpr float kGammaK = (float)(1.936492);
+
+template <class T>
+hipError_t dalloc(T **p, size_t n) {
+  return hipMalloc((void **)p, sizeof(T) * (n ? n : 1));
+}
+
+}  // namespace
+
+struct swsph_engine {
+  swsph_props props{};
+  GridDesc grid{};
+  int ncells = 0;
+  int64_t n = 0;
+  std::vector<int> cell_start;    // host copy [ncells+1]
+  std::vector<int64_t> perm;      // sorted position -> caller index
+  std::vector<int4> tiles_h;
+  std::vector<int> cell_tiles;    // tiles per cell
+  DevParts d{};
+  int *d_cell_start = nullptr;
+  int4 *d_tiles = nullptr;
+  int ntiles = 0;
+  int *d_redo = nullptr;          // redo counter
+  int *h_redo = nullptr;          // pinned readback
+  unsigned long long *d_cnt = nullptr;
+  unsigned long long *h_cnt = nullptr;
+  hipStream_t st = nullptr;
+  std::mutex mtx;
+  bool timing = false;
+  double t_ms[3] = {0, 0, 0};
+  int64_t t_launch[3] = {0, 0, 0};
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  size_t ev_used = 0;
+  int64_t launches = 0;
+};
+
+namespace {
+
+LoopArgs loop_args(const swsph_engine *e) {
+  LoopArgs la;
+  la.grid = e->grid;
+  la.cell_start = e->d_cell_start;
+  la.max_active_bin = e->props.max_active_bin;
+  return la;
+}
+
+GhostArgs ghost_args(const swsph_engine *e) {
+  GhostArgs g;
+  const float eta = e->props.eta;
+  g.eta_dim = eta * eta * eta;
+  g.eps = e->props.h_tolerance;
+  g.h_max = e->props.h_max;
+  g.h_min = e->props.h_min;
+  g.max_active_bin = e->props.max_active_bin;
+  g.time_base = e->props.time_base;
+  g.alpha_max = e->props.alpha_max;
+  g.alpha_min = e->props.alpha_min;
+  g.visc_length = e->props.visc_length;
+  g.diff_beta = e->props.diff_beta;
+  g.diff_alpha_max = e->props.diff_alpha_max;
+  g.diff_alpha_min = e->props.diff_alpha_min;
+  return g;
+}
+
+void free_dev(swsph_engine *e) {
+  void *ptrs[] = {e->d.px, e->d.py, e->d.pz, e->d.vx, e->d.vy, e->d.vz,
+                  e->d.m, e->d.u, e->d.h, e->d.tb, e->d.rho, e->d.rho_dh,
+                  e->d.wc, e->d.wc_dh, e->d.divv, e->d.rotx, e->d.roty,
+                  e->d.rotz, e->d.nd, e->d.left, e->d.right, e->d.redo,
+                  e->d.P, e->d.c, e->d.f, e->d.bal, e->d.vsig, e->d.lapu,
+                  e->d.avmax, e->d.ng, e->d.alpha, e->d.adiff, e->d.divv_prev,
+                  e->d.divv_dt, e->d.ax, e->d.ay, e->d.az, e->d.udt, e->d.hdt,
+                  e->d.nf, e->d.mintb, e->d_cell_start, e->d_tiles};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  e->d = DevParts{};
+  e->d_cell_start = nullptr;
+  e->d_tiles = nullptr;
+}
+
+int alloc_dev(swsph_engine *e, size_t n) {
+  DevParts &d = e->d;
+  HIPCHK(dalloc(&d.px, n));
+  HIPCHK(dalloc(&d.py, n));
+  HIPCHK(dalloc(&d.pz, n));
+  float **f32[] = {&d.vx, &d.vy, &d.vz, &d.m, &d.u, &d.h, &d.rho, &d.rho_dh,
+                   &d.wc, &d.wc_dh, &d.divv, &d.rotx, &d.roty, &d.rotz,
+                   &d.left, &d.right, &d.P, &d.c, &d.f, &d.bal, &d.vsig,
+                   &d.lapu, &d.avmax, &d.alpha, &d.adiff, &d.divv_prev,
+                   &d.divv_dt, &d.ax, &d.ay, &d.az, &d.udt, &d.hdt};
+  for (float **p : f32) HIPCHK(dalloc(p, n));
+  int **i32[] = {&d.nd, &d.redo, &d.ng, &d.nf};
+  for (int **p : i32) HIPCHK(dalloc(p, n));
+  HIPCHK(dalloc(&d.tb, n));
+  HIPCHK(dalloc(&d.mintb, n));
+  // Zero everything once so that fields never written for inactive
+  // particles read as zero.
+  const size_t nn = n ? n : 1;
+  for (float **p : f32) HIPCHK(hipMemsetAsync(*p, 0, sizeof(float) * nn, e->st));
+  for (int **p : i32) HIPCHK(hipMemsetAsync(*p, 0, sizeof(int) * nn, e->st));
+  HIPCHK(hipMemsetAsync(d.mintb, 0, nn, e->st));
+  // The uploads that follow use blocking copies on the null stream, which
+  // does not order against this non-blocking stream.
+  HIPCHK(hipStreamSynchronize(e->st));
+  return 0;
+}
+
+int check_props(const swsph_props *p) {
+  for (int k = 0; k < 3; k++) {
+    if (p->ncell[k] < 3)
+      return fail(1, "need >= 3 top-level cells per axis (got %d)", p->ncell[k]);
+    if (!(p->dim[k] > 0.0)) return fail(1, "box size must be positive");
+  }
+  if (p->max_iter <= 0) return fail(1, "max_iter must be positive");
+  if (!(p->eta > 0.f)) return fail(1, "eta must be positive");
+  return 0;
+}
+
+bool adjacent(const swsph_engine *e, int ci, int cj, double t[3]) {
+  const int nc[3] = {e->grid.nc[0], e->grid.nc[1], e->grid.nc[2]};
+  const int a[3] = {ci / (nc[1] * nc[2]), (ci / nc[2]) % nc[1], ci % nc[2]};
+  const int b[3] = {cj / (nc[1] * nc[2]), (cj / nc[2]) % nc[1], cj % nc[2]};
+  for (int k = 0; k < 3; k++) {
+    int dd = b[k] - a[k];
+    t[k] = 0.0;
+    if (dd > nc[k] / 2) {
+      dd -= nc[k];
+      t[k] = -e->grid.dim[k];
+    } else if (dd < -(nc[k] / 2)) {
+      dd += nc[k];
+      t[k] = e->grid.dim[k];
+    }
+    if (dd < -1 || dd > 1) return false;
+  }
+  return true;
+}
+
+int cell_ok(const swsph_engine *e, int c) {
+  if (e->n == 0 && e->cell_start.empty()) return fail(1, "no particles set");
+  if (c < 0 || c >= e->ncells) return fail(1, "cell index %d out of range", c);
+  return 0;
+}
+
+int tiles_of(const swsph_engine *e, int c) { return e->cell_tiles[c]; }
+
+hipEvent_t next_event(swsph_engine *e) {
+  if (e->ev_used == e->ev_pool.size()) {
+    hipEvent_t ev;
+    if (hipEventCreate(&ev) != hipSuccess) return nullptr;
+    e->ev_pool.push_back(ev);
+  }
+  return e->ev_pool[e->ev_used++];
+}
+
+// Launch a batched loop, bracketed by timing events when enabled.
+int stencil(swsph_engine *e, LoopKind k, bool subset) {
+  hipEvent_t a = nullptr, b = nullptr;
+  if (e->timing) {
+    a = next_event(e);
+    b = next_event(e);
+    if (!a || !b) return fail(2, "hipEventCreate failed");
+    HIPCHK(hipEventRecord(a, e->st));
+  }
+  HIPCHK(sphk::launch_stencil(k, subset, e->d, loop_args(e), e->d_tiles,
+                              e->ntiles, e->st));
+  e->launches++;
+  if (e->timing) {
+    HIPCHK(hipEventRecord(b, e->st));
+    e->ev_pending.push_back({(int)k, {a, b}});
+  }
+  return 0;
+}
+
+int collect_timing(swsph_engine *e) {
+  if (!e->timing) return 0;
+  HIPCHK(hipStreamSynchronize(e->st));
+  for (auto &p : e->ev_pending) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, p.second.first, p.second.second));
+    e->t_ms[p.first] += ms;
+    e->t_launch[p.first] += 1;
+  }
+  e->ev_pending.clear();
+  e->ev_used = 0;
+  return 0;
+}
+
+int task(swsph_engine *e, LoopKind k, bool subset, int ci, int cj, bool both) {
+  int rc;
+  if ((rc = cell_ok(e, ci)) || (rc = cell_ok(e, cj))) return rc;
+  TaskArgs ta{};
+  double t[3];
+  if (!adjacent(e, ci, cj, t))
+    return fail(1, "cells %d and %d are not neighbours", ci, cj);
+  ta.icell[0] = ci;
+  ta.jcell[0] = cj;
+  ta.ntiles[0] = tiles_of(e, ci);
+  for (int k2 = 0; k2 < 3; k2++) ta.t[k2] = t[k2];
+  if (both && ci != cj) {
+    ta.icell[1] = cj;
+    ta.jcell[1] = ci;
+    ta.ntiles[1] = tiles_of(e, cj);
+    for (int k2 = 0; k2 < 3; k2++) ta.t[3 + k2] = -t[k2];
+  }
+  if (ta.ntiles[0] + ta.ntiles[1] == 0) return 0;
+  HIPCHK(sphk::launch_task(k, subset, e->d, loop_args(e), ta, e->st));
+  e->launches++;
+  return 0;
+}
+
+// The 13 "upper" neighbours of a cell: offsets whose first non-zero
+// component is positive, so that every adjacent pair is visited once.
+void upper_neighbours(const swsph_engine *e, int ci, std::vector<int> &out) {
+  out.clear();
+  const int *nc = e->grid.nc;
+  const int a[3] = {ci / (nc[1] * nc[2]), (ci / nc[2]) % nc[1], ci % nc[2]};
+  for (int o = 14; o < 27; o++) {
+    const int ox = o / 9 - 1, oy = (o / 3) % 3 - 1, oz = o % 3 - 1;
+    const int b[3] = {(a[0] + ox + nc[0]) % nc[0], (a[1] + oy + nc[1]) % nc[1],
+                      (a[2] + oz + nc[2]) % nc[2]};
+    out.push_back((b[0] * nc[1] + b[1]) * nc[2] + b[2]);
+  }
+}
+
+void all_neighbours(const swsph_engine *e, int ci, std::vector<int> &out) {
+  out.clear();
+  const int *nc = e->grid.nc;
+  const int a[3] = {ci / (nc[1] * nc[2]), (ci / nc[2]) % nc[1], ci % nc[2]};
+  for (int o = 0; o < 27; o++) {
+    if (o == 13) continue;
+    const int ox = o / 9 - 1, oy = (o / 3) % 3 - 1, oz = o % 3 - 1;
+    const int b[3] = {(a[0] + ox + nc[0]) % nc[0], (a[1] + oy + nc[1]) % nc[1],
+                      (a[2] + oz + nc[2]) % nc[2]};
+    out.push_back((b[0] * nc[1] + b[1]) * nc[2] + b[2]);
+  }
+}
+
+int read_redo(swsph_engine *e, int *count) {
+  HIPCHK(hipMemcpyAsync(e->h_redo, e->d_redo, sizeof(int),
+                        hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  *count = *e->h_redo;
+  return 0;
+}
+
+int ghost_cell(swsph_engine *e, int ci, int *rounds) {
+  const int lo = e->cell_start[ci], hi = e->cell_start[ci + 1];
+  const GhostArgs ga = ghost_args(e);
+  std::vector<int> nb;
+  all_neighbours(e, ci, nb);
+  int redo = 0;
+  int it = 0;
+  for (; it < e->props.max_iter; it++) {
+    HIPCHK(sphk::launch_ghost(e->d, ga, lo, hi, e->d_redo, e->st));
+    e->launches += 2;
+    int rc = read_redo(e, &redo);
+    if (rc) return rc;
+    if (redo == 0) break;
+    // Subset re-runs over every density interaction of the cell
+    // (runner_ghost.c:1548-1580).
+    if ((rc = task(e, LoopKind::Density, true, ci, ci, false))) return rc;
+    for (int cj : nb)
+      if ((rc = task(e, LoopKind::Density, true, ci, cj, false))) return rc;
+  }
+  if (redo > 0)
+    return fail(3, "Smoothing length failed to converge on %d particles", redo);
+  if (rounds) *rounds = std::max(*rounds, it + 1);
+  return 0;
+}
+
+int count_into(swsph_engine *e, swsph_pass_stats *s) {
+  HIPCHK(sphk::launch_count(e->d, (int)e->n, e->props.max_active_bin, e->d_cnt,
+                            e->st));
+  HIPCHK(hipMemcpyAsync(e->h_cnt, e->d_cnt, 3 * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  if (s) {
+    s->n_density = (int64_t)e->h_cnt[0];
+    s->n_gradient = (int64_t)e->h_cnt[1];
+    s->n_force = (int64_t)e->h_cnt[2];
+  }
+  return 0;
+}
+
+int check_h_fits(swsph_engine *e) {
+  // Kernel support must fit in a top-level cell (27-cell stencil).
+  std::vector<float> h(e->n);
+  if (e->n) {
+    HIPCHK(hipMemcpy(h.data(), e->d.h, sizeof(float) * e->n,
+                     hipMemcpyDeviceToHost));
+  }
+  float hmax = 0.f;
+  for (float v : h) hmax = std::max(hmax, v);
+  const double H = (double)hmax * (double)kGammaK;
+  for (int k = 0; k < 3; k++) {
+    const double w = e->grid.dim[k] / e->grid.nc[k];
+    if (H > w)
+      return fail(4,
+                  "kernel support %g exceeds the top-level cell width %g: "
+                  "regrid with fewer cells",
+                  H, w);
+  }
+  return 0;
+}
+
+}  // namespace
+
+// ========================================================================= //
+extern "C" {
+
+const char *swsph_last_error(void) { return g_err.c_str(); }
+
+int swsph_engine_create(const swsph_props *props, swsph_engine **out) {
+  if (!props || !out) return fail(1, "null argument");
+  int rc = check_props(props);
+  if (rc) return rc;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return fail(5, "no HIP device available");
+  swsph_engine *e = new swsph_engine();
+  e->props = *props;
+  for (int k = 0; k < 3; k++) {
+    e->grid.nc[k] = props->ncell[k];
+    e->grid.dim[k] = props->dim[k];
+  }
+  e->ncells = props->ncell[0] * props->ncell[1] * props->ncell[2];
+  if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) {
+    delete e;
+    return fail(2, "hipStreamCreate failed");
+  }
+  if (hipMalloc((void **)&e->d_redo, sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void **)&e->h_redo, sizeof(int), 0) != hipSuccess ||
+      hipMalloc((void **)&e->d_cnt, 3 * sizeof(unsigned long long)) != hipSuccess ||
+      hipHostMalloc((void **)&e->h_cnt, 3 * sizeof(unsigned long long), 0) !=
+          hipSuccess) {
+    delete e;
+    return fail(2, "allocation of engine counters failed");
+  }
+  *out = e;
+  return 0;
+}
+
+int swsph_engine_destroy(swsph_engine *e) {
+  if (!e) return 0;
+  (void)hipStreamSynchronize(e->st);
+  free_dev(e);
+  for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
+  if (e->d_redo) (void)hipFree(e->d_redo);
+  if (e->h_redo) (void)hipHostFree(e->h_redo);
+  if (e->d_cnt) (void)hipFree(e->d_cnt);
+  if (e->h_cnt) (void)hipHostFree(e->h_cnt);
+  if (e->st) (void)hipStreamDestroy(e->st);
+  delete e;
+  return 0;
+}
+
+int swsph_set_props(swsph_engine *e, const swsph_props *props) {
+  if (!e || !props) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  int rc = check_props(props);
+  if (rc) return rc;
+  for (int k = 0; k < 3; k++)
+    if (props->ncell[k] != e->props.ncell[k] || props->dim[k] != e->props.dim[k])
+      return fail(1, "grid changes need swsph_set_particles on a new engine");
+  e->props = *props;
+  return 0;
+}
+
+int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
+  if (!e || !p) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  const int64_t n = p->n;
+  if (n < 0 || n > (int64_t)0x7fffffff - 64)
+    return fail(1, "particle count %lld out of range", (long long)n);
+  if (!p->x || !p->v || !p->m || !p->u || !p->time_bin || !p->h)
+    return fail(1, "missing particle arrays");
+  HIPCHK(hipStreamSynchronize(e->st));
+  free_dev(e);
+  e->n = n;
+  int rc = alloc_dev(e, (size_t)n);
+  if (rc) return rc;
+
+  // Counting sort by top-level cell (the binning rule of the oracle).
+  const int *nc = e->grid.nc;
+  double inv_w[3];
+  for (int k = 0; k < 3; k++) inv_w[k] = (double)nc[k] / e->grid.dim[k];
+  std::vector<int> cid(n);
+  e->cell_start.assign(e->ncells + 1, 0);
+  for (int64_t i = 0; i < n; i++) {
+    int c[3];
+    for (int k = 0; k < 3; k++) {
+      int v = (int)std::floor(p->x[3 * i + k] * inv_w[k]);
+      c[k] = v < 0 ? 0 : (v >= nc[k] ? nc[k] - 1 : v);
+    }
+    cid[i] = (c[0] * nc[1] + c[1]) * nc[2] + c[2];
+    e->cell_start[cid[i] + 1]++;
+  }
+  for (int c = 0; c < e->ncells; c++) e->cell_start[c + 1] += e->cell_start[c];
+  std::vector<int> fill(e->cell_start.begin(), e->cell_start.end() - 1);
+  e->perm.assign(n, 0);
+  for (int64_t i = 0; i < n; i++) e->perm[fill[cid[i]]++] = i;
+
+  // Tiles of <= 64 consecutive particles of one cell.
+  e->tiles_h.clear();
+  e->cell_tiles.assign(e->ncells, 0);
+  for (int c = 0; c < e->ncells; c++) {
+    const int lo = e->cell_start[c], hi = e->cell_start[c + 1];
+    for (int f = lo; f < hi; f += 64) {
+      e->tiles_h.push_back(make_int4(c, f, std::min(64, hi - f), 0));
+      e->cell_tiles[c]++;
+    }
+  }
+  e->ntiles = (int)e->tiles_h.size();
+
+  // Gather to sorted SoA and upload.
+  std::vector<double> xd(n);
+  std::vector<float> fd(n);
+  std::vector<int8_t> bd(n);
+  auto up_d = [&](double *dst, int k) -> int {
+    for (int64_t s = 0; s < n; s++) xd[s] = p->x[3 * e->perm[s] + k];
+    HIPCHK(hipMemcpy(dst, xd.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    return 0;
+  };
+  auto up_f = [&](float *dst, const float *src, int stride, int k,
+                  float dflt) -> int {
+    for (int64_t s = 0; s < n; s++)
+      fd[s] = src ? src[stride * e->perm[s] + k] : dflt;
+    HIPCHK(hipMemcpy(dst, fd.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+    return 0;
+  };
+  if (n > 0) {
+    if ((rc = up_d(e->d.px, 0)) || (rc = up_d(e->d.py, 1)) ||
+        (rc = up_d(e->d.pz, 2)))
+      return rc;
+    if ((rc = up_f(e->d.vx, p->v, 3, 0, 0.f)) ||
+        (rc = up_f(e->d.vy, p->v, 3, 1, 0.f)) ||
+        (rc = up_f(e->d.vz, p->v, 3, 2, 0.f)) ||
+        (rc = up_f(e->d.m, p->m, 1, 0, 0.f)) ||
+        (rc = up_f(e->d.u, p->u, 1, 0, 0.f)) ||
+        (rc = up_f(e->d.h, p->h, 1, 0, 0.f)) ||
+        (rc = up_f(e->d.rho, p->rho, 1, 0, 0.f)) ||
+        (rc = up_f(e->d.P, p->pressure, 1, 0, 0.f)) ||
+        (rc = up_f(e->d.c, p->soundspeed, 1, 0, 0.f)) ||
+        (rc = up_f(e->d.f, p->f, 1, 0, 0.f)) ||
+        (rc = up_f(e->d.bal, p->balsara, 1, 0, 0.f)) ||
+        (rc = up_f(e->d.alpha, p->alpha_visc, 1, 0, 0.1f)) ||
+        (rc = up_f(e->d.adiff, p->alpha_diff, 1, 0, 0.f)) ||
+        (rc = up_f(e->d.divv_prev, p->div_v_prev, 1, 0, 0.f)))
+      return rc;
+    for (int64_t s = 0; s < n; s++) bd[s] = p->time_bin[e->perm[s]];
+    HIPCHK(hipMemcpy(e->d.tb, bd.data(), n, hipMemcpyHostToDevice));
+  }
+  HIPCHK(dalloc(&e->d_cell_start, (size_t)e->ncells + 1));
+  HIPCHK(hipMemcpy(e->d_cell_start, e->cell_start.data(),
+                   sizeof(int) * (e->ncells + 1), hipMemcpyHostToDevice));
+  HIPCHK(dalloc(&e->d_tiles, (size_t)std::max(1, e->ntiles)));
+  if (e->ntiles)
+    HIPCHK(hipMemcpy(e->d_tiles, e->tiles_h.data(), sizeof(int4) * e->ntiles,
+                     hipMemcpyHostToDevice));
+  return check_h_fits(e);
+}
+
+int swsph_get_field(swsph_engine *e, int field, void *out) {
+  if (!e || !out) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  const int64_t n = e->n;
+  HIPCHK(hipStreamSynchronize(e->st));
+  const DevParts &d = e->d;
+  auto get_f = [&](const float *src, int stride, int k) -> int {
+    std::vector<float> t(n);
+    if (n) HIPCHK(hipMemcpy(t.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost));
+    float *o = (float *)out;
+    for (int64_t s = 0; s < n; s++) o[stride * e->perm[s] + k] = t[s];
+    return 0;
+  };
+  auto get_i = [&](const int *src) -> int {
+    std::vector<int> t(n);
+    if (n) HIPCHK(hipMemcpy(t.data(), src, sizeof(int) * n, hipMemcpyDeviceToHost));
+    int32_t *o = (int32_t *)out;
+    for (int64_t s = 0; s < n; s++) o[e->perm[s]] = t[s];
+    return 0;
+  };
+  int rc = 0;
+  switch (field) {
+    case SWSPH_H: return get_f(d.h, 1, 0);
+    case SWSPH_RHO: return get_f(d.rho, 1, 0);
+    case SWSPH_RHO_DH: return get_f(d.rho_dh, 1, 0);
+    case SWSPH_WCOUNT: return get_f(d.wc, 1, 0);
+    case SWSPH_WCOUNT_DH: return get_f(d.wc_dh, 1, 0);
+    case SWSPH_DIV_V: return get_f(d.divv, 1, 0);
+    case SWSPH_ROT_V:
+      if ((rc = get_f(d.rotx, 3, 0)) || (rc = get_f(d.roty, 3, 1))) return rc;
+      return get_f(d.rotz, 3, 2);
+    case SWSPH_PRESSURE: return get_f(d.P, 1, 0);
+    case SWSPH_SOUNDSPEED: return get_f(d.c, 1, 0);
+    case SWSPH_F: return get_f(d.f, 1, 0);
+    case SWSPH_BALSARA: return get_f(d.bal, 1, 0);
+    case SWSPH_V_SIG: return get_f(d.vsig, 1, 0);
+    case SWSPH_LAPLACE_U: return get_f(d.lapu, 1, 0);
+    case SWSPH_ALPHA_VISC_MAX_NGB: return get_f(d.avmax, 1, 0);
+    case SWSPH_ALPHA_VISC: return get_f(d.alpha, 1, 0);
+    case SWSPH_ALPHA_DIFF: return get_f(d.adiff, 1, 0);
+    case SWSPH_DIV_V_DT: return get_f(d.divv_dt, 1, 0);
+    case SWSPH_DIV_V_PREV: return get_f(d.divv_prev, 1, 0);
+    case SWSPH_A_HYDRO:
+      if ((rc = get_f(d.ax, 3, 0)) || (rc = get_f(d.ay, 3, 1))) return rc;
+      return get_f(d.az, 3, 2);
+    case SWSPH_U_DT: return get_f(d.udt, 1, 0);
+    case SWSPH_H_DT: return get_f(d.hdt, 1, 0);
+    case SWSPH_N_DENSITY: return get_i(d.nd);
+    case SWSPH_N_GRADIENT: return get_i(d.ng);
+    case SWSPH_N_FORCE: return get_i(d.nf);
+    case SWSPH_MIN_NGB_TIME_BIN: {
+      std::vector<int8_t> t(n);
+      if (n) HIPCHK(hipMemcpy(t.data(), d.mintb, n, hipMemcpyDeviceToHost));
+      int8_t *o = (int8_t *)out;
+      for (int64_t s = 0; s < n; s++) o[e->perm[s]] = t[s];
+      return 0;
+    }
+    default:
+      return fail(1, "unknown field %d", field);
+  }
+}
+
+int swsph_num_cells(swsph_engine *e, int *ncells) {
+  if (!e || !ncells) return fail(1, "null argument");
+  *ncells = e->ncells;
+  return 0;
+}
+
+int swsph_cell_count(swsph_engine *e, int cid, int *count) {
+  if (!e || !count) return fail(1, "null argument");
+  int rc = cell_ok(e, cid);
+  if (rc) return rc;
+  *count = e->cell_start[cid + 1] - e->cell_start[cid];
+  return 0;
+}
+
+int swsph_sync(swsph_engine *e) {
+  if (!e) return fail(1, "null argument");
+  HIPCHK(hipStreamSynchronize(e->st));
+  return 0;
+}
+
+int swsph_stream(swsph_engine *e, void **stream) {
+  if (!e || !stream) return fail(1, "null argument");
+  *stream = (void *)e->st;
+  return 0;
+}
+
+// ---- per-task entry points ------------------------------------------------
+int swsph_init_cell(swsph_engine *e, int ci) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  int rc = cell_ok(e, ci);
+  if (rc) return rc;
+  HIPCHK(sphk::launch_init_density(e->d, e->cell_start[ci],
+                                   e->cell_start[ci + 1],
+                                   e->props.max_active_bin, e->props.h_max,
+                                   e->st));
+  e->launches++;
+  return 0;
+}
+
+#define TASK_ENTRY(name, kind, subset, both)                        \
+  int name(swsph_engine *e, int ci, int cj) {                       \
+    if (!e) return fail(1, "null argument");                        \
+    std::lock_guard<std::mutex> lk(e->mtx);                         \
+    return task(e, kind, subset, ci, cj, both);                     \
+  }
+#define SELF_ENTRY(name, kind, subset)                              \
+  int name(swsph_engine *e, int ci) {                               \
+    if (!e) return fail(1, "null argument");                        \
+    std::lock_guard<std::mutex> lk(e->mtx);                         \
+    return task(e, kind, subset, ci, ci, false);                    \
+  }
+
+SELF_ENTRY(swsph_doself1_density, LoopKind::Density, false)
+TASK_ENTRY(swsph_dopair1_density, LoopKind::Density, false, true)
+SELF_ENTRY(swsph_doself_subset_density, LoopKind::Density, true)
+TASK_ENTRY(swsph_dopair_subset_density, LoopKind::Density, true, false)
+SELF_ENTRY(swsph_doself1_gradient, LoopKind::Gradient, false)
+TASK_ENTRY(swsph_dopair1_gradient, LoopKind::Gradient, false, true)
+SELF_ENTRY(swsph_doself2_force, LoopKind::Force, false)
+TASK_ENTRY(swsph_dopair2_force, LoopKind::Force, false, true)
+
+int swsph_do_ghost(swsph_engine *e, int ci) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  int rc = cell_ok(e, ci);
+  if (rc) return rc;
+  int rounds = 0;
+  return ghost_cell(e, ci, &rounds);
+}
+
+int swsph_do_extra_ghost(swsph_engine *e, int ci) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  int rc = cell_ok(e, ci);
+  if (rc) return rc;
+  HIPCHK(sphk::launch_extra_ghost(e->d, ghost_args(e), e->cell_start[ci],
+                                  e->cell_start[ci + 1], e->st));
+  e->launches++;
+  return 0;
+}
+
+int swsph_do_end_hydro_force(swsph_engine *e, int ci) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  int rc = cell_ok(e, ci);
+  if (rc) return rc;
+  HIPCHK(sphk::launch_end_force(e->d, e->cell_start[ci], e->cell_start[ci + 1],
+                                e->props.max_active_bin, e->st));
+  e->launches++;
+  return 0;
+}
+
+// ---- whole passes -----------------------------------------------------------
+int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  int rc = 0;
+  const int64_t l0 = e->launches;
+  std::vector<int> up;
+  const GhostArgs ga = ghost_args(e);
+  for (int c = 0; c < e->ncells; c++) {
+    HIPCHK(sphk::launch_init_density(e->d, e->cell_start[c], e->cell_start[c + 1],
+                                     e->props.max_active_bin, e->props.h_max,
+                                     e->st));
+    e->launches++;
+  }
+  auto loop_tasks = [&](LoopKind k) -> int {
+    int r;
+    for (int c = 0; c < e->ncells; c++) {
+      if ((r = task(e, k, false, c, c, false))) return r;
+      upper_neighbours(e, c, up);
+      for (int cj : up)
+        if ((r = task(e, k, false, c, cj, true))) return r;
+    }
+    return 0;
+  };
+  if ((rc = loop_tasks(LoopKind::Density))) return rc;
+  int rounds = 0;
+  for (int c = 0; c < e->ncells; c++)
+    if ((rc = ghost_cell(e, c, &rounds))) return rc;
+  if ((rc = loop_tasks(LoopKind::Gradient))) return rc;
+  for (int c = 0; c < e->ncells; c++) {
+    HIPCHK(sphk::launch_extra_ghost(e->d, ga, e->cell_start[c],
+                                    e->cell_start[c + 1], e->st));
+    e->launches++;
+  }
+  if ((rc = loop_tasks(LoopKind::Force))) return rc;
+  for (int c = 0; c < e->ncells; c++) {
+    HIPCHK(sphk::launch_end_force(e->d, e->cell_start[c], e->cell_start[c + 1],
+                                  e->props.max_active_bin, e->st));
+    e->launches++;
+  }
+  if (stats) {
+    stats->ghost_rounds = rounds;
+    stats->launches = e->launches - l0;
+  }
+  return count_into(e, stats);
+}
+
+int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  int rc = 0;
+  const int64_t l0 = e->launches;
+  const int n = (int)e->n;
+  const GhostArgs ga = ghost_args(e);
+  HIPCHK(sphk::launch_init_density(e->d, 0, n, e->props.max_active_bin,
+                                   e->props.h_max, e->st));
+  e->launches++;
+  if ((rc = stencil(e, LoopKind::Density, false))) return rc;
+  int redo = 0, it = 0;
+  for (; it < e->props.max_iter; it++) {
+    HIPCHK(sphk::launch_ghost(e->d, ga, 0, n, e->d_redo, e->st));
+    e->launches += 2;
+    if ((rc = read_redo(e, &redo))) return rc;
+    if (redo == 0) break;
+    if ((rc = stencil(e, LoopKind::Density, true))) return rc;
+  }
+  if (redo > 0)
+    return fail(3, "Smoothing length failed to converge on %d particles", redo);
+  if ((rc = stencil(e, LoopKind::Gradient, false))) return rc;
+  HIPCHK(sphk::launch_extra_ghost(e->d, ga, 0, n, e->st));
+  e->launches++;
+  if ((rc = stencil(e, LoopKind::Force, false))) return rc;
+  HIPCHK(sphk::launch_end_force(e->d, 0, n, e->props.max_active_bin, e->st));
+  e->launches++;
+  if (stats) {
+    stats->ghost_rounds = it + 1;
+    stats->launches = e->launches - l0;
+  }
+  if ((rc = count_into(e, stats))) return rc;
+  return collect_timing(e);
+}
+
+int swsph_count_interactions(swsph_engine *e, int64_t out3[3]) {
+  if (!e || !out3) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  swsph_pass_stats s{};
+  int rc = count_into(e, &s);
+  if (rc) return rc;
+  out3[0] = s.n_density;
+  out3[1] = s.n_gradient;
+  out3[2] = s.n_force;
+  return 0;
+}
+
+int swsph_set_timing(swsph_engine *e, int on) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  e->timing = on != 0;
+  for (int k = 0; k < 3; k++) {
+    e->t_ms[k] = 0.0;
+    e->t_launch[k] = 0;
+  }
+  e->ev_pending.clear();
+  e->ev_used = 0;
+  return 0;
+}
+
+int swsph_get_timing(swsph_engine *e, double ms[3], int64_t launches[3]) {
+  if (!e || !ms || !launches) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  int rc = collect_timing(e);
+  if (rc) return rc;
+  for (int k = 0; k < 3; k++) {
+    ms[k] = e->t_ms[k];
+    launches[k] = e->t_launch[k];
+  }
+  return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,77 @@
+// sph_kernels.h -- device data layout and launchers shared by the engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+// Structure-of-arrays particle store in HBM, sorted by top-level cell.
+struct DevParts {
+  double *px, *py, *pz;
+  float *vx, *vy, *vz, *m, *u, *h;
+  int8_t *tb;
+  // density sums (raw until the ghost finalises them)
+  float *rho, *rho_dh, *wc, *wc_dh, *divv, *rotx, *roty, *rotz;
+  int *nd;
+  // ghost state
+  float *left, *right;
+  int *redo;
+  // prepared by the ghost (hydro_prepare_gradient / hydro_reset_gradient)
+  float *P, *c, *f, *bal;
+  // gradient sums
+  float *vsig, *lapu, *avmax;
+  int *ng;
+  // viscosity / diffusion state (hydro_prepare_force)
+  float *alpha, *adiff, *divv_prev, *divv_dt;
+  // force sums
+  float *ax, *ay, *az, *udt, *hdt;
+  int *nf;
+  int8_t *mintb;
+};
+
+struct GridDesc {
+  int nc[3];
+  double dim[3];
+};
+
+struct LoopArgs {
+  GridDesc grid;
+  const int *cell_start;  // [ncells + 1]
+  int max_active_bin;
+};
+
+// Up to two directed cell interactions (i-cell gathers from j-cell, the
+// image of the j-cell being translated by t).
+struct TaskArgs {
+  int icell[2];
+  int jcell[2];
+  int ntiles[2];
+  double t[6];
+};
+
+struct GhostArgs {
+  float eta_dim, eps, h_max, h_min;
+  int max_active_bin;
+  double time_base;
+  float alpha_max, alpha_min, visc_length;
+  float diff_beta, diff_alpha_max, diff_alpha_min;
+};
+
+enum class LoopKind { Density, Gradient, Force };
+
+namespace sphk {
+hipError_t launch_stencil(LoopKind kind, bool subset, const DevParts &d,
+                          const LoopArgs &la, const int4 *tiles, int ntiles,
+                          hipStream_t st);
+hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
+                       const LoopArgs &la, const TaskArgs &ta, hipStream_t st);
+hipError_t launch_init_density(const DevParts &d, int lo, int hi,
+                               int max_active_bin, float h_max, hipStream_t st);
+hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,
+                        int *redo_count, hipStream_t st);
+hipError_t launch_extra_ghost(const DevParts &d, const GhostArgs &ga, int lo,
+                              int hi, hipStream_t st);
+hipError_t launch_end_force(const DevParts &d, int lo, int hi,
+                            int max_active_bin, hipStream_t st);
+hipError_t launch_count(const DevParts &d, int n, int max_active_bin,
+                        unsigned long long *out3, hipStream_t st);
+}  // namespace sphk

@@ -1,0 +1,221 @@
+"""Host-side mirror of SWIFT's task interface for the SPHENIX neighbour pass.
+
+``Engine`` owns a device-resident particle store (libswsph.so) and exposes
+the reference's task functions under their SWIFT names, with the same
+argument meaning: a top-level cell index where SWIFT passes a ``struct cell``
+(src/runner_doiact_hydro.h:177-210, src/runner.h:94-121).  Errors raise
+``SwsphError`` where SWIFT calls error().
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import SwsphError, check, lib
+from .ics import HydroProps
+
+FIELDS = {
+    "h": (0, np.float32, 1), "rho": (1, np.float32, 1),
+    "rho_dh": (2, np.float32, 1), "wcount": (3, np.float32, 1),
+    "wcount_dh": (4, np.float32, 1), "div_v": (5, np.float32, 1),
+    "rot_v": (6, np.float32, 3), "pressure": (7, np.float32, 1),
+    "soundspeed": (8, np.float32, 1), "f": (9, np.float32, 1),
+    "balsara": (10, np.float32, 1), "v_sig": (11, np.float32, 1),
+    "laplace_u": (12, np.float32, 1), "alpha_visc_max_ngb": (13, np.float32, 1),
+    "alpha_visc": (14, np.float32, 1), "alpha_diff": (15, np.float32, 1),
+    "div_v_dt": (16, np.float32, 1), "div_v_prev": (17, np.float32, 1),
+    "a_hydro": (18, np.float32, 3), "u_dt": (19, np.float32, 1),
+    "h_dt": (20, np.float32, 1), "n_density": (21, np.int32, 1),
+    "n_gradient": (22, np.int32, 1), "n_force": (23, np.int32, 1),
+    "min_ngb_time_bin": (24, np.int8, 1),
+}
+
+
+class _Props(C.Structure):
+    _fields_ = [("dim", C.c_double * 3), ("ncell", C.c_int * 3),
+                ("eta", C.c_float), ("h_tolerance", C.c_float),
+                ("h_max", C.c_float), ("h_min", C.c_float),
+                ("max_iter", C.c_int), ("alpha_max", C.c_float),
+                ("alpha_min", C.c_float), ("visc_length", C.c_float),
+                ("diff_beta", C.c_float), ("diff_alpha_max", C.c_float),
+                ("diff_alpha_min", C.c_float), ("time_base", C.c_double),
+                ("max_active_bin", C.c_int)]
+
+
+class _HostParts(C.Structure):
+    _fields_ = [("n", C.c_int64)] + [(k, C.c_void_p) for k in (
+        "x", "v", "m", "u", "time_bin", "h", "rho", "pressure", "soundspeed",
+        "f", "balsara", "alpha_visc", "alpha_diff", "div_v_prev")]
+
+
+class PassStats(C.Structure):
+    _fields_ = [("ghost_rounds", C.c_int), ("n_density", C.c_int64),
+                ("n_gradient", C.c_int64), ("n_force", C.c_int64),
+                ("launches", C.c_int64)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def _props(p: HydroProps) -> _Props:
+    s = _Props()
+    s.dim[:] = list(p.dim)
+    s.ncell[:] = list(p.ncell)
+    for k in ("eta", "h_tolerance", "h_max", "h_min", "max_iter", "alpha_max",
+              "alpha_min", "visc_length", "diff_beta", "diff_alpha_max",
+              "diff_alpha_min", "time_base", "max_active_bin"):
+        setattr(s, k, getattr(p, k))
+    return s
+
+
+class Engine:
+    """A device-resident particle store plus the SWIFT task entry points."""
+
+    def __init__(self, props: HydroProps):
+        self.props = props
+        self._h = C.c_void_p()
+        check(lib().swsph_engine_create(C.byref(_props(props)), C.byref(self._h)),
+              "swsph_engine_create")
+        self.n = 0
+
+    def close(self) -> None:
+        if self._h:
+            lib().swsph_engine_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- particles ---------------------------------------------------------
+    def set_particles(self, st: dict) -> None:
+        hp = _HostParts()
+        keep = []
+        n = st["h"].shape[0]
+        hp.n = n
+        want = {"x": np.float64, "v": np.float32, "m": np.float32,
+                "u": np.float32, "time_bin": np.int8, "h": np.float32,
+                "rho": np.float32, "pressure": np.float32,
+                "soundspeed": np.float32, "f": np.float32,
+                "balsara": np.float32, "alpha_visc": np.float32,
+                "alpha_diff": np.float32, "div_v_prev": np.float32}
+        for k, dt in want.items():
+            a = np.ascontiguousarray(st[k], dtype=dt)
+            keep.append(a)
+            setattr(hp, k, a.ctypes.data)
+        check(lib().swsph_set_particles(self._h, C.byref(hp)),
+              "swsph_set_particles")
+        self.n = n
+
+    def get(self, name: str) -> np.ndarray:
+        fid, dt, w = FIELDS[name]
+        out = np.empty((self.n, 3) if w == 3 else (self.n,), dt)
+        check(lib().swsph_get_field(self._h, fid, out.ctypes.data_as(C.c_void_p)),
+              f"swsph_get_field({name})")
+        return out
+
+    def results(self, names=None) -> dict:
+        return {k: self.get(k) for k in (names or FIELDS)}
+
+    def num_cells(self) -> int:
+        n = C.c_int()
+        check(lib().swsph_num_cells(self._h, C.byref(n)), "swsph_num_cells")
+        return n.value
+
+    def cell_count(self, c: int) -> int:
+        n = C.c_int()
+        check(lib().swsph_cell_count(self._h, c, C.byref(n)), "swsph_cell_count")
+        return n.value
+
+    def sync(self) -> None:
+        check(lib().swsph_sync(self._h), "swsph_sync")
+
+    # -- SWIFT task entry points (src/runner_doiact_hydro.h, src/runner.h) --
+    def hydro_init_cell(self, ci: int) -> None:
+        check(lib().swsph_init_cell(self._h, ci), "swsph_init_cell")
+
+    def runner_doself1_branch_density(self, ci: int) -> None:
+        check(lib().swsph_doself1_density(self._h, ci), "runner_doself1_branch_density")
+
+    def runner_dopair1_branch_density(self, ci: int, cj: int) -> None:
+        check(lib().swsph_dopair1_density(self._h, ci, cj),
+              "runner_dopair1_branch_density")
+
+    def runner_doself_subset_branch_density(self, ci: int) -> None:
+        check(lib().swsph_doself_subset_density(self._h, ci),
+              "runner_doself_subset_branch_density")
+
+    def runner_dopair_subset_branch_density(self, ci: int, cj: int) -> None:
+        check(lib().swsph_dopair_subset_density(self._h, ci, cj),
+              "runner_dopair_subset_branch_density")
+
+    def runner_do_ghost(self, ci: int) -> None:
+        check(lib().swsph_do_ghost(self._h, ci), "runner_do_ghost")
+
+    def runner_doself1_branch_gradient(self, ci: int) -> None:
+        check(lib().swsph_doself1_gradient(self._h, ci),
+              "runner_doself1_branch_gradient")
+
+    def runner_dopair1_branch_gradient(self, ci: int, cj: int) -> None:
+        check(lib().swsph_dopair1_gradient(self._h, ci, cj),
+              "runner_dopair1_branch_gradient")
+
+    def runner_do_extra_ghost(self, ci: int) -> None:
+        check(lib().swsph_do_extra_ghost(self._h, ci), "runner_do_extra_ghost")
+
+    def runner_doself2_branch_force(self, ci: int) -> None:
+        check(lib().swsph_doself2_force(self._h, ci), "runner_doself2_branch_force")
+
+    def runner_dopair2_branch_force(self, ci: int, cj: int) -> None:
+        check(lib().swsph_dopair2_force(self._h, ci, cj),
+              "runner_dopair2_branch_force")
+
+    def runner_do_end_hydro_force(self, ci: int) -> None:
+        check(lib().swsph_do_end_hydro_force(self._h, ci),
+              "runner_do_end_hydro_force")
+
+    # -- whole passes --------------------------------------------------------
+    def pass_plain(self) -> dict:
+        s = PassStats()
+        check(lib().swsph_pass_plain(self._h, C.byref(s)), "swsph_pass_plain")
+        return s.as_dict()
+
+    def pass_batched(self) -> dict:
+        s = PassStats()
+        check(lib().swsph_pass_batched(self._h, C.byref(s)), "swsph_pass_batched")
+        return s.as_dict()
+
+    def count_interactions(self) -> tuple:
+        out = (C.c_int64 * 3)()
+        check(lib().swsph_count_interactions(self._h, out),
+              "swsph_count_interactions")
+        return tuple(out)
+
+    def set_timing(self, on: bool) -> None:
+        check(lib().swsph_set_timing(self._h, int(on)), "swsph_set_timing")
+
+    def timing(self) -> tuple:
+        ms = (C.c_double * 3)()
+        nl = (C.c_int64 * 3)()
+        check(lib().swsph_get_timing(self._h, ms, nl), "swsph_get_timing")
+        return list(ms), list(nl)
+
+    def cell_coords(self, c: int) -> tuple:
+        nx, ny, nz = self.props.ncell
+        return c // (ny * nz), (c // nz) % ny, c % nz
+
+    def cell_id(self, cx: int, cy: int, cz: int) -> int:
+        nx, ny, nz = self.props.ncell
+        return ((cx % nx) * ny + (cy % ny)) * nz + (cz % nz)
+
+
+__all__ = ["Engine", "SwsphError", "FIELDS", "PassStats"]

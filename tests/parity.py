@@ -1,0 +1,82 @@
+"""Comparison helpers shared by the parity tests.
+
+Error measure (the tolerance stated by the task: 1e-5 for density
+quantities, 1e-4 for force quantities, against an FP64 rerun of the same
+pass at the same smoothing lengths):
+
+    err_i = |x_i - x64_i| / max(|x64_i|, S_i)
+
+where S_i is the FP64 magnitude sum sum_j |term_ij| of the accumulation
+(0 for quantities that are not sums).  For sums without cancellation
+(rho, wcount) S_i = |x64_i| and this is the plain relative error; for
+cancelling sums (div_v, curl, a_hydro, u_dt, ...) it is the error relative
+to the size of the contributions, the scale FP32 summation can resolve.
+Neighbour counts are compared exactly.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+DENSITY_TOL = 1e-5
+FORCE_TOL = 1e-4
+
+# field -> (tolerance, scale key in the FP64 oracle output or None)
+DENSITY_FIELDS = {
+    "rho": (DENSITY_TOL, None),
+    "wcount": (DENSITY_TOL, None),
+    "div_v": (DENSITY_TOL, "s_div_v"),
+    "rot_v": (DENSITY_TOL, "s_rot_v"),
+    "rho_dh": (DENSITY_TOL, "s_rho_dh"),
+    "wcount_dh": (DENSITY_TOL, "s_wcount_dh"),
+}
+FORCE_FIELDS = {
+    "pressure": (FORCE_TOL, None),
+    "soundspeed": (FORCE_TOL, None),
+    "v_sig": (FORCE_TOL, None),
+    "laplace_u": (FORCE_TOL, "s_laplace_u"),
+    "a_hydro": (FORCE_TOL, "s_a_hydro"),
+    "u_dt": (FORCE_TOL, "s_u_dt"),
+    "h_dt": (FORCE_TOL, "s_h_dt"),
+    "alpha_visc": (FORCE_TOL, None),
+    "alpha_diff": (FORCE_TOL, None),
+}
+COUNT_FIELDS = ["n_density", "n_gradient", "n_force", "min_ngb_time_bin"]
+
+
+def rel_err(x, ref, scale=None, mask=None) -> float:
+    x = np.asarray(x, np.float64)
+    ref = np.asarray(ref, np.float64)
+    den = np.abs(ref)
+    if scale is not None:
+        den = np.maximum(den, np.asarray(scale, np.float64))
+    den = np.where(den > 0, den, 1.0)
+    e = np.abs(x - ref) / den
+    if mask is not None:
+        e = e[mask] if e.ndim == 1 else e[mask, :]
+    return float(e.max()) if e.size else 0.0
+
+
+def compare(res: dict, ref: dict, active: np.ndarray, fields=None) -> dict:
+    """Max error per field over active particles; counts as mismatch counts."""
+    out = {}
+    fields = fields or {**DENSITY_FIELDS, **FORCE_FIELDS}
+    for k, (tol, sk) in fields.items():
+        out[k] = rel_err(res[k], ref[k], ref[sk] if sk else None, active)
+    for k in COUNT_FIELDS:
+        out[k] = int(np.count_nonzero(res[k][active] != ref[k][active]))
+    return out
+
+
+def assert_parity(res: dict, ref: dict, active: np.ndarray, fields=None,
+                  tol_scale: float = 1.0) -> dict:
+    fields = fields or {**DENSITY_FIELDS, **FORCE_FIELDS}
+    errs = compare(res, ref, active, fields)
+    bad = []
+    for k, (tol, _) in fields.items():
+        if not errs[k] <= tol * tol_scale:
+            bad.append(f"{k}: {errs[k]:.3e} > {tol * tol_scale:.1e}")
+    for k in COUNT_FIELDS:
+        if errs[k] != 0:
+            bad.append(f"{k}: {errs[k]} mismatching particles")
+    assert not bad, "; ".join(bad) + f"  (all: {errs})"
+    return errs

@@ -1,0 +1,74 @@
+"""The C-ABI library loads and exports every entry point of include/swsph.h
+(no compute: this runs without a GPU), and the host-side checks that do not
+need a device behave."""
+from __future__ import annotations
+
+import ctypes as C
+
+import pytest
+
+
+def test_library_exports_declared_symbols():
+    from swift_amd._lib import declared_symbols, lib
+    L = lib()
+    syms = declared_symbols()
+    assert len(syms) >= 25
+    missing = [s for s in syms if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_null_arguments_are_errors():
+    from swift_amd._lib import lib
+    L = lib()
+    assert L.swsph_engine_create(None, None) != 0
+    assert b"null" in L.swsph_last_error()
+    assert L.swsph_pass_batched(None, None) != 0
+    assert L.swsph_get_field(None, 0, None) != 0
+
+
+def test_bad_grid_rejected_before_device():
+    """Fewer than 3 top-level cells per axis is an error (the 27-cell stencil
+    would visit a cell twice); checked before any device call."""
+    from swift_amd import Engine, HydroProps, SwsphError
+    with pytest.raises(SwsphError, match="3 top-level cells"):
+        Engine(HydroProps(ncell=(2, 4, 4)))
+
+
+def test_no_cpu_fallback(monkeypatch, tmp_path):
+    """A missing library fails loudly instead of falling back to the CPU."""
+    import swift_amd._lib as L
+    monkeypatch.setattr(L, "LIB_PATH", str(tmp_path / "libswsph.so"))
+    monkeypatch.setattr(L, "_lib", None)
+    with pytest.raises(L.SwsphError, match="no CPU fallback"):
+        L.lib()
+
+
+def test_structs_match_header(tmp_path):
+    """The ctypes mirrors have the C layout of include/swsph.h: a C program
+    compiled against the header prints sizeof/offsetof of every field."""
+    import shutil
+    import subprocess
+    from swift_amd._lib import HEADER_PATH
+    from swift_amd.engine import _Props, _HostParts, PassStats
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {"swsph_props": _Props, "swsph_host_parts": _HostParts,
+               "swsph_pass_stats": PassStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>',
+             f'#include "{HEADER_PATH}"', 'int main(void) {']
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ['return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-o", str(exe), str(src)], check=True)
+    out = dict(l.rsplit(" ", 1) for l in
+               subprocess.run([str(exe)], check=True, capture_output=True,
+                              text=True).stdout.split("\n") if l)
+    for cname, py in structs.items():
+        assert int(out[cname]) == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert int(out[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
