@@ -147,6 +147,13 @@ int swsph_do_end_hydro_force(swsph_engine *e, int ci);
 int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats);
 int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats);
 
+/* Device-to-device save / restore of the state a pass reads and rewrites
+ * (h and the (io) fields), on the engine stream: the device-resident
+ * equivalent of restart_write_blocks / restart_read for this path
+ * (src/restart.c:130, 217). */
+int swsph_save_state(swsph_engine *e);
+int swsph_restore_state(swsph_engine *e);
+
 /* Interaction counters summed over active particles (device reduction). */
 int swsph_count_interactions(swsph_engine *e, int64_t out3[3]);
 

@@ -80,7 +80,20 @@ struct swsph_engine {
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
   size_t ev_used = 0;
   int64_t launches = 0;
+  std::vector<float *> saved;     // swsph_save_state copies
 };
+
+namespace {
+// The (io) state of a pass, in the order saved by swsph_save_state.
+std::vector<float *> io_fields(DevParts &d) {
+  return {d.h, d.rho, d.P, d.c, d.f, d.bal, d.alpha, d.adiff, d.divv_prev};
+}
+void free_saved(swsph_engine *e) {
+  for (float *p : e->saved)
+    if (p) (void)hipFree(p);
+  e->saved.clear();
+}
+}  // namespace
 
 namespace {
 
@@ -124,6 +137,7 @@ void free_dev(swsph_engine *e) {
   e->d = DevParts{};
   e->d_cell_start = nullptr;
   e->d_tiles = nullptr;
+  free_saved(e);
 }
 
 int alloc_dev(swsph_engine *e, size_t n) {
@@ -743,6 +757,33 @@ int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats) {
   }
   if ((rc = count_into(e, stats))) return rc;
   return collect_timing(e);
+}
+
+int swsph_save_state(swsph_engine *e) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  std::vector<float *> src = io_fields(e->d);
+  const size_t n = (size_t)(e->n ? e->n : 1);
+  if (e->saved.empty()) {
+    e->saved.assign(src.size(), nullptr);
+    for (auto &p : e->saved) HIPCHK(dalloc(&p, n));
+  }
+  for (size_t k = 0; k < src.size(); k++)
+    HIPCHK(hipMemcpyAsync(e->saved[k], src[k], sizeof(float) * n,
+                          hipMemcpyDeviceToDevice, e->st));
+  return 0;
+}
+
+int swsph_restore_state(swsph_engine *e) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->saved.empty()) return fail(1, "no saved state");
+  std::vector<float *> dst = io_fields(e->d);
+  const size_t n = (size_t)(e->n ? e->n : 1);
+  for (size_t k = 0; k < dst.size(); k++)
+    HIPCHK(hipMemcpyAsync(dst[k], e->saved[k], sizeof(float) * n,
+                          hipMemcpyDeviceToDevice, e->st));
+  return 0;
 }
 
 int swsph_count_interactions(swsph_engine *e, int64_t out3[3]) {

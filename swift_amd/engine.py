@@ -194,6 +194,12 @@ class Engine:
         check(lib().swsph_pass_batched(self._h, C.byref(s)), "swsph_pass_batched")
         return s.as_dict()
 
+    def save_state(self) -> None:
+        check(lib().swsph_save_state(self._h), "swsph_save_state")
+
+    def restore_state(self) -> None:
+        check(lib().swsph_restore_state(self._h), "swsph_restore_state")
+
     def count_interactions(self) -> tuple:
         out = (C.c_int64 * 3)()
         check(lib().swsph_count_interactions(self._h, out),
