@@ -44,7 +44,6 @@ int fail(int code, const char *fmt, ...) {
                   hipGetErrorString(err_), __FILE__, __LINE__);            \
   } while (0)
 
-constexpr int kNumTimeBins = 56;
 constexpr float kGammaK = (float)(1.936492);
 
 template <class T>
@@ -131,7 +130,8 @@ void free_dev(swsph_engine *e) {
                   e->d.P, e->d.c, e->d.f, e->d.bal, e->d.vsig, e->d.lapu,
                   e->d.avmax, e->d.ng, e->d.alpha, e->d.adiff, e->d.divv_prev,
                   e->d.divv_dt, e->d.ax, e->d.ay, e->d.az, e->d.udt, e->d.hdt,
-                  e->d.nf, e->d.mintb, e->d_cell_start, e->d_tiles};
+                  e->d.nf, e->d.mintb, e->d.rd, e->d.rg, e->d.rf,
+                  e->d_cell_start, e->d_tiles};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   e->d = DevParts{};
@@ -155,6 +155,9 @@ int alloc_dev(swsph_engine *e, size_t n) {
   for (int **p : i32) HIPCHK(dalloc(p, n));
   HIPCHK(dalloc(&d.tb, n));
   HIPCHK(dalloc(&d.mintb, n));
+  HIPCHK(dalloc(&d.rd, n));
+  HIPCHK(dalloc(&d.rg, n));
+  HIPCHK(dalloc(&d.rf, n));
   // Zero everything once so that fields never written for inactive
   // particles read as zero.
   const size_t nn = n ? n : 1;
@@ -327,6 +330,18 @@ int ghost_cell(swsph_engine *e, int ci, int *rounds) {
   if (redo > 0)
     return fail(3, "Smoothing length failed to converge on %d particles", redo);
   if (rounds) *rounds = std::max(*rounds, it + 1);
+  // The gradient loop reads rho, c, u, alpha of this cell's particles.
+  HIPCHK(sphk::launch_build_records(LoopKind::Gradient, e->d, lo, hi, e->st));
+  e->launches++;
+  return 0;
+}
+
+// runner_do_extra_ghost on one cell, then the force records of its particles.
+int extra_ghost_cell(swsph_engine *e, int c) {
+  const int lo = e->cell_start[c], hi = e->cell_start[c + 1];
+  HIPCHK(sphk::launch_extra_ghost(e->d, ghost_args(e), lo, hi, e->st));
+  HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, lo, hi, e->st));
+  e->launches += 2;
   return 0;
 }
 
@@ -519,6 +534,11 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
   if (e->ntiles)
     HIPCHK(hipMemcpy(e->d_tiles, e->tiles_h.data(), sizeof(int4) * e->ntiles,
                      hipMemcpyHostToDevice));
+  // Neighbour records of every particle from the state just uploaded
+  // (inactive particles keep these for the whole pass).
+  for (LoopKind k : {LoopKind::Density, LoopKind::Gradient, LoopKind::Force})
+    HIPCHK(sphk::launch_build_records(k, e->d, 0, (int)n, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
   return check_h_fits(e);
 }
 
@@ -660,10 +680,7 @@ int swsph_do_extra_ghost(swsph_engine *e, int ci) {
   std::lock_guard<std::mutex> lk(e->mtx);
   int rc = cell_ok(e, ci);
   if (rc) return rc;
-  HIPCHK(sphk::launch_extra_ghost(e->d, ghost_args(e), e->cell_start[ci],
-                                  e->cell_start[ci + 1], e->st));
-  e->launches++;
-  return 0;
+  return extra_ghost_cell(e, ci);
 }
 
 int swsph_do_end_hydro_force(swsph_engine *e, int ci) {
@@ -684,7 +701,6 @@ int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
   int rc = 0;
   const int64_t l0 = e->launches;
   std::vector<int> up;
-  const GhostArgs ga = ghost_args(e);
   for (int c = 0; c < e->ncells; c++) {
     HIPCHK(sphk::launch_init_density(e->d, e->cell_start[c], e->cell_start[c + 1],
                                      e->props.max_active_bin, e->props.h_max,
@@ -706,11 +722,8 @@ int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
   for (int c = 0; c < e->ncells; c++)
     if ((rc = ghost_cell(e, c, &rounds))) return rc;
   if ((rc = loop_tasks(LoopKind::Gradient))) return rc;
-  for (int c = 0; c < e->ncells; c++) {
-    HIPCHK(sphk::launch_extra_ghost(e->d, ga, e->cell_start[c],
-                                    e->cell_start[c + 1], e->st));
-    e->launches++;
-  }
+  for (int c = 0; c < e->ncells; c++)
+    if ((rc = extra_ghost_cell(e, c))) return rc;
   if ((rc = loop_tasks(LoopKind::Force))) return rc;
   for (int c = 0; c < e->ncells; c++) {
     HIPCHK(sphk::launch_end_force(e->d, e->cell_start[c], e->cell_start[c + 1],
@@ -745,9 +758,11 @@ int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats) {
   }
   if (redo > 0)
     return fail(3, "Smoothing length failed to converge on %d particles", redo);
+  HIPCHK(sphk::launch_build_records(LoopKind::Gradient, e->d, 0, n, e->st));
   if ((rc = stencil(e, LoopKind::Gradient, false))) return rc;
   HIPCHK(sphk::launch_extra_ghost(e->d, ga, 0, n, e->st));
-  e->launches++;
+  HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, 0, n, e->st));
+  e->launches += 3;
   if ((rc = stencil(e, LoopKind::Force, false))) return rc;
   HIPCHK(sphk::launch_end_force(e->d, 0, n, e->props.max_active_bin, e->st));
   e->launches++;

@@ -4,6 +4,35 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+// Neighbour records: the fields a neighbour loop reads from particle j,
+// packed so that the interaction phase fetches them with a few 16-byte
+// loads.  The leading double position (and, for the force loop, the search
+// radius) is what the candidate test reads.  Inhibited particles get a far
+// position (and a negative radius) so that every test rejects them.
+struct __attribute__((aligned(16))) DensRec {
+  double x, y, z;
+  float vx, vy, vz, m;
+  float pad[2];
+};
+// Record sizes are odd multiples of 16 bytes so that lanes gathering
+// different records from LDS spread over the banks.
+struct __attribute__((aligned(16))) GradRec {
+  double x, y, z;
+  float vx, vy, vz, mrho;  // m_j / rho_j
+  float u, c, alpha, pad0;
+  float pad[6];
+};
+struct __attribute__((aligned(16))) ForceRec {
+  double x, y, z;
+  float hg2;               // (gamma_K h_j)^2
+  float vx, vy, vz;
+  float m, inv_m, h_inv, h_inv4;
+  float rho, inv_rho, pr2, f;  // pr2 = P / rho^2
+  float bal, c, alpha, P;
+  float Pad, u, tb, pad0;      // Pad = P * alpha_diff
+  float pad[2];
+};
+
 // Structure-of-arrays particle store in HBM, sorted by top-level cell.
 struct DevParts {
   double *px, *py, *pz;
@@ -26,6 +55,10 @@ struct DevParts {
   float *ax, *ay, *az, *udt, *hdt;
   int *nf;
   int8_t *mintb;
+  // neighbour records (rebuilt after the stage that changes their fields)
+  DensRec *rd;
+  GradRec *rg;
+  ForceRec *rf;
 };
 
 struct GridDesc {
@@ -74,4 +107,6 @@ hipError_t launch_end_force(const DevParts &d, int lo, int hi,
                             int max_active_bin, hipStream_t st);
 hipError_t launch_count(const DevParts &d, int n, int max_active_bin,
                         unsigned long long *out3, hipStream_t st);
+hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
+                                int hi, hipStream_t st);
 }  // namespace sphk
