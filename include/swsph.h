@@ -154,6 +154,15 @@ int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats);
 int swsph_save_state(swsph_engine *e);
 int swsph_restore_state(swsph_engine *e);
 
+/* Tile shape of a neighbour loop (which = 0 density, 1 gradient, 2 force):
+ * particles are grouped in chunks of C consecutive particles of a cell
+ * (C in {8, 16, 32, 64}); one wave64 owns one chunk of i-particles with
+ * 64 / C candidate streams per particle.  No counterpart in the reference
+ * (its loops are scalar); the shape changes speed, not results beyond FP32
+ * summation order. */
+int swsph_set_chunk(swsph_engine *e, int which, int C);
+int swsph_get_chunk(swsph_engine *e, int which, int *C);
+
 /* Interaction counters summed over active particles (device reduction). */
 int swsph_count_interactions(swsph_engine *e, int64_t out3[3]);
 

@@ -80,6 +80,20 @@ struct swsph_engine {
   size_t ev_used = 0;
   int64_t launches = 0;
   std::vector<float *> saved;     // swsph_save_state copies
+  // Chunk tables, one per chunk capacity in use, and the capacity used by
+  // each loop (density, gradient, force).
+  struct ChunkTable {
+    int C = 0;
+    int nchunks = 0;
+    std::vector<int> cell_chunk_h;  // [ncells + 1]
+    int *d_start = nullptr, *d_cell_chunk = nullptr, *d_chunk_cell = nullptr;
+    double *d_bbox = nullptr;
+    float *d_hg2max = nullptr;
+  };
+  std::vector<ChunkTable> tables;
+  int loop_c[3] = {16, 16, 16};
+  unsigned int *d_hg2_all = nullptr;  // max (gamma h)^2 over all chunks
+  unsigned int *h_hg2_all = nullptr;
 };
 
 namespace {
@@ -130,7 +144,7 @@ void free_dev(swsph_engine *e) {
                   e->d.P, e->d.c, e->d.f, e->d.bal, e->d.vsig, e->d.lapu,
                   e->d.avmax, e->d.ng, e->d.alpha, e->d.adiff, e->d.divv_prev,
                   e->d.divv_dt, e->d.ax, e->d.ay, e->d.az, e->d.udt, e->d.hdt,
-                  e->d.nf, e->d.mintb, e->d.rd, e->d.rg, e->d.rf,
+                  e->d.nf, e->d.mintb, e->d.rd, e->d.rg, e->d.rf, e->d.pos,
                   e->d_cell_start, e->d_tiles};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -138,6 +152,91 @@ void free_dev(swsph_engine *e) {
   e->d_cell_start = nullptr;
   e->d_tiles = nullptr;
   free_saved(e);
+  for (auto &t : e->tables) {
+    void *tp[] = {t.d_start, t.d_cell_chunk, t.d_chunk_cell, t.d_bbox,
+                  t.d_hg2max};
+    for (void *p : tp)
+      if (p) (void)hipFree(p);
+  }
+  e->tables.clear();
+}
+
+// Chunk table for capacity C: runs of <= C consecutive particles of a cell.
+// Built on first use; boxes and radii are filled by refresh_bounds.
+int refresh_bounds(swsph_engine *e, int c_lo, int c_hi);
+
+swsph_engine::ChunkTable *chunk_table(swsph_engine *e, int C, int *rc) {
+  for (auto &t : e->tables)
+    if (t.C == C) return &t;
+  *rc = 0;
+  if (C != 8 && C != 16 && C != 32 && C != 64) {
+    *rc = fail(1, "chunk capacity %d not in {8, 16, 32, 64}", C);
+    return nullptr;
+  }
+  swsph_engine::ChunkTable t;
+  t.C = C;
+  std::vector<int> start, cell_of;
+  t.cell_chunk_h.assign(e->ncells + 1, 0);
+  for (int c = 0; c < e->ncells; c++) {
+    t.cell_chunk_h[c] = (int)start.size();
+    for (int f = e->cell_start[c]; f < e->cell_start[c + 1]; f += C) {
+      start.push_back(f);
+      cell_of.push_back(c);
+    }
+  }
+  t.nchunks = (int)start.size();
+  t.cell_chunk_h[e->ncells] = t.nchunks;
+  start.push_back((int)e->n);
+  auto up = [&](int **dst, const std::vector<int> &v) -> int {
+    HIPCHK(dalloc(dst, std::max<size_t>(1, v.size())));
+    if (!v.empty())
+      HIPCHK(hipMemcpy(*dst, v.data(), sizeof(int) * v.size(),
+                       hipMemcpyHostToDevice));
+    return 0;
+  };
+  if ((*rc = up(&t.d_start, start)) || (*rc = up(&t.d_cell_chunk, t.cell_chunk_h)) ||
+      (*rc = up(&t.d_chunk_cell, cell_of)))
+    return nullptr;
+  if (hipMalloc((void **)&t.d_bbox, sizeof(double) * 6 * std::max(1, t.nchunks)) !=
+          hipSuccess ||
+      hipMalloc((void **)&t.d_hg2max, sizeof(float) * std::max(1, t.nchunks)) !=
+          hipSuccess) {
+    *rc = fail(2, "allocation of chunk table failed");
+    return nullptr;
+  }
+  e->tables.push_back(t);
+  swsph_engine::ChunkTable *tp = &e->tables.back();
+  if ((*rc = refresh_bounds(e, 0, e->ncells))) return nullptr;
+  return tp;
+}
+
+ChunkArgs chunk_args(const swsph_engine::ChunkTable *t) {
+  ChunkArgs ca;
+  ca.chunk_start = t->d_start;
+  ca.cell_chunk = t->d_cell_chunk;
+  ca.chunk_cell = t->d_chunk_cell;
+  ca.bbox = t->d_bbox;
+  ca.hg2max = t->d_hg2max;
+  ca.nchunks = t->nchunks;
+  ca.C = t->C;
+  return ca;
+}
+
+// Candidate positions / radii and chunk bounds of cells [c_lo, c_hi), after
+// smoothing lengths changed.
+int refresh_bounds(swsph_engine *e, int c_lo, int c_hi) {
+  const int lo = e->cell_start[c_lo], hi = e->cell_start[c_hi];
+  if (c_lo == 0 && c_hi == e->ncells)
+    HIPCHK(sphk::launch_zero_u32(e->d_hg2_all, 1, e->st));
+  HIPCHK(sphk::launch_build_pos(e->d, lo, hi, e->st));
+  e->launches++;
+  for (auto &t : e->tables) {
+    HIPCHK(sphk::launch_chunk_bounds(e->d, t.d_start, t.cell_chunk_h[c_lo],
+                                     t.cell_chunk_h[c_hi], t.d_bbox,
+                                     t.d_hg2max, e->d_hg2_all, e->st));
+    e->launches++;
+  }
+  return 0;
 }
 
 int alloc_dev(swsph_engine *e, size_t n) {
@@ -158,6 +257,7 @@ int alloc_dev(swsph_engine *e, size_t n) {
   HIPCHK(dalloc(&d.rd, n));
   HIPCHK(dalloc(&d.rg, n));
   HIPCHK(dalloc(&d.rf, n));
+  HIPCHK(dalloc(&d.pos, n));
   // Zero everything once so that fields never written for inactive
   // particles read as zero.
   const size_t nn = n ? n : 1;
@@ -206,8 +306,6 @@ int cell_ok(const swsph_engine *e, int c) {
   return 0;
 }
 
-int tiles_of(const swsph_engine *e, int c) { return e->cell_tiles[c]; }
-
 hipEvent_t next_event(swsph_engine *e) {
   if (e->ev_used == e->ev_pool.size()) {
     hipEvent_t ev;
@@ -219,6 +317,9 @@ hipEvent_t next_event(swsph_engine *e) {
 
 // Launch a batched loop, bracketed by timing events when enabled.
 int stencil(swsph_engine *e, LoopKind k, bool subset) {
+  int rc = 0;
+  const swsph_engine::ChunkTable *t = chunk_table(e, e->loop_c[(int)k], &rc);
+  if (!t) return rc;
   hipEvent_t a = nullptr, b = nullptr;
   if (e->timing) {
     a = next_event(e);
@@ -226,8 +327,8 @@ int stencil(swsph_engine *e, LoopKind k, bool subset) {
     if (!a || !b) return fail(2, "hipEventCreate failed");
     HIPCHK(hipEventRecord(a, e->st));
   }
-  HIPCHK(sphk::launch_stencil(k, subset, e->d, loop_args(e), e->d_tiles,
-                              e->ntiles, e->st));
+  HIPCHK(sphk::launch_stencil(k, subset, e->d, loop_args(e), chunk_args(t),
+                              e->st));
   e->launches++;
   if (e->timing) {
     HIPCHK(hipEventRecord(b, e->st));
@@ -257,18 +358,22 @@ int task(swsph_engine *e, LoopKind k, bool subset, int ci, int cj, bool both) {
   double t[3];
   if (!adjacent(e, ci, cj, t))
     return fail(1, "cells %d and %d are not neighbours", ci, cj);
+  const swsph_engine::ChunkTable *tb = chunk_table(e, e->loop_c[(int)k], &rc);
+  if (!tb) return rc;
+  auto nch = [&](int c) { return tb->cell_chunk_h[c + 1] - tb->cell_chunk_h[c]; };
   ta.icell[0] = ci;
   ta.jcell[0] = cj;
-  ta.ntiles[0] = tiles_of(e, ci);
+  ta.ntiles[0] = nch(ci);
   for (int k2 = 0; k2 < 3; k2++) ta.t[k2] = t[k2];
   if (both && ci != cj) {
     ta.icell[1] = cj;
     ta.jcell[1] = ci;
-    ta.ntiles[1] = tiles_of(e, cj);
+    ta.ntiles[1] = nch(cj);
     for (int k2 = 0; k2 < 3; k2++) ta.t[3 + k2] = -t[k2];
   }
   if (ta.ntiles[0] + ta.ntiles[1] == 0) return 0;
-  HIPCHK(sphk::launch_task(k, subset, e->d, loop_args(e), ta, e->st));
+  HIPCHK(sphk::launch_task(k, subset, e->d, loop_args(e), chunk_args(tb), ta,
+                           e->st));
   e->launches++;
   return 0;
 }
@@ -330,10 +435,11 @@ int ghost_cell(swsph_engine *e, int ci, int *rounds) {
   if (redo > 0)
     return fail(3, "Smoothing length failed to converge on %d particles", redo);
   if (rounds) *rounds = std::max(*rounds, it + 1);
-  // The gradient loop reads rho, c, u, alpha of this cell's particles.
+  // The gradient loop reads rho, c, u, alpha of this cell's particles; the
+  // candidate radii and chunk bounds follow the new smoothing lengths.
   HIPCHK(sphk::launch_build_records(LoopKind::Gradient, e->d, lo, hi, e->st));
   e->launches++;
-  return 0;
+  return refresh_bounds(e, ci, ci + 1);
 }
 
 // runner_do_extra_ghost on one cell, then the force records of its particles.
@@ -355,6 +461,25 @@ int count_into(swsph_engine *e, swsph_pass_stats *s) {
     s->n_density = (int64_t)e->h_cnt[0];
     s->n_gradient = (int64_t)e->h_cnt[1];
     s->n_force = (int64_t)e->h_cnt[2];
+  }
+  return 0;
+}
+
+// After a pass: the largest kernel support met by the chunk bounds must fit
+// in a top-level cell, else the 27-cell stencil may have missed neighbours
+// (the reference would have to rebuild its cell tree).
+int check_support(swsph_engine *e) {
+  HIPCHK(hipMemcpy(e->h_hg2_all, e->d_hg2_all, sizeof(unsigned int),
+                   hipMemcpyDeviceToHost));
+  float hg2;
+  std::memcpy(&hg2, e->h_hg2_all, sizeof(float));
+  for (int k = 0; k < 3; k++) {
+    const double w = e->grid.dim[k] / e->grid.nc[k];
+    if ((double)hg2 > w * w)
+      return fail(4,
+                  "kernel support %g exceeds the top-level cell width %g after "
+                  "the ghost: regrid with fewer cells",
+                  std::sqrt((double)hg2), w);
   }
   return 0;
 }
@@ -409,10 +534,16 @@ int swsph_engine_create(const swsph_props *props, swsph_engine **out) {
       hipHostMalloc((void **)&e->h_redo, sizeof(int), 0) != hipSuccess ||
       hipMalloc((void **)&e->d_cnt, 3 * sizeof(unsigned long long)) != hipSuccess ||
       hipHostMalloc((void **)&e->h_cnt, 3 * sizeof(unsigned long long), 0) !=
+          hipSuccess ||
+      hipMalloc((void **)&e->d_hg2_all, sizeof(unsigned int)) != hipSuccess ||
+      hipHostMalloc((void **)&e->h_hg2_all, sizeof(unsigned int), 0) !=
           hipSuccess) {
     delete e;
     return fail(2, "allocation of engine counters failed");
   }
+  // One table per admissible chunk capacity at most: never reallocated, so
+  // table pointers stay valid.
+  e->tables.reserve(4);
   *out = e;
   return 0;
 }
@@ -425,6 +556,8 @@ int swsph_engine_destroy(swsph_engine *e) {
   if (e->d_redo) (void)hipFree(e->d_redo);
   if (e->h_redo) (void)hipHostFree(e->h_redo);
   if (e->d_cnt) (void)hipFree(e->d_cnt);
+  if (e->d_hg2_all) (void)hipFree(e->d_hg2_all);
+  if (e->h_hg2_all) (void)hipHostFree(e->h_hg2_all);
   if (e->h_cnt) (void)hipHostFree(e->h_cnt);
   if (e->st) (void)hipStreamDestroy(e->st);
   delete e;
@@ -447,8 +580,9 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
   if (!e || !p) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
   const int64_t n = p->n;
-  if (n < 0 || n > (int64_t)0x7fffffff - 64)
-    return fail(1, "particle count %lld out of range", (long long)n);
+  // Neighbour-loop row tables pack a particle index in 27 bits.
+  if (n < 0 || n > ((int64_t)1 << 27))
+    return fail(1, "particle count %lld out of range (max 2^27)", (long long)n);
   if (!p->x || !p->v || !p->m || !p->u || !p->time_bin || !p->h)
     return fail(1, "missing particle arrays");
   HIPCHK(hipStreamSynchronize(e->st));
@@ -477,17 +611,40 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
   e->perm.assign(n, 0);
   for (int64_t i = 0; i < n; i++) e->perm[fill[cid[i]]++] = i;
 
-  // Tiles of <= 64 consecutive particles of one cell.
-  e->tiles_h.clear();
-  e->cell_tiles.assign(e->ncells, 0);
-  for (int c = 0; c < e->ncells; c++) {
-    const int lo = e->cell_start[c], hi = e->cell_start[c + 1];
-    for (int f = lo; f < hi; f += 64) {
-      e->tiles_h.push_back(make_int4(c, f, std::min(64, hi - f), 0));
-      e->cell_tiles[c]++;
+  // Inside each cell, order particles along a Morton curve (10 bits per
+  // axis of the cell-relative position) so that any run of consecutive
+  // particles is spatially compact.
+  {
+    auto spread = [](uint32_t v) {
+      uint64_t x = v & 0x3ff;
+      x = (x | (x << 16)) & 0x030000FFull;
+      x = (x | (x << 8)) & 0x0300F00Full;
+      x = (x | (x << 4)) & 0x030C30C3ull;
+      x = (x | (x << 2)) & 0x09249249ull;
+      return x;
+    };
+    std::vector<std::pair<uint64_t, int64_t>> keyed;
+    for (int c = 0; c < e->ncells; c++) {
+      const int lo = e->cell_start[c], hi = e->cell_start[c + 1];
+      if (hi - lo < 2) continue;
+      const int cc[3] = {c / (nc[1] * nc[2]), (c / nc[2]) % nc[1], c % nc[2]};
+      keyed.resize(hi - lo);
+      for (int s = lo; s < hi; s++) {
+        const int64_t i = e->perm[s];
+        uint64_t key = 0;
+        for (int k = 0; k < 3; k++) {
+          double f = (p->x[3 * i + k] * inv_w[k] - cc[k]) * 1024.0;
+          int q = (int)f;
+          q = q < 0 ? 0 : (q > 1023 ? 1023 : q);
+          key |= spread((uint32_t)q) << (2 - k);
+        }
+        keyed[s - lo] = {key, i};
+      }
+      std::sort(keyed.begin(), keyed.end());
+      for (int s = lo; s < hi; s++) e->perm[s] = keyed[s - lo].second;
     }
   }
-  e->ntiles = (int)e->tiles_h.size();
+  e->ntiles = 0;
 
   // Gather to sorted SoA and upload.
   std::vector<double> xd(n);
@@ -530,14 +687,14 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
   HIPCHK(dalloc(&e->d_cell_start, (size_t)e->ncells + 1));
   HIPCHK(hipMemcpy(e->d_cell_start, e->cell_start.data(),
                    sizeof(int) * (e->ncells + 1), hipMemcpyHostToDevice));
-  HIPCHK(dalloc(&e->d_tiles, (size_t)std::max(1, e->ntiles)));
-  if (e->ntiles)
-    HIPCHK(hipMemcpy(e->d_tiles, e->tiles_h.data(), sizeof(int4) * e->ntiles,
-                     hipMemcpyHostToDevice));
   // Neighbour records of every particle from the state just uploaded
-  // (inactive particles keep these for the whole pass).
+  // (inactive particles keep these for the whole pass), candidate positions
+  // and the chunk tables of the loops.
   for (LoopKind k : {LoopKind::Density, LoopKind::Gradient, LoopKind::Force})
     HIPCHK(sphk::launch_build_records(k, e->d, 0, (int)n, e->st));
+  HIPCHK(sphk::launch_build_pos(e->d, 0, (int)n, e->st));
+  for (int k = 0; k < 3; k++)
+    if (!chunk_table(e, e->loop_c[k], &rc)) return rc;
   HIPCHK(hipStreamSynchronize(e->st));
   return check_h_fits(e);
 }
@@ -701,6 +858,7 @@ int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
   int rc = 0;
   const int64_t l0 = e->launches;
   std::vector<int> up;
+  HIPCHK(sphk::launch_zero_u32(e->d_hg2_all, 1, e->st));
   for (int c = 0; c < e->ncells; c++) {
     HIPCHK(sphk::launch_init_density(e->d, e->cell_start[c], e->cell_start[c + 1],
                                      e->props.max_active_bin, e->props.h_max,
@@ -734,7 +892,8 @@ int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
     stats->ghost_rounds = rounds;
     stats->launches = e->launches - l0;
   }
-  return count_into(e, stats);
+  if ((rc = count_into(e, stats))) return rc;
+  return check_support(e);
 }
 
 int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats) {
@@ -759,10 +918,12 @@ int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats) {
   if (redo > 0)
     return fail(3, "Smoothing length failed to converge on %d particles", redo);
   HIPCHK(sphk::launch_build_records(LoopKind::Gradient, e->d, 0, n, e->st));
+  e->launches++;
+  if ((rc = refresh_bounds(e, 0, e->ncells))) return rc;
   if ((rc = stencil(e, LoopKind::Gradient, false))) return rc;
   HIPCHK(sphk::launch_extra_ghost(e->d, ga, 0, n, e->st));
   HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, 0, n, e->st));
-  e->launches += 3;
+  e->launches += 2;
   if ((rc = stencil(e, LoopKind::Force, false))) return rc;
   HIPCHK(sphk::launch_end_force(e->d, 0, n, e->props.max_active_bin, e->st));
   e->launches++;
@@ -771,6 +932,7 @@ int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats) {
     stats->launches = e->launches - l0;
   }
   if ((rc = count_into(e, stats))) return rc;
+  if ((rc = check_support(e))) return rc;
   return collect_timing(e);
 }
 
@@ -798,6 +960,27 @@ int swsph_restore_state(swsph_engine *e) {
   for (size_t k = 0; k < dst.size(); k++)
     HIPCHK(hipMemcpyAsync(dst[k], e->saved[k], sizeof(float) * n,
                           hipMemcpyDeviceToDevice, e->st));
+  return 0;
+}
+
+int swsph_set_chunk(swsph_engine *e, int which, int C) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (which < 0 || which > 2) return fail(1, "loop index %d out of range", which);
+  if (C != 8 && C != 16 && C != 32 && C != 64)
+    return fail(1, "chunk capacity %d not in {8, 16, 32, 64}", C);
+  e->loop_c[which] = C;
+  if (!e->cell_start.empty()) {
+    int rc = 0;
+    if (!chunk_table(e, C, &rc)) return rc;
+  }
+  return 0;
+}
+
+int swsph_get_chunk(swsph_engine *e, int which, int *C) {
+  if (!e || !C) return fail(1, "null argument");
+  if (which < 0 || which > 2) return fail(1, "loop index %d out of range", which);
+  *C = e->loop_c[which];
   return 0;
 }
 

@@ -33,6 +33,25 @@ struct __attribute__((aligned(16))) ForceRec {
   float pad[2];
 };
 
+// Position of a candidate for the pair test, and its search radius
+// (gamma_K h)^2 (read by the force test).
+struct __attribute__((aligned(16))) PosRec {
+  double x, y, z;
+  float hg2, pad;
+};
+
+// Chunks: runs of <= C consecutive particles of one cell (Morton order inside
+// the cell), with their bounding box and largest search radius.
+struct ChunkArgs {
+  const int *chunk_start;   // [nchunks + 1] first particle of each chunk
+  const int *cell_chunk;    // [ncells + 1] first chunk of each cell
+  const int *chunk_cell;    // [nchunks] cell of each chunk
+  const double *bbox;       // [6 nchunks] lo x,y,z, hi x,y,z
+  const float *hg2max;      // [nchunks] max (gamma_K h)^2
+  int nchunks;
+  int C;                    // chunk capacity (8, 16, 32 or 64)
+};
+
 // Structure-of-arrays particle store in HBM, sorted by top-level cell.
 struct DevParts {
   double *px, *py, *pz;
@@ -59,6 +78,7 @@ struct DevParts {
   DensRec *rd;
   GradRec *rg;
   ForceRec *rf;
+  PosRec *pos;
 };
 
 struct GridDesc {
@@ -93,10 +113,19 @@ enum class LoopKind { Density, Gradient, Force };
 
 namespace sphk {
 hipError_t launch_stencil(LoopKind kind, bool subset, const DevParts &d,
-                          const LoopArgs &la, const int4 *tiles, int ntiles,
+                          const LoopArgs &la, const ChunkArgs &ca,
                           hipStream_t st);
 hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
-                       const LoopArgs &la, const TaskArgs &ta, hipStream_t st);
+                       const LoopArgs &la, const ChunkArgs &ca,
+                       const TaskArgs &ta, hipStream_t st);
+// Positions / search radii of the candidate test for particles [lo, hi).
+hipError_t launch_build_pos(const DevParts &d, int lo, int hi, hipStream_t st);
+hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t st);
+// Bounding boxes and largest search radius of chunks [c0, c1); the largest
+// search radius over them is max-reduced into *hg2_all (float bits).
+hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
+                               int c0, int c1, double *bbox, float *hg2max,
+                               unsigned int *hg2_all, hipStream_t st);
 hipError_t launch_init_density(const DevParts &d, int lo, int hi,
                                int max_active_bin, float h_max, hipStream_t st);
 hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,

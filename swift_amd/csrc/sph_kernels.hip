@@ -20,12 +20,49 @@ using namespace sphl;
 
 namespace {
 
-// Chunks of 64 candidates tested before each interaction phase.
-constexpr int kWinDensity = 2;
-constexpr int kWinGradient = 2;
-constexpr int kWinForce = 2;
-
 // ------------------------------------------------------ neighbour records --
+__global__ void k_build_pos(DevParts d, int lo, int hi) {
+  const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= hi) return;
+  const bool inh = d.tb[i] == kBinInhibited;
+  PosRec p;
+  p.x = inh ? kFar : d.px[i];
+  p.y = inh ? kFar : d.py[i];
+  p.z = inh ? kFar : d.pz[i];
+  p.hg2 = inh ? -1.f : search_radius2(d.h[i]);
+  p.pad = 0.f;
+  d.pos[i] = p;
+}
+
+// Bounding box (non-inhibited particles) and largest search radius of each
+// chunk in [c0, c1); an empty chunk gets an inverted box that no test passes.
+__global__ void k_chunk_bounds(DevParts d, const int *chunk_start, int c0,
+                               int c1, double *bbox, float *hg2max,
+                               unsigned int *hg2_all) {
+  const int k = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  float hm = 0.f;
+  if (k < c1) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = chunk_start[k]; i < chunk_start[k + 1]; i++) {
+      if (d.tb[i] == kBinInhibited) continue;
+      const double x[3] = {d.px[i], d.py[i], d.pz[i]};
+      for (int a = 0; a < 3; a++) {
+        lo[a] = fmin(lo[a], x[a]);
+        hi[a] = fmax(hi[a], x[a]);
+      }
+      hm = fmaxf(hm, search_radius2(d.h[i]));
+    }
+    for (int a = 0; a < 3; a++) {
+      bbox[6 * k + a] = lo[a];
+      bbox[6 * k + 3 + a] = hi[a];
+    }
+    hg2max[k] = hm;
+  }
+  // positive floats order like their bit patterns
+  for (int off = 32; off > 0; off >>= 1) hm = fmaxf(hm, __shfl_down(hm, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(hg2_all, __float_as_uint(hm));
+}
+
 __global__ void k_build_dens_rec(DevParts d, int lo, int hi) {
   const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= hi) return;
@@ -383,56 +420,102 @@ namespace sphk {
 
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
-hipError_t launch_stencil(LoopKind kind, bool subset, const DevParts &d,
-                          const LoopArgs &la, const int4 *tiles, int ntiles,
-                          hipStream_t st) {
-  if (ntiles <= 0) return hipSuccess;
-  const int per = (ntiles + 7) >> 3;
-  dim3 grid(per * 8), block(64);
-  switch (kind) {
-    case LoopKind::Density:
-      if (subset)
-        hipLaunchKernelGGL((k_loop_stencil<DensityLoop, true, kWinDensity>), grid, block, 0,
-                           st, d, la, (const DensRec *)d.rd, tiles, ntiles);
-      else
-        hipLaunchKernelGGL((k_loop_stencil<DensityLoop, false, kWinDensity>), grid, block, 0,
-                           st, d, la, (const DensRec *)d.rd, tiles, ntiles);
-      break;
-    case LoopKind::Gradient:
-      hipLaunchKernelGGL((k_loop_stencil<GradientLoop, false, kWinGradient>), grid, block, 0,
-                         st, d, la, (const GradRec *)d.rg, tiles, ntiles);
-      break;
-    case LoopKind::Force:
-      hipLaunchKernelGGL((k_loop_stencil<ForceLoop, false, kWinForce>), grid, block, 0,
-                         st, d, la, (const ForceRec *)d.rf, tiles, ntiles);
-      break;
+template <class L, bool SUB, int C>
+void stencil_c(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
+               const typename L::Rec *rec, hipStream_t st) {
+  const int per = (ca.nchunks + 7) >> 3;
+  hipLaunchKernelGGL((k_loop_stencil<L, C, SUB>), dim3(per * 8), dim3(64), 0,
+                     st, d, la, ca, rec, (const PosRec *)d.pos);
+}
+
+template <class L, bool SUB>
+hipError_t stencil_any(const DevParts &d, const LoopArgs &la,
+                       const ChunkArgs &ca, const typename L::Rec *rec,
+                       hipStream_t st) {
+  switch (ca.C) {
+    case 8: stencil_c<L, SUB, 8>(d, la, ca, rec, st); break;
+    case 16: stencil_c<L, SUB, 16>(d, la, ca, rec, st); break;
+    case 32: stencil_c<L, SUB, 32>(d, la, ca, rec, st); break;
+    case 64: stencil_c<L, SUB, 64>(d, la, ca, rec, st); break;
+    default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
-                       const LoopArgs &la, const TaskArgs &ta, hipStream_t st) {
+template <class L, bool SUB, int C>
+void task_c(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
+            const typename L::Rec *rec, const TaskArgs &ta, int nb,
+            hipStream_t st) {
+  hipLaunchKernelGGL((k_loop_task<L, C, SUB>), dim3(nb), dim3(64), 0, st, d, la,
+                     ca, rec, (const PosRec *)d.pos, ta);
+}
+
+template <class L, bool SUB>
+hipError_t task_any(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
+                    const typename L::Rec *rec, const TaskArgs &ta,
+                    hipStream_t st) {
   const int nb = ta.ntiles[0] + ta.ntiles[1];
   if (nb <= 0) return hipSuccess;
-  dim3 grid(nb), block(64);
+  switch (ca.C) {
+    case 8: task_c<L, SUB, 8>(d, la, ca, rec, ta, nb, st); break;
+    case 16: task_c<L, SUB, 16>(d, la, ca, rec, ta, nb, st); break;
+    case 32: task_c<L, SUB, 32>(d, la, ca, rec, ta, nb, st); break;
+    case 64: task_c<L, SUB, 64>(d, la, ca, rec, ta, nb, st); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_stencil(LoopKind kind, bool subset, const DevParts &d,
+                          const LoopArgs &la, const ChunkArgs &ca,
+                          hipStream_t st) {
+  if (ca.nchunks <= 0) return hipSuccess;
   switch (kind) {
     case LoopKind::Density:
-      if (subset)
-        hipLaunchKernelGGL((k_loop_task<DensityLoop, true, kWinDensity>), grid, block, 0, st,
-                           d, la, (const DensRec *)d.rd, ta);
-      else
-        hipLaunchKernelGGL((k_loop_task<DensityLoop, false, kWinDensity>), grid, block, 0,
-                           st, d, la, (const DensRec *)d.rd, ta);
-      break;
+      return subset ? stencil_any<DensityLoop, true>(d, la, ca, d.rd, st)
+                    : stencil_any<DensityLoop, false>(d, la, ca, d.rd, st);
     case LoopKind::Gradient:
-      hipLaunchKernelGGL((k_loop_task<GradientLoop, false, kWinGradient>), grid, block, 0, st,
-                         d, la, (const GradRec *)d.rg, ta);
-      break;
+      return stencil_any<GradientLoop, false>(d, la, ca, d.rg, st);
     case LoopKind::Force:
-      hipLaunchKernelGGL((k_loop_task<ForceLoop, false, kWinForce>), grid, block, 0, st, d,
-                         la, (const ForceRec *)d.rf, ta);
-      break;
+      return stencil_any<ForceLoop, false>(d, la, ca, d.rf, st);
   }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
+                       const LoopArgs &la, const ChunkArgs &ca,
+                       const TaskArgs &ta, hipStream_t st) {
+  switch (kind) {
+    case LoopKind::Density:
+      return subset ? task_any<DensityLoop, true>(d, la, ca, d.rd, ta, st)
+                    : task_any<DensityLoop, false>(d, la, ca, d.rd, ta, st);
+    case LoopKind::Gradient:
+      return task_any<GradientLoop, false>(d, la, ca, d.rg, ta, st);
+    case LoopKind::Force:
+      return task_any<ForceLoop, false>(d, la, ca, d.rf, ta, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_int, dim3(nblk(n, 64)), dim3(64), 0, st, (int *)p,
+                     n);
+  return hipGetLastError();
+}
+
+hipError_t launch_build_pos(const DevParts &d, int lo, int hi, hipStream_t st) {
+  if (hi <= lo) return hipSuccess;
+  hipLaunchKernelGGL(k_build_pos, dim3(nblk(hi - lo, 256)), dim3(256), 0, st, d,
+                     lo, hi);
+  return hipGetLastError();
+}
+
+hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
+                               int c0, int c1, double *bbox, float *hg2max,
+                               unsigned int *hg2_all, hipStream_t st) {
+  if (c1 <= c0) return hipSuccess;
+  hipLaunchKernelGGL(k_chunk_bounds, dim3(nblk(c1 - c0, 256)), dim3(256), 0, st,
+                     d, chunk_start, c0, c1, bbox, hg2max, hg2_all);
   return hipGetLastError();
 }
 

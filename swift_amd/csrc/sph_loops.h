@@ -63,8 +63,27 @@ struct DensityLoop {
     d.rotz[i] = a.rz;
     d.nd[i] = a.n;
   }
-  // The LDS stage holds whole records: the interaction phase reads j there.
-  static constexpr bool kStageFull = true;
+  static constexpr bool kSymmetric = false;  // search radius is H_i only
+  static __device__ __forceinline__ void identity(Acc &a) {
+    a.rho = a.rho_dh = a.wc = a.wc_dh = a.divv = a.rx = a.ry = a.rz = 0.f;
+    a.n = 0;
+  }
+  // Sum the partial sums of the S streams of a particle.
+  template <int S>
+  static __device__ __forceinline__ void combine(Acc &a) {
+#pragma unroll
+    for (int off = 64 / S; off < 64; off <<= 1) {
+      a.rho += __shfl_xor(a.rho, off);
+      a.rho_dh += __shfl_xor(a.rho_dh, off);
+      a.wc += __shfl_xor(a.wc, off);
+      a.wc_dh += __shfl_xor(a.wc_dh, off);
+      a.divv += __shfl_xor(a.divv, off);
+      a.rx += __shfl_xor(a.rx, off);
+      a.ry += __shfl_xor(a.ry, off);
+      a.rz += __shfl_xor(a.rz, off);
+      a.n += __shfl_xor(a.n, off);
+    }
+  }
   // Candidate test: r < H_i (DOPAIR1 :1386, DOSELF1 :2396).
   template <class R>
   static __device__ __forceinline__ bool test(float r2, const I &s, const R *) {
@@ -134,7 +153,24 @@ struct GradientLoop {
     d.avmax[i] = a.avmax;
     d.ng[i] = a.n;
   }
-  static constexpr bool kStageFull = true;
+  static constexpr bool kSymmetric = false;
+  // v_sig >= 0 and alpha >= 0, so 0 is the identity of both maxima.
+  static __device__ __forceinline__ void identity(Acc &a) {
+    a.vsig = 0.f;
+    a.lapu = 0.f;
+    a.avmax = 0.f;
+    a.n = 0;
+  }
+  template <int S>
+  static __device__ __forceinline__ void combine(Acc &a) {
+#pragma unroll
+    for (int off = 64 / S; off < 64; off <<= 1) {
+      a.vsig = fmaxf(a.vsig, __shfl_xor(a.vsig, off));
+      a.lapu += __shfl_xor(a.lapu, off);
+      a.avmax = fmaxf(a.avmax, __shfl_xor(a.avmax, off));
+      a.n += __shfl_xor(a.n, off);
+    }
+  }
   template <class R>
   static __device__ __forceinline__ bool test(float r2, const I &s, const R *) {
     return r2 < s.hg2;
@@ -212,9 +248,25 @@ struct ForceLoop {
     d.nf[i] = a.n;
     d.mintb[i] = (int8_t)a.mintb;
   }
-  // The record is large: the LDS stage holds only the position and search
-  // radius (PosRec) and the interaction phase gathers j from L1/L2.
-  static constexpr bool kStageFull = false;
+  static constexpr bool kSymmetric = true;  // r < max(H_i, H_j)
+  static __device__ __forceinline__ void identity(Acc &a) {
+    a.ax = a.ay = a.az = a.udt = a.hdt = 0.f;
+    a.n = 0;
+    a.mintb = kNumTimeBins + 1;
+  }
+  template <int S>
+  static __device__ __forceinline__ void combine(Acc &a) {
+#pragma unroll
+    for (int off = 64 / S; off < 64; off <<= 1) {
+      a.ax += __shfl_xor(a.ax, off);
+      a.ay += __shfl_xor(a.ay, off);
+      a.az += __shfl_xor(a.az, off);
+      a.udt += __shfl_xor(a.udt, off);
+      a.hdt += __shfl_xor(a.hdt, off);
+      a.n += __shfl_xor(a.n, off);
+      a.mintb = min(a.mintb, __shfl_xor(a.mintb, off));
+    }
+  }
   // Symmetric criterion r < max(H_i, H_j) (DOPAIR2 :1915-2167, DOSELF2).
   template <class R>
   static __device__ __forceinline__ bool test(float r2, const I &s, const R *j) {

@@ -194,6 +194,20 @@ class Engine:
         check(lib().swsph_pass_batched(self._h, C.byref(s)), "swsph_pass_batched")
         return s.as_dict()
 
+    LOOPS = {"density": 0, "gradient": 1, "force": 2}
+
+    def set_chunk(self, loop: str, C: int) -> None:
+        """Tile shape of one neighbour loop: chunks of C particles, 64 / C
+        candidate streams per particle (C in 8, 16, 32, 64)."""
+        check(lib().swsph_set_chunk(self._h, self.LOOPS[loop], int(C)),
+              "swsph_set_chunk")
+
+    def get_chunk(self, loop: str) -> int:
+        c = C.c_int()
+        check(lib().swsph_get_chunk(self._h, self.LOOPS[loop], C.byref(c)),
+              "swsph_get_chunk")
+        return c.value
+
     def save_state(self) -> None:
         check(lib().swsph_save_state(self._h), "swsph_save_state")
 

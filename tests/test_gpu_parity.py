@@ -130,6 +130,34 @@ def test_time_base_alpha_evolution(orc):
     assert np.any(np.abs(res["alpha_visc"] - 0.1) > 1e-3)
 
 
+@pytest.mark.parametrize("C", [8, 16, 32, 64])
+def test_chunk_shapes(orc, C):
+    """Every tile shape (chunks of C particles, 64 / C candidate streams)
+    gives the oracle's neighbour sets and values."""
+    from swift_amd import Engine, HydroProps, ics
+    st = ics.modulated_box(16, seed=12, amp=0.5)
+    props = HydroProps(ncell=(4, 4, 4))
+    with Engine(props) as eng:
+        for loop in ("density", "gradient", "force"):
+            eng.set_chunk(loop, C)
+        eng.set_particles(st)
+        eng.pass_batched()
+        res = eng.results()
+    _check(orc, props, st, res)
+
+
+def test_clustered_small(orc):
+    """Plummer spheres on a uniform background: h spread over two decades,
+    cells of very uneven occupancy, chunk box culling doing real work."""
+    from swift_amd import HydroProps, ics
+    st = ics.clustered_box(12000, seed=3, n_spheres=6, a_min=0.01, a_max=0.04)
+    props = HydroProps(ncell=(4, 4, 4))
+    res, stats = _run(props, st)
+    assert res["h"].max() / res["h"].min() > 10
+    _check(orc, props, st, res)
+    _h_converged(orc, props, st, res)
+
+
 def test_config0_full(orc):
     """32^3 jittered lattice on 8^3 top-level cells, all active."""
     from swift_amd import HydroProps, ics
