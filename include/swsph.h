@@ -91,6 +91,8 @@ typedef struct swsph_pass_stats {
   int64_t n_gradient;
   int64_t n_force;
   int64_t launches;              /* kernel launches issued by the pass */
+  int64_t list_rebuilt;          /* batched: particles re-listed by re-runs */
+  int64_t list_fallback;         /* batched: gradient via the chunk loop */
 } swsph_pass_stats;
 
 const char *swsph_last_error(void);
@@ -162,6 +164,18 @@ int swsph_restore_state(swsph_engine *e);
  * summation order. */
 int swsph_set_chunk(swsph_engine *e, int which, int C);
 int swsph_get_chunk(swsph_engine *e, int which, int *C);
+
+/* Neighbour lists of the batched pass.  With lists enabled, the first
+ * density round keeps, per particle, the candidates within `skin` times its
+ * search radius (`slots` entries per candidate stream, 4 streams); the
+ * ghost's density re-runs and the gradient loop then walk those lists
+ * instead of the 27-cell stencil, and a particle whose radius outgrows its
+ * list (or whose list overflowed) goes through the stencil loop again.
+ * The role of SWIFT's cell sorts and sub-cell recursion
+ * (src/runner_sort.c, src/runner_doiact_functions_hydro.h) in pruning
+ * candidates; neighbour sets and results do not depend on the setting.
+ * Defaults: disabled, skin 1.2, 48 slots. */
+int swsph_set_lists(swsph_engine *e, int enable, float skin, int slots);
 
 /* Interaction counters summed over active particles (device reduction). */
 int swsph_count_interactions(swsph_engine *e, int64_t out3[3]);

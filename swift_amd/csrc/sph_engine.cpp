@@ -66,7 +66,7 @@ struct swsph_engine {
   int *d_cell_start = nullptr;
   int4 *d_tiles = nullptr;
   int ntiles = 0;
-  int *d_redo = nullptr;          // redo counter
+  int *d_redo = nullptr;          // ghost counters (see launch_ghost)
   int *h_redo = nullptr;          // pinned readback
   unsigned long long *d_cnt = nullptr;
   unsigned long long *h_cnt = nullptr;
@@ -94,6 +94,14 @@ struct swsph_engine {
   int loop_c[3] = {16, 16, 16};
   unsigned int *d_hg2_all = nullptr;  // max (gamma h)^2 over all chunks
   unsigned int *h_hg2_all = nullptr;
+  // Neighbour lists of the batched pass (allocated on first use).
+  bool use_lists = false;
+  float list_skin = 1.2f;  // list radius / search radius
+  int list_slots = 48;     // slots per stream
+  NbrList nl{};
+  size_t nl_elems = 0;
+  int64_t list_rebuilt = 0;   // particles re-listed in the last pass
+  int64_t list_fallback = 0;  // gradient fall-backs in the last pass
 };
 
 namespace {
@@ -114,6 +122,7 @@ LoopArgs loop_args(const swsph_engine *e) {
   LoopArgs la;
   la.grid = e->grid;
   la.cell_start = e->d_cell_start;
+  la.flag = e->d.redo;
   la.max_active_bin = e->props.max_active_bin;
   return la;
 }
@@ -145,10 +154,14 @@ void free_dev(swsph_engine *e) {
                   e->d.avmax, e->d.ng, e->d.alpha, e->d.adiff, e->d.divv_prev,
                   e->d.divv_dt, e->d.ax, e->d.ay, e->d.az, e->d.udt, e->d.hdt,
                   e->d.nf, e->d.mintb, e->d.rd, e->d.rg, e->d.rf, e->d.pos,
+                  e->d.r2b, e->d.lfb, e->d.sel, e->nl.idx, e->nl.cnt,
                   e->d_cell_start, e->d_tiles};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   e->d = DevParts{};
+  e->nl.idx = nullptr;
+  e->nl.cnt = nullptr;
+  e->nl_elems = 0;
   e->d_cell_start = nullptr;
   e->d_tiles = nullptr;
   free_saved(e);
@@ -250,7 +263,8 @@ int alloc_dev(swsph_engine *e, size_t n) {
                    &d.lapu, &d.avmax, &d.alpha, &d.adiff, &d.divv_prev,
                    &d.divv_dt, &d.ax, &d.ay, &d.az, &d.udt, &d.hdt};
   for (float **p : f32) HIPCHK(dalloc(p, n));
-  int **i32[] = {&d.nd, &d.redo, &d.ng, &d.nf};
+  HIPCHK(dalloc(&d.r2b, n));
+  int **i32[] = {&d.nd, &d.redo, &d.ng, &d.nf, &d.lfb, &d.sel};
   for (int **p : i32) HIPCHK(dalloc(p, n));
   HIPCHK(dalloc(&d.tb, n));
   HIPCHK(dalloc(&d.mintb, n));
@@ -264,6 +278,7 @@ int alloc_dev(swsph_engine *e, size_t n) {
   for (float **p : f32) HIPCHK(hipMemsetAsync(*p, 0, sizeof(float) * nn, e->st));
   for (int **p : i32) HIPCHK(hipMemsetAsync(*p, 0, sizeof(int) * nn, e->st));
   HIPCHK(hipMemsetAsync(d.mintb, 0, nn, e->st));
+  HIPCHK(sphk::launch_fill_f32(d.r2b, -1.f, (int)nn, e->st));  // no lists yet
   // The uploads that follow use blocking copies on the null stream, which
   // does not order against this non-blocking stream.
   HIPCHK(hipStreamSynchronize(e->st));
@@ -315,26 +330,87 @@ hipEvent_t next_event(swsph_engine *e) {
   return e->ev_pool[e->ev_used++];
 }
 
-// Launch a batched loop, bracketed by timing events when enabled.
-int stencil(swsph_engine *e, LoopKind k, bool subset) {
+// Timing events around the launches of one loop kind, when enabled.
+int time_begin(swsph_engine *e, hipEvent_t *a) {
+  *a = nullptr;
+  if (!e->timing) return 0;
+  *a = next_event(e);
+  if (!*a) return fail(2, "hipEventCreate failed");
+  HIPCHK(hipEventRecord(*a, e->st));
+  return 0;
+}
+
+int time_end(swsph_engine *e, LoopKind k, hipEvent_t a) {
+  if (!e->timing) return 0;
+  hipEvent_t b = next_event(e);
+  if (!b) return fail(2, "hipEventCreate failed");
+  HIPCHK(hipEventRecord(b, e->st));
+  e->ev_pending.push_back({(int)k, {a, b}});
+  return 0;
+}
+
+// Launch a batched loop; a subset loop takes the particles with flag[i].
+int stencil(swsph_engine *e, LoopKind k, bool subset,
+            const int *flag = nullptr) {
   int rc = 0;
   const swsph_engine::ChunkTable *t = chunk_table(e, e->loop_c[(int)k], &rc);
   if (!t) return rc;
-  hipEvent_t a = nullptr, b = nullptr;
-  if (e->timing) {
-    a = next_event(e);
-    b = next_event(e);
-    if (!a || !b) return fail(2, "hipEventCreate failed");
-    HIPCHK(hipEventRecord(a, e->st));
-  }
-  HIPCHK(sphk::launch_stencil(k, subset, e->d, loop_args(e), chunk_args(t),
-                              e->st));
+  hipEvent_t a;
+  if ((rc = time_begin(e, &a))) return rc;
+  LoopArgs la = loop_args(e);
+  if (flag) la.flag = flag;
+  HIPCHK(sphk::launch_stencil(k, subset, e->d, la, chunk_args(t), e->st));
   e->launches++;
-  if (e->timing) {
-    HIPCHK(hipEventRecord(b, e->st));
-    e->ev_pending.push_back({(int)k, {a, b}});
-  }
+  return time_end(e, k, a);
+}
+
+// Neighbour-list storage for the current particle count and slot count.
+int ensure_lists(swsph_engine *e) {
+  const size_t npad = (size_t)std::max<int64_t>(1, e->n);
+  const size_t elems = (size_t)e->list_slots * npad * kListStreams;
+  if (e->nl.idx && e->nl_elems == elems) return 0;
+  if (e->nl.idx) (void)hipFree(e->nl.idx);
+  if (e->nl.cnt) (void)hipFree(e->nl.cnt);
+  e->nl.idx = nullptr;
+  e->nl.cnt = nullptr;
+  e->nl_elems = 0;
+  if (hipMalloc((void **)&e->nl.idx, sizeof(unsigned int) * elems) != hipSuccess ||
+      hipMalloc((void **)&e->nl.cnt, sizeof(int) * npad * kListStreams) !=
+          hipSuccess)
+    return fail(2, "allocation of neighbour lists (%zu entries) failed", elems);
+  e->nl_elems = elems;
+  e->nl.Ks = e->list_slots;
+  e->nl.npad = (int)npad;
+  e->nl.lam2 = e->list_skin * e->list_skin;
   return 0;
+}
+
+// Density loop that (re)builds neighbour lists: over all active particles,
+// or (subset) the ones flagged for another round whose list is too short.
+int build_lists(swsph_engine *e, bool subset) {
+  int rc = 0;
+  const swsph_engine::ChunkTable *t = chunk_table(e, kListChunk, &rc);
+  if (!t) return rc;
+  hipEvent_t a;
+  if ((rc = time_begin(e, &a))) return rc;
+  HIPCHK(sphk::launch_build_lists(subset, e->d, loop_args(e), chunk_args(t),
+                                  e->nl, e->st));
+  e->launches++;
+  return time_end(e, LoopKind::Density, a);
+}
+
+// A density / gradient loop over the neighbour lists: all active particles,
+// or the nsel particles of d.sel.
+int list_loop(swsph_engine *e, LoopKind k, bool compact, int nsel) {
+  int rc = 0;
+  const swsph_engine::ChunkTable *t = chunk_table(e, kListChunk, &rc);
+  if (!t) return rc;
+  hipEvent_t a;
+  if ((rc = time_begin(e, &a))) return rc;
+  HIPCHK(sphk::launch_list_loop(k, compact, e->d, loop_args(e), chunk_args(t),
+                                e->nl, nsel, e->st));
+  e->launches++;
+  return time_end(e, k, a);
 }
 
 int collect_timing(swsph_engine *e) {
@@ -405,11 +481,16 @@ void all_neighbours(const swsph_engine *e, int ci, std::vector<int> &out) {
   }
 }
 
-int read_redo(swsph_engine *e, int *count) {
-  HIPCHK(hipMemcpyAsync(e->h_redo, e->d_redo, sizeof(int),
+// Ghost counters: [0] particles flagged for another round, [1] of those
+// needing a new neighbour list, [2] converged without a usable list.
+int read_redo(swsph_engine *e, int *count, int *relist = nullptr,
+              int *nolist = nullptr) {
+  HIPCHK(hipMemcpyAsync(e->h_redo, e->d_redo, 3 * sizeof(int),
                         hipMemcpyDeviceToHost, e->st));
   HIPCHK(hipStreamSynchronize(e->st));
-  *count = *e->h_redo;
+  *count = e->h_redo[0];
+  if (relist) *relist = e->h_redo[1];
+  if (nolist) *nolist = e->h_redo[2];
   return 0;
 }
 
@@ -530,8 +611,8 @@ int swsph_engine_create(const swsph_props *props, swsph_engine **out) {
     delete e;
     return fail(2, "hipStreamCreate failed");
   }
-  if (hipMalloc((void **)&e->d_redo, sizeof(int)) != hipSuccess ||
-      hipHostMalloc((void **)&e->h_redo, sizeof(int), 0) != hipSuccess ||
+  if (hipMalloc((void **)&e->d_redo, 4 * sizeof(int)) != hipSuccess ||
+      hipHostMalloc((void **)&e->h_redo, 4 * sizeof(int), 0) != hipSuccess ||
       hipMalloc((void **)&e->d_cnt, 3 * sizeof(unsigned long long)) != hipSuccess ||
       hipHostMalloc((void **)&e->h_cnt, 3 * sizeof(unsigned long long), 0) !=
           hipSuccess ||
@@ -906,21 +987,48 @@ int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats) {
   HIPCHK(sphk::launch_init_density(e->d, 0, n, e->props.max_active_bin,
                                    e->props.h_max, e->st));
   e->launches++;
-  if ((rc = stencil(e, LoopKind::Density, false))) return rc;
+  const bool lists = e->use_lists;
+  e->list_rebuilt = 0;
+  e->list_fallback = 0;
+  if (lists) {
+    // First round: every active particle gets a fresh list.
+    if ((rc = ensure_lists(e)) || (rc = build_lists(e, false))) return rc;
+  } else if ((rc = stencil(e, LoopKind::Density, false))) {
+    return rc;
+  }
   int redo = 0, it = 0;
   for (; it < e->props.max_iter; it++) {
     HIPCHK(sphk::launch_ghost(e->d, ga, 0, n, e->d_redo, e->st));
     e->launches += 2;
-    if ((rc = read_redo(e, &redo))) return rc;
+    int relist = 0, nolist = 0;
+    if ((rc = read_redo(e, &redo, &relist, &nolist))) return rc;
+    e->list_fallback += nolist;
     if (redo == 0) break;
-    if ((rc = stencil(e, LoopKind::Density, true))) return rc;
+    if (lists) {
+      // Re-runs walk the lists of the flagged particles; the few whose
+      // radius outgrew their list go through the chunk loop again.
+      if ((rc = list_loop(e, LoopKind::Density, true, redo))) return rc;
+      if (relist > 0) {
+        e->list_rebuilt += relist;
+        if ((rc = build_lists(e, true))) return rc;
+      }
+    } else if ((rc = stencil(e, LoopKind::Density, true))) {
+      return rc;
+    }
   }
   if (redo > 0)
     return fail(3, "Smoothing length failed to converge on %d particles", redo);
   HIPCHK(sphk::launch_build_records(LoopKind::Gradient, e->d, 0, n, e->st));
   e->launches++;
   if ((rc = refresh_bounds(e, 0, e->ncells))) return rc;
-  if ((rc = stencil(e, LoopKind::Gradient, false))) return rc;
+  if (lists) {
+    if ((rc = list_loop(e, LoopKind::Gradient, false, 0))) return rc;
+    if (e->list_fallback > 0 &&
+        (rc = stencil(e, LoopKind::Gradient, true, e->d.lfb)))
+      return rc;
+  } else if ((rc = stencil(e, LoopKind::Gradient, false))) {
+    return rc;
+  }
   HIPCHK(sphk::launch_extra_ghost(e->d, ga, 0, n, e->st));
   HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, 0, n, e->st));
   e->launches += 2;
@@ -930,6 +1038,8 @@ int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats) {
   if (stats) {
     stats->ghost_rounds = it + 1;
     stats->launches = e->launches - l0;
+    stats->list_rebuilt = e->list_rebuilt;
+    stats->list_fallback = e->list_fallback;
   }
   if ((rc = count_into(e, stats))) return rc;
   if ((rc = check_support(e))) return rc;
@@ -974,6 +1084,24 @@ int swsph_set_chunk(swsph_engine *e, int which, int C) {
     int rc = 0;
     if (!chunk_table(e, C, &rc)) return rc;
   }
+  return 0;
+}
+
+int swsph_set_lists(swsph_engine *e, int enable, float skin, int slots) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (!(skin >= 1.f && skin <= 2.f))
+    return fail(1, "list skin %g outside [1, 2]", (double)skin);
+  if (slots < 1 || slots > 4096)
+    return fail(1, "list slots %d outside [1, 4096]", slots);
+  e->use_lists = enable != 0;
+  e->list_skin = skin;
+  if (slots != e->list_slots) {
+    e->list_slots = slots;
+    e->nl_elems = 0;  // reallocated on next use
+  }
+  e->nl.lam2 = skin * skin;
+  e->nl.Ks = slots;
   return 0;
 }
 

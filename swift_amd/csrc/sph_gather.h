@@ -8,18 +8,27 @@
 // bounding box and the largest search radius of its particles.
 //
 // One wave64 owns one i-chunk: lane = g + C * s holds particle g of the
-// chunk and is "stream" s of S = 64 / C streams.  For each of the 27
-// neighbour cells the wave tests the cell's chunks against the i-chunk box
-// (one chunk per lane, then a ballot), and walks the surviving j-chunks in
-// rows of S particles: stream s tests candidate (row start + s) against its
-// particle g.  A hit sets one bit of the lane's row mask; the row table in
-// LDS remembers where each row starts and which image translation it used.
-// When 128 rows are tested, every lane walks its own set bits, gathers its
-// j record and runs the interaction body, so the expensive bodies run dense
-// although only a few per cent of the candidates are neighbours.  At the end
-// the S partial accumulators of a particle are combined with cross-lane
-// shuffles and written once.  No atomics: every directed interaction is
-// evaluated by one lane.
+// chunk and is "stream" s of S = 64 / C streams.  The wave first builds its
+// neighbour table (the 27 neighbour cells, their images and chunk ranges:
+// one lane per cell, all loads in flight together), then culls the flat list
+// of candidate chunks 64 at a time (one chunk per lane: box distance against
+// the largest search radius, then a ballot) and compacts the survivors into
+// LDS.  Survivors are staged 64 particles at a time into LDS and tested in
+// rows of S: stream s tests candidate (row start + s) against its particle
+// g.  A hit sets one bit of the lane's 64-bit row mask; a per-chunk table in
+// LDS remembers where each chunk starts and which image it uses.  After 64
+// rows every lane walks its own set bits, gathers its j record and runs the
+// interaction body, so the expensive bodies run dense although only a few
+// per cent of the candidates are neighbours.  At the end the S partial
+// accumulators of a particle are combined with cross-lane shuffles and
+// written once.  No atomics: every directed interaction is evaluated by one
+// lane.
+//
+// The BUILD form (density loop, C = kListChunk) tests against the skinned
+// radius lam2 * (gamma_K h_i)^2 instead, appends every hit to the lane's
+// stream of the particle's neighbour list, and interacts with the hits
+// inside the plain radius.  Per stream, hits are visited in the same order
+// as by the plain form, so both give bit-identical sums.
 #pragma once
 
 #include "sph_loops.h"
@@ -30,8 +39,6 @@ namespace sphl {
 __constant__ const int kStencilOrder[27] = {13, 0,  26, 1,  25, 2,  24, 3,  23,
                                             4,  22, 5,  21, 6,  20, 7,  19, 8,
                                             18, 9,  17, 10, 16, 11, 15, 12, 14};
-
-constexpr int kRows = 128;  // rows tested between interaction phases
 
 __device__ __forceinline__ void cell_coords(int cid, const GridDesc &g,
                                             int &cx, int &cy, int &cz) {
@@ -70,89 +77,86 @@ __device__ __forceinline__ int xcd_remap(int b, int n) {
   return (b & 7) * per + (b >> 3);
 }
 
-template <int S>
-__device__ __forceinline__ float reduce_max_streams(float v) {
-  // lanes g, g + C, g + 2C, ... hold the S streams of particle g
-#pragma unroll
-  for (int off = 64 / S; off < 64; off <<= 1) v = fmaxf(v, __shfl_xor(v, off));
-  return v;
-}
-
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) v = fmaxf(v, __shfl_xor(v, off));
   return v;
 }
 
-// Row state of the wave: 4 x 32-bit hit masks per lane, the row counter,
-// and the row table in registers: lane k of `ra` (`rb`) describes row k
-// (64 + k) as (neighbour slot q << 27) | first particle of the row.
-struct Rows {
-  unsigned int m0, m1, m2, m3;
-  unsigned int ra, rb;
-  int n;  // rows recorded (uniform)
-};
-
 constexpr unsigned int kRowMask = (1u << 27) - 1u;
 
-// Interaction phase.  Called from converged code: the row table moves from
-// registers to LDS first, because the per-lane loop below is divergent and a
-// cross-lane read from a lane that has left the loop is not defined.
-template <class L, int C>
+// Image code 9 (sx + 1) + 3 (sy + 1) + (sz + 1) of a translation.
+__device__ __forceinline__ unsigned int image_code(double tx, double ty,
+                                                   double tz) {
+  const int sx = tx < 0.0 ? 0 : (tx > 0.0 ? 2 : 1);
+  const int sy = ty < 0.0 ? 0 : (ty > 0.0 ? 2 : 1);
+  const int sz = tz < 0.0 ? 0 : (tz > 0.0 ? 2 : 1);
+  return (unsigned int)(9 * sx + 3 * sy + sz);
+}
+
+// LDS of one wave: the neighbour table, the compacted surviving chunks, the
+// staged candidate positions and the chunk table of the current 64 rows.
+template <int C>
+struct GatherLds {
+  static constexpr int S = 64 / C;
+  static constexpr int RPC = C / S > 0 ? C / S : 1;  // rows per chunk
+  static constexpr int SLOTS = 64 / RPC;             // chunks per 64 rows
+  PosRec lpos[64];
+  double ttab[27 * 3];
+  unsigned int tcode[27];
+  int qk0[27], qpre[28], qj0[27], qj1[27], qcell[27];
+  unsigned int plist[64];
+  unsigned int cinfo[SLOTS];
+};
+
+// Interaction phase over the rows recorded in the lanes' masks.  Called from
+// converged code; the per-lane loop below is divergent, so everything it
+// shares is in LDS.
+template <class L, int C, bool BUILD>
 __device__ __forceinline__ void interact_rows(
-    const typename L::Rec *__restrict__ rec, const double *ttab,
-    unsigned int *row_info, const typename L::I &st, int s, Rows &r,
-    typename L::Acc &acc) {
-  __syncthreads();
-  row_info[threadIdx.x] = r.ra;
-  row_info[64 + threadIdx.x] = r.rb;
-  __syncthreads();
-  unsigned int m0 = r.m0, m1 = r.m1, m2 = r.m2, m3 = r.m3;
-  for (;;) {
-    int row;
-    if (m0) {
-      row = __builtin_ctz(m0);
-      m0 &= m0 - 1;
-    } else if (m1) {
-      row = 32 + __builtin_ctz(m1);
-      m1 &= m1 - 1;
-    } else if (m2) {
-      row = 64 + __builtin_ctz(m2);
-      m2 &= m2 - 1;
-    } else if (m3) {
-      row = 96 + __builtin_ctz(m3);
-      m3 &= m3 - 1;
-    } else {
-      break;
-    }
-    const unsigned int info = row_info[row];
-    const int j = (int)(info & kRowMask) + s;
-    const int tc = (int)(info >> 27);
-    const double xs = st.x - ttab[3 * tc];
-    const double ys = st.y - ttab[3 * tc + 1];
-    const double zs = st.z - ttab[3 * tc + 2];
+    const typename L::Rec *__restrict__ rec, const GatherLds<C> &sh,
+    const typename L::I &st, int s, unsigned long long m,
+    typename L::Acc &acc, const NbrList &nl, int i, int &ncnt) {
+  constexpr int S = GatherLds<C>::S, RPC = GatherLds<C>::RPC;
+  while (m) {
+    const int row = __builtin_ctzll(m);
+    m &= m - 1ull;
+    const unsigned int info = sh.cinfo[row / RPC];
+    const int j = (int)(info & kRowMask) + (row % RPC) * S + s;
+    const int q = (int)(info >> 27);
+    const double xs = st.x - sh.ttab[3 * q];
+    const double ys = st.y - sh.ttab[3 * q + 1];
+    const double zs = st.z - sh.ttab[3 * q + 2];
     const typename L::Rec &jr = rec[j];
     float dx, dy, dz;
     const float r2 = pair_r2(xs, ys, zs, jr.x, jr.y, jr.z, dx, dy, dz);
-    L::interact(r2, dx, dy, dz, st, jr, acc);
+    if (BUILD) {
+      if (ncnt < nl.Ks)
+        nl.idx[((size_t)ncnt * nl.npad + i) * kListStreams + s] =
+            (sh.tcode[q] << 27) | (unsigned int)j;
+      ncnt++;
+      if (L::test(r2, st, &jr)) L::interact(r2, dx, dy, dz, st, jr, acc);
+    } else {
+      L::interact(r2, dx, dy, dz, st, jr, acc);
+    }
   }
-  r.m0 = r.m1 = r.m2 = r.m3 = 0u;
-  r.n = 0;
 }
 
 // One i-chunk against a list of neighbour cells.  `stencil` selects the 27
 // neighbours of the i-chunk's cell; otherwise the single cell `cj0` with
 // translation (tx0, ty0, tz0).
-template <class L, int C, bool SUBSET>
+template <class L, int C, bool SUBSET, bool BUILD>
 __device__ __forceinline__ void run_chunk(
     const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
     const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
-    int ich, bool stencil, int cj0, double tx0, double ty0, double tz0) {
-  constexpr int S = 64 / C;           // candidate streams per particle
-  constexpr int RPC = C / S > 0 ? C / S : 1;  // rows of a full chunk
-  __shared__ PosRec lpos[64];
-  __shared__ double ttab[27 * 3];
-  __shared__ unsigned int row_info[kRows];
+    int ich, bool stencil, int cj0, double tx0, double ty0, double tz0,
+    const NbrList &nl) {
+  using Sh = GatherLds<C>;
+  constexpr int S = Sh::S, RPC = Sh::RPC;
+  constexpr int G = 64 / C;  // chunks per staging group
+  static_assert(!BUILD || (C == kListChunk && !L::kSymmetric),
+                "lists are built by the density loop in list tiles");
+  __shared__ Sh sh;
   const int lane = threadIdx.x;
   const int g = lane % C;
   const int s = lane / C;
@@ -161,7 +165,9 @@ __device__ __forceinline__ void run_chunk(
   bool act = i < i1;
   if (act) {
     act = is_active(d.tb[i], la.max_active_bin);
-    if (SUBSET) act = act && d.redo[i];
+    if (SUBSET) act = act && la.flag[i];
+    // rebuild only the lists that do not cover the current radius
+    if (BUILD && SUBSET) act = act && !(search_radius2(d.h[i]) <= d.r2b[i]);
   }
   if (!__any(act)) return;  // uniform
   typename L::I st;
@@ -176,103 +182,154 @@ __device__ __forceinline__ void run_chunk(
     L::load_acc(d, i, acc);
   else
     L::identity(acc);
-  // i-chunk box and the largest search radius of its active particles
+  // squared radius of the candidate test (skinned when building lists)
+  const float rt2 = BUILD ? nl.lam2 * st.hg2 : st.hg2;
+  const float hg2i = wave_max(act ? rt2 : 0.f);
+  int ncnt = 0;  // list entries found by this lane (BUILD)
+
+  // Neighbour table: lane q < nq describes neighbour q.
+  const int cell_i = ca.chunk_cell[ich];
+  const int nq = stencil ? 27 : 1;
+  int nchq = 0;
+  if (lane < nq) {
+    double tx = tx0, ty = ty0, tz = tz0;
+    int cj = cj0;
+    if (stencil) {
+      int cx, cy, cz;
+      cell_coords(cell_i, la.grid, cx, cy, cz);
+      cj = neighbour(la.grid, cx, cy, cz, kStencilOrder[lane], tx, ty, tz);
+    }
+    const int k0 = ca.cell_chunk[cj], k1 = ca.cell_chunk[cj + 1];
+    sh.qk0[lane] = k0;
+    sh.qj0[lane] = la.cell_start[cj];
+    sh.qj1[lane] = la.cell_start[cj + 1];
+    sh.qcell[lane] = cj;
+    sh.ttab[3 * lane] = tx;
+    sh.ttab[3 * lane + 1] = ty;
+    sh.ttab[3 * lane + 2] = tz;
+    sh.tcode[lane] = image_code(tx, ty, tz);
+    nchq = k1 - k0;
+  }
+  int incl = nchq;  // inclusive prefix over lanes 0..31
+#pragma unroll
+  for (int off = 1; off < 32; off <<= 1) {
+    const int v = __shfl_up(incl, off);
+    if (lane >= off) incl += v;
+  }
+  if (lane < nq) sh.qpre[lane] = incl - nchq;
+  const int total = __shfl(incl, nq - 1);
+  // i-chunk box
   const double *bb = ca.bbox + 6 * ich;
   const double ilx = bb[0], ily = bb[1], ilz = bb[2];
   const double ihx = bb[3], ihy = bb[4], ihz = bb[5];
-  const float hg2i = wave_max(act ? st.hg2 : 0.f);
+  __syncthreads();
 
-  Rows r;
-  r.m0 = r.m1 = r.m2 = r.m3 = 0u;
-  r.ra = r.rb = 0u;
-  r.n = 0;
-  int cx = 0, cy = 0, cz = 0;
-  const int cell_i = ca.chunk_cell[ich];
-  if (stencil) cell_coords(cell_i, la.grid, cx, cy, cz);
-  const int nq = stencil ? 27 : 1;
-  for (int q = 0; q < nq; q++) {
-    double tx = tx0, ty = ty0, tz = tz0;
-    int cj = cj0;
-    if (stencil) cj = neighbour(la.grid, cx, cy, cz, kStencilOrder[q], tx, ty, tz);
-    const int k0 = ca.cell_chunk[cj], k1 = ca.cell_chunk[cj + 1];
-    if (k0 == k1) continue;
-    const bool own = cj == cell_i;  // the lane's own particle is a candidate
-    if (lane == 0) {
-      ttab[3 * q] = tx;
-      ttab[3 * q + 1] = ty;
-      ttab[3 * q + 2] = tz;
-    }
-    const double xs = st.x - tx, ys = st.y - ty, zs = st.z - tz;
-    const int jlo = ca.chunk_start[k0], jhi = ca.chunk_start[k1];
-    for (int base = jlo; base < jhi; base += 64) {
-      // The 64 / C chunks of this batch of <= 64 particles.
-      const int kb0 = k0 + (base - jlo) / C;
-      const int kb1 = min(k1, kb0 + 64 / C);
-      const int kc = kb0 + lane;
-      bool pass = false;
-      if (kc < kb1) {
-        const double *jb = ca.bbox + 6 * kc;
-        const double gx = fmax(0.0, fmax(jb[0] + tx - ihx, ilx - (jb[3] + tx)));
-        const double gy = fmax(0.0, fmax(jb[1] + ty - ihy, ily - (jb[4] + ty)));
-        const double gz = fmax(0.0, fmax(jb[2] + tz - ihz, ilz - (jb[5] + tz)));
-        const float gap2 = (float)(gx * gx + gy * gy + gz * gz);
-        const float rad2 = L::kSymmetric ? fmaxf(hg2i, ca.hg2max[kc]) : hg2i;
-        pass = gap2 <= rad2 * 1.0001f;
-      }
-      unsigned long long pm = __ballot(pass);
-      if (pm == 0ull) continue;  // uniform
-      // Stage the batch's candidate positions (one coalesced load per lane).
-      const int cnt = min(64, jhi - base);
-      __syncthreads();
-      if (lane < cnt) lpos[lane] = pos[base + lane];
-      __syncthreads();
-      while (pm) {
-        const int kk = kb0 + __builtin_ctzll(pm);
-        pm &= pm - 1;
-        const int j0 = ca.chunk_start[kk], j1 = ca.chunk_start[kk + 1];
+  unsigned long long m = 0ull;  // row hits of this lane
+  int nrow = 0;                 // rows recorded (uniform, multiple of RPC)
+  for (int fb = 0; fb < total; fb += 64) {
+    // Cull: one candidate chunk per lane.
+    const int f = fb + lane;
+    bool pass = false;
+    unsigned int pe = 0u;
+    if (f < total) {
+      int q = 0;
 #pragma unroll
-        for (int rr = 0; rr < RPC; rr++) {
-          const int rb = j0 + rr * S;
-          if (rb >= j1) break;  // uniform
-          const int j = rb + s;
-          bool hit = false;
-          if (rb + S <= j1 || j < j1) {
-            const PosRec &p = lpos[j - base];
-            float dx, dy, dz;
-            const float r2 = pair_r2(xs, ys, zs, p.x, p.y, p.z, dx, dy, dz);
-            hit = act && L::test(r2, st, &p);
-            if (own) hit = hit && j != i;
-          }
-          const unsigned int bitv = hit ? (1u << (r.n & 31)) : 0u;
-          const unsigned int info = ((unsigned int)q << 27) | (unsigned int)rb;
-          switch (r.n >> 5) {
-            case 0: r.m0 |= bitv; r.ra = lane == r.n ? info : r.ra; break;
-            case 1: r.m1 |= bitv; r.ra = lane == r.n ? info : r.ra; break;
-            case 2: r.m2 |= bitv; r.rb = lane == r.n - 64 ? info : r.rb; break;
-            default: r.m3 |= bitv; r.rb = lane == r.n - 64 ? info : r.rb; break;
-          }
-          r.n++;
-          if (r.n == kRows) interact_rows<L, C>(rec, ttab, row_info, st, s, r, acc);
+      for (int b = 16; b > 0; b >>= 1)
+        if (q + b < nq && sh.qpre[q + b] <= f) q += b;
+      const int kc = sh.qk0[q] + (f - sh.qpre[q]);
+      const double tx = sh.ttab[3 * q], ty = sh.ttab[3 * q + 1],
+                   tz = sh.ttab[3 * q + 2];
+      const double *jb = ca.bbox + 6 * kc;
+      const double gx = fmax(0.0, fmax(jb[0] + tx - ihx, ilx - (jb[3] + tx)));
+      const double gy = fmax(0.0, fmax(jb[1] + ty - ihy, ily - (jb[4] + ty)));
+      const double gz = fmax(0.0, fmax(jb[2] + tz - ihz, ilz - (jb[5] + tz)));
+      const float gap2 = (float)(gx * gx + gy * gy + gz * gz);
+      const float rad2 = L::kSymmetric ? fmaxf(hg2i, ca.hg2max[kc]) : hg2i;
+      pass = gap2 <= rad2 * 1.0001f;
+      pe = ((unsigned int)q << 27) | (unsigned int)(kc - sh.qk0[q]);
+    }
+    const unsigned long long pm = __ballot(pass);
+    if (pm == 0ull) continue;  // uniform
+    const int np = __popcll(pm);
+    if (pass) sh.plist[__popcll(pm & ((1ull << lane) - 1ull))] = pe;
+    __syncthreads();
+    for (int pb = 0; pb < np; pb += G) {
+      // Stage the candidates of up to G surviving chunks.
+      {
+        const int c = lane / C;
+        if (pb + c < np) {
+          const unsigned int e = sh.plist[pb + c];
+          const int q = (int)(e >> 27);
+          const int j = sh.qj0[q] + (int)(e & kRowMask) * C + lane % C;
+          if (j < sh.qj1[q]) sh.lpos[lane] = pos[j];
         }
       }
+      __syncthreads();
+      for (int c = 0; c < G; c++) {
+        if (pb + c >= np) break;  // uniform
+        const unsigned int e = sh.plist[pb + c];
+        const int q = (int)(e >> 27);
+        const int js = sh.qj0[q] + (int)(e & kRowMask) * C;
+        const int nj = min(C, sh.qj1[q] - js);
+        // chunk position of the lane's own particle (never its own
+        // neighbour), or -1
+        const int self = sh.qcell[q] == cell_i ? i - js : -1;
+        const double xs = st.x - sh.ttab[3 * q];
+        const double ys = st.y - sh.ttab[3 * q + 1];
+        const double zs = st.z - sh.ttab[3 * q + 2];
+        if (lane == 0)
+          sh.cinfo[nrow / RPC] = ((unsigned int)q << 27) | (unsigned int)js;
+#pragma unroll
+        for (int rr = 0; rr < RPC; rr++) {
+          if (rr * S >= nj) break;  // uniform
+          const int jl = rr * S + s;
+          // Slots past the end of a short chunk hold stale positions; their
+          // test result is discarded.
+          const PosRec &p = sh.lpos[c * C + jl];
+          float dx, dy, dz;
+          const float r2 = pair_r2(xs, ys, zs, p.x, p.y, p.z, dx, dy, dz);
+          const bool hit = act && jl < nj && jl != self &&
+                           (BUILD ? r2 < rt2 : L::test(r2, st, &p));
+          m |= (unsigned long long)hit << (nrow + rr);
+        }
+        nrow += RPC;
+        if (nrow == 64) {
+          __syncthreads();
+          interact_rows<L, C, BUILD>(rec, sh, st, s, m, acc, nl, i, ncnt);
+          m = 0ull;
+          nrow = 0;
+        }
+      }
+      __syncthreads();
     }
   }
-  if (r.n > 0) {
+  if (nrow > 0) {
     __syncthreads();
-    interact_rows<L, C>(rec, ttab, row_info, st, s, r, acc);
+    interact_rows<L, C, BUILD>(rec, sh, st, s, m, acc, nl, i, ncnt);
   }
   L::template combine<S>(acc);
   if (act && s == 0) L::store_acc(d, i, acc);
+  if (BUILD) {
+    int most = ncnt;
+#pragma unroll
+    for (int off = 64 / S; off < 64; off <<= 1)
+      most = max(most, __shfl_xor(most, off));
+    if (act) nl.cnt[i * kListStreams + s] = ncnt;
+    // a list with an overflowing stream is unusable
+    if (act && s == 0) d.r2b[i] = most > nl.Ks ? -1.f : rt2;
+  }
 }
 
 // Batched: every chunk against its 27-cell stencil.
-template <class L, int C, bool SUBSET>
+template <class L, int C, bool SUBSET, bool BUILD>
 __global__ __launch_bounds__(64) void k_loop_stencil(
     DevParts d, LoopArgs la, ChunkArgs ca,
-    const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos) {
+    const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
+    NbrList nl) {
   const int ich = xcd_remap(blockIdx.x, ca.nchunks);
   if (ich >= ca.nchunks) return;
-  run_chunk<L, C, SUBSET>(d, la, ca, rec, pos, ich, true, 0, 0.0, 0.0, 0.0);
+  run_chunk<L, C, SUBSET, BUILD>(d, la, ca, rec, pos, ich, true, 0, 0.0, 0.0,
+                                 0.0, nl);
 }
 
 // Per task: up to two directed (i-cell <- j-cell) interactions; blocks
@@ -289,8 +346,68 @@ __global__ __launch_bounds__(64) void k_loop_task(
     b -= ta.ntiles[0];
   }
   const int ich = ca.cell_chunk[ta.icell[q]] + b;
-  run_chunk<L, C, SUBSET>(d, la, ca, rec, pos, ich, false, ta.jcell[q],
-                          ta.t[3 * q], ta.t[3 * q + 1], ta.t[3 * q + 2]);
+  const NbrList none{};
+  run_chunk<L, C, SUBSET, false>(d, la, ca, rec, pos, ich, false, ta.jcell[q],
+                                 ta.t[3 * q], ta.t[3 * q + 1], ta.t[3 * q + 2],
+                                 none);
+}
+
+// List loop: lane g + kListChunk * s walks stream s of the neighbour list of
+// particle g of its tile (COMPACT: of d.sel[blockIdx.x * kListChunk + g]),
+// re-evaluates the pair contract for each entry and interacts with the ones
+// that pass the loop's own test.  Particles whose list does not cover their
+// search radius are left to the chunk loop.
+template <class L, bool COMPACT>
+__global__ __launch_bounds__(64) void k_loop_list(
+    DevParts d, LoopArgs la, ChunkArgs ca, NbrList nl,
+    const typename L::Rec *__restrict__ rec, int nsel) {
+  constexpr int C = kListChunk, S = kListStreams;
+  static_assert(!L::kSymmetric, "lists hold gather neighbours only");
+  const int lane = threadIdx.x;
+  const int g = lane % C;
+  const int s = lane / C;
+  int i = 0;
+  bool act;
+  if (COMPACT) {
+    const int k = blockIdx.x * C + g;
+    act = k < nsel;
+    if (act) i = d.sel[k];
+  } else {
+    const int ich = xcd_remap(blockIdx.x, ca.nchunks);
+    if (ich >= ca.nchunks) return;
+    i = ca.chunk_start[ich] + g;
+    act = i < ca.chunk_start[ich + 1];
+  }
+  if (act) act = is_active(d.tb[i], la.max_active_bin);
+  typename L::I st;
+  if (act) {
+    L::load_i(d, i, st);
+    act = st.hg2 <= d.r2b[i];
+  }
+  if (!__any(act)) return;  // uniform
+  typename L::Acc acc;
+  if (act && s == 0)
+    L::load_acc(d, i, acc);
+  else
+    L::identity(acc);
+  const int n = act ? min(nl.cnt[i * S + s], nl.Ks) : 0;
+  const unsigned int *ip = nl.idx + (size_t)i * S + s;
+  const size_t stride = (size_t)nl.npad * S;
+  const double dxm = la.grid.dim[0], dym = la.grid.dim[1], dzm = la.grid.dim[2];
+  for (int k = 0; k < n; k++) {
+    const unsigned int e = ip[k * stride];
+    const int j = (int)(e & kRowMask);
+    const int tc = (int)(e >> 27);
+    const double xs = st.x - (double)(tc / 9 - 1) * dxm;
+    const double ys = st.y - (double)((tc / 3) % 3 - 1) * dym;
+    const double zs = st.z - (double)(tc % 3 - 1) * dzm;
+    const typename L::Rec &jr = rec[j];
+    float dx, dy, dz;
+    const float r2 = pair_r2(xs, ys, zs, jr.x, jr.y, jr.z, dx, dy, dz);
+    if (L::test(r2, st, &jr)) L::interact(r2, dx, dy, dz, st, jr, acc);
+  }
+  L::template combine<S>(acc);
+  if (act && s == 0) L::store_acc(d, i, acc);
 }
 
 }  // namespace sphl

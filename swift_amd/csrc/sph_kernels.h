@@ -79,6 +79,30 @@ struct DevParts {
   GradRec *rg;
   ForceRec *rf;
   PosRec *pos;
+  // neighbour lists: squared radius each particle's list was built with
+  // (negative: no usable list), gradient fall-back flags, and the compact
+  // list of particles the ghost sent back for another density round
+  float *r2b;
+  int *lfb;
+  int *sel;
+};
+
+// Per-particle neighbour lists (Verlet lists with a skin): the candidates
+// j with r_ij^2 < lam2 * (gamma_K h_i)^2 found by the chunk loop, kept for
+// the density re-runs of the ghost and the gradient loop, which then test
+// only these.  Stream s (of kListStreams) of particle i owns slots
+// idx[(k * npad + i) * kListStreams + s], k < cnt[i * kListStreams + s];
+// an entry is (image code << 27) | j with image code
+// 9 (sx + 1) + 3 (sy + 1) + (sz + 1) for the periodic translation
+// (sx dim_x, sy dim_y, sz dim_z) of the j-cell.
+constexpr int kListStreams = 4;
+constexpr int kListChunk = 64 / kListStreams;
+struct NbrList {
+  unsigned int *idx;
+  int *cnt;
+  int Ks;      // slots per stream
+  int npad;
+  float lam2;  // skin factor on the squared search radius
 };
 
 struct GridDesc {
@@ -89,6 +113,7 @@ struct GridDesc {
 struct LoopArgs {
   GridDesc grid;
   const int *cell_start;  // [ncells + 1]
+  const int *flag;        // subset loops: particles with flag[i] != 0
   int max_active_bin;
 };
 
@@ -115,6 +140,20 @@ namespace sphk {
 hipError_t launch_stencil(LoopKind kind, bool subset, const DevParts &d,
                           const LoopArgs &la, const ChunkArgs &ca,
                           hipStream_t st);
+// Density loop over the 27-cell stencil that also (re)builds the neighbour
+// lists of the particles it processes (all active ones, or with `subset`
+// the flagged ones whose list does not cover their search radius).  The
+// chunk table must have capacity kListChunk.
+hipError_t launch_build_lists(bool subset, const DevParts &d,
+                              const LoopArgs &la, const ChunkArgs &ca,
+                              const NbrList &nl, hipStream_t st);
+// A density or gradient loop over the neighbour lists of the active
+// particles whose list covers their search radius: all of them (chunk
+// table of capacity kListChunk), or the nsel particles d.sel[0..nsel).
+hipError_t launch_list_loop(LoopKind kind, bool compact, const DevParts &d,
+                            const LoopArgs &la, const ChunkArgs &ca,
+                            const NbrList &nl, int nsel, hipStream_t st);
+hipError_t launch_fill_f32(float *p, float v, int n, hipStream_t st);
 hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
                        const LoopArgs &la, const ChunkArgs &ca,
                        const TaskArgs &ta, hipStream_t st);
@@ -128,8 +167,12 @@ hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
                                unsigned int *hg2_all, hipStream_t st);
 hipError_t launch_init_density(const DevParts &d, int lo, int hi,
                                int max_active_bin, float h_max, hipStream_t st);
+// counters[0]: particles sent back for another density round (their
+// indices are appended to d.sel); [1]: of those, the ones whose list no
+// longer covers the new radius; [2]: converged particles whose list does
+// not cover their final radius (flagged in d.lfb).
 hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,
-                        int *redo_count, hipStream_t st);
+                        int *counters, hipStream_t st);
 hipError_t launch_extra_ghost(const DevParts &d, const GhostArgs &ga, int lo,
                               int hi, hipStream_t st);
 hipError_t launch_end_force(const DevParts &d, int lo, int hi,

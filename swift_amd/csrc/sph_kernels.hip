@@ -144,6 +144,7 @@ __global__ void k_init_density(DevParts d, int lo, int hi, int max_active_bin,
   if (i >= hi) return;
   const bool act = is_active(d.tb[i], max_active_bin);
   d.redo[i] = act ? 1 : 0;
+  d.lfb[i] = 0;
   if (!act) return;
   d.rho[i] = 0.f;
   d.rho_dh[i] = 0.f;
@@ -192,8 +193,11 @@ __device__ void prepare_gradient(const DevParts &d, int i, float h, float rho,
 // particles flagged in redo[] within [lo, hi): hydro_end_density, one
 // Newton-Raphson / bisection step on h, and either convergence (prepare the
 // gradient loop) or a re-initialised particle flagged for another density
-// round.  redo_count[0] receives the number of particles flagged again.
-__device__ bool ghost_one(const DevParts &d, const GhostArgs &ga, int i) {
+// round.  `relist` tells whether the particle's neighbour list no longer
+// covers its new search radius (for another round, or for the gradient
+// loop once converged).
+__device__ bool ghost_one(const DevParts &d, const GhostArgs &ga, int i,
+                          bool &relist) {
   const float h_old = d.h[i];
   const float h_old_dim = h_old * h_old * h_old;
   const float h_old_dim_m1 = h_old * h_old;
@@ -261,6 +265,7 @@ __device__ bool ghost_one(const DevParts &d, const GhostArgs &ga, int i) {
       d.roty[i] = 0.f;
       d.rotz[i] = 0.f;
       d.nd[i] = 0;
+      relist = !(search_radius2(hn) <= d.r2b[i]);
       return true;
     }
     if (hn <= ga.h_min) {
@@ -283,6 +288,8 @@ __device__ bool ghost_one(const DevParts &d, const GhostArgs &ga, int i) {
   // Converged.
   d.h[i] = h_fin;
   d.redo[i] = 0;
+  relist = !(search_radius2(h_fin) <= d.r2b[i]);
+  d.lfb[i] = relist ? 1 : 0;
   d.rho[i] = rho;
   d.rho_dh[i] = rho_dh;
   d.wc[i] = wc;
@@ -295,14 +302,48 @@ __device__ bool ghost_one(const DevParts &d, const GhostArgs &ga, int i) {
   return false;
 }
 
-__global__ void k_ghost(DevParts d, GhostArgs ga, int lo, int hi,
-                        int *redo_count) {
-  const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  bool again = false;
-  if (i < hi && d.redo[i]) again = ghost_one(d, ga, i);
-  // one atomic per wave
-  const unsigned long long b = __ballot(again);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd(redo_count, __popcll(b));
+// Counts are reduced over the block first: one atomic per block and
+// counter keeps the same-address traffic small.
+constexpr int kGhostBlock = 1024;
+__global__ __launch_bounds__(kGhostBlock) void k_ghost(DevParts d, GhostArgs ga,
+                                                       int lo, int hi,
+                                                       int *counters) {
+  constexpr int W = kGhostBlock / 64;
+  __shared__ int wcnt[3][W];
+  __shared__ int base;
+  const int i = lo + blockIdx.x * kGhostBlock + threadIdx.x;
+  bool again = false, relist = false;
+  if (i < hi && d.redo[i]) again = ghost_one(d, ga, i, relist);
+  const unsigned long long ba = __ballot(again);
+  const unsigned long long bb = __ballot(again && relist);
+  const unsigned long long bc = __ballot(!again && relist);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+    wcnt[0][w] = __popcll(ba);
+    wcnt[1][w] = __popcll(bb);
+    wcnt[2][w] = __popcll(bc);
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    int t = 0;
+    for (int k = 0; k < W; k++) t += wcnt[threadIdx.x][k];
+    int b0 = 0;
+    if (t) b0 = atomicAdd(&counters[threadIdx.x], t);
+    if (threadIdx.x == 0) base = b0;
+  }
+  __syncthreads();
+  if (again) {
+    // append to the compact list of particles for the next round
+    int off = base;
+    for (int k = 0; k < w; k++) off += wcnt[0][k];
+    off += __popcll(ba & ((1ull << lane) - 1ull));
+    d.sel[off] = i;
+  }
+}
+
+__global__ void k_fill_f32(float *p, float v, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 // runner_do_extra_ghost (runner_ghost.c:1016-1111): hydro_end_gradient,
@@ -424,8 +465,65 @@ template <class L, bool SUB, int C>
 void stencil_c(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
                const typename L::Rec *rec, hipStream_t st) {
   const int per = (ca.nchunks + 7) >> 3;
-  hipLaunchKernelGGL((k_loop_stencil<L, C, SUB>), dim3(per * 8), dim3(64), 0,
-                     st, d, la, ca, rec, (const PosRec *)d.pos);
+  const NbrList none{};
+  hipLaunchKernelGGL((k_loop_stencil<L, C, SUB, false>), dim3(per * 8),
+                     dim3(64), 0, st, d, la, ca, rec, (const PosRec *)d.pos,
+                     none);
+}
+
+hipError_t launch_build_lists(bool subset, const DevParts &d,
+                              const LoopArgs &la, const ChunkArgs &ca,
+                              const NbrList &nl, hipStream_t st) {
+  if (ca.nchunks <= 0) return hipSuccess;
+  if (ca.C != kListChunk) return hipErrorInvalidValue;
+  const int per = (ca.nchunks + 7) >> 3;
+  if (subset)
+    hipLaunchKernelGGL((k_loop_stencil<DensityLoop, kListChunk, true, true>),
+                       dim3(per * 8), dim3(64), 0, st, d, la, ca, d.rd,
+                       (const PosRec *)d.pos, nl);
+  else
+    hipLaunchKernelGGL((k_loop_stencil<DensityLoop, kListChunk, false, true>),
+                       dim3(per * 8), dim3(64), 0, st, d, la, ca, d.rd,
+                       (const PosRec *)d.pos, nl);
+  return hipGetLastError();
+}
+
+template <class L>
+void list_loop(bool compact, const DevParts &d, const LoopArgs &la,
+               const ChunkArgs &ca, const NbrList &nl,
+               const typename L::Rec *rec, int nsel, hipStream_t st) {
+  if (compact) {
+    hipLaunchKernelGGL((k_loop_list<L, true>), dim3(nblk(nsel, kListChunk)),
+                       dim3(64), 0, st, d, la, ca, nl, rec, nsel);
+  } else {
+    const int per = (ca.nchunks + 7) >> 3;
+    hipLaunchKernelGGL((k_loop_list<L, false>), dim3(per * 8), dim3(64), 0, st,
+                       d, la, ca, nl, rec, 0);
+  }
+}
+
+hipError_t launch_list_loop(LoopKind kind, bool compact, const DevParts &d,
+                            const LoopArgs &la, const ChunkArgs &ca,
+                            const NbrList &nl, int nsel, hipStream_t st) {
+  if (compact ? nsel <= 0 : ca.nchunks <= 0) return hipSuccess;
+  if (!compact && ca.C != kListChunk) return hipErrorInvalidValue;
+  switch (kind) {
+    case LoopKind::Density:
+      list_loop<DensityLoop>(compact, d, la, ca, nl, d.rd, nsel, st);
+      break;
+    case LoopKind::Gradient:
+      list_loop<GradientLoop>(compact, d, la, ca, nl, d.rg, nsel, st);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_f32(float *p, float v, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_fill_f32, dim3(nblk(n, 256)), dim3(256), 0, st, p, v, n);
+  return hipGetLastError();
 }
 
 template <class L, bool SUB>
@@ -475,7 +573,8 @@ hipError_t launch_stencil(LoopKind kind, bool subset, const DevParts &d,
       return subset ? stencil_any<DensityLoop, true>(d, la, ca, d.rd, st)
                     : stencil_any<DensityLoop, false>(d, la, ca, d.rd, st);
     case LoopKind::Gradient:
-      return stencil_any<GradientLoop, false>(d, la, ca, d.rg, st);
+      return subset ? stencil_any<GradientLoop, true>(d, la, ca, d.rg, st)
+                    : stencil_any<GradientLoop, false>(d, la, ca, d.rg, st);
     case LoopKind::Force:
       return stencil_any<ForceLoop, false>(d, la, ca, d.rf, st);
   }
@@ -546,11 +645,11 @@ hipError_t launch_init_density(const DevParts &d, int lo, int hi,
 }
 
 hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,
-                        int *redo_count, hipStream_t st) {
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, redo_count, 1);
+                        int *counters, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, counters, 3);
   if (hi > lo)
-    hipLaunchKernelGGL(k_ghost, dim3(nblk(hi - lo, 256)), dim3(256), 0, st, d,
-                       ga, lo, hi, redo_count);
+    hipLaunchKernelGGL(k_ghost, dim3(nblk(hi - lo, kGhostBlock)),
+                       dim3(kGhostBlock), 0, st, d, ga, lo, hi, counters);
   return hipGetLastError();
 }
 

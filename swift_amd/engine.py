@@ -52,7 +52,8 @@ class _HostParts(C.Structure):
 class PassStats(C.Structure):
     _fields_ = [("ghost_rounds", C.c_int), ("n_density", C.c_int64),
                 ("n_gradient", C.c_int64), ("n_force", C.c_int64),
-                ("launches", C.c_int64)]
+                ("launches", C.c_int64), ("list_rebuilt", C.c_int64),
+                ("list_fallback", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -207,6 +208,13 @@ class Engine:
         check(lib().swsph_get_chunk(self._h, self.LOOPS[loop], C.byref(c)),
               "swsph_get_chunk")
         return c.value
+
+    def set_lists(self, enable: bool = True, skin: float = 1.2,
+                  slots: int = 48) -> None:
+        """Neighbour lists of the batched pass: candidates within `skin`
+        times the search radius, `slots` entries per candidate stream."""
+        check(lib().swsph_set_lists(self._h, int(bool(enable)), C.c_float(skin),
+                                    int(slots)), "swsph_set_lists")
 
     def save_state(self) -> None:
         check(lib().swsph_save_state(self._h), "swsph_save_state")
