@@ -77,6 +77,17 @@ __device__ __forceinline__ float search_radius2(float h) {
   return (h * h) * kGamma2;
 }
 
+// Radius of the FP32 pre-test for an exact squared radius R.  The pre-test
+// distance is formed from float positions relative to top-level cell
+// corners; each component differs from the pair contract's
+// (float)((x_i - t) - x_j) by at most eps (both are within a few ulp of the
+// real difference, and all terms are bounded by two cell widths).  Then
+// |r_f^2 - r^2| <= eps (2 sqrt(3) r + 3 eps) + 2 x 2^-22 r^2 (rounding of
+// both squares), so r^2 < R implies r_f^2 < pretest_radius2(R, eps).
+__device__ __forceinline__ float pretest_radius2(float R, float eps) {
+  return fmaf(eps, fmaf(4.f, sqrtf(R), 4.f * eps), R * 1.000002f);
+}
+
 __device__ __forceinline__ float fast_rsq(float x) {
   return __builtin_amdgcn_rsqf(x);
 }

@@ -118,12 +118,22 @@ void free_saved(swsph_engine *e) {
 
 namespace {
 
+// Component error bound of the FP32 pre-test: 2^-19 of the widest cell,
+// four times the worst case of the few float roundings of values bounded by
+// two cell widths that separate the pre-test distance from the contract's.
+float pos_eps(const swsph_engine *e) {
+  double w = 0.0;
+  for (int k = 0; k < 3; k++) w = std::max(w, e->grid.dim[k] / e->grid.nc[k]);
+  return (float)(w * 0x1p-19);
+}
+
 LoopArgs loop_args(const swsph_engine *e) {
   LoopArgs la;
   la.grid = e->grid;
   la.cell_start = e->d_cell_start;
   la.flag = e->d.redo;
   la.max_active_bin = e->props.max_active_bin;
+  la.pos_eps = pos_eps(e);
   return la;
 }
 
@@ -154,7 +164,8 @@ void free_dev(swsph_engine *e) {
                   e->d.avmax, e->d.ng, e->d.alpha, e->d.adiff, e->d.divv_prev,
                   e->d.divv_dt, e->d.ax, e->d.ay, e->d.az, e->d.udt, e->d.hdt,
                   e->d.nf, e->d.mintb, e->d.rd, e->d.rg, e->d.rf, e->d.pos,
-                  e->d.r2b, e->d.lfb, e->d.sel, e->nl.idx, e->nl.cnt,
+                  e->d.r2b, e->d.lfb, e->d.sel, e->d.cell, e->nl.idx,
+                  e->nl.cnt,
                   e->d_cell_start, e->d_tiles};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -241,11 +252,11 @@ int refresh_bounds(swsph_engine *e, int c_lo, int c_hi) {
   const int lo = e->cell_start[c_lo], hi = e->cell_start[c_hi];
   if (c_lo == 0 && c_hi == e->ncells)
     HIPCHK(sphk::launch_zero_u32(e->d_hg2_all, 1, e->st));
-  HIPCHK(sphk::launch_build_pos(e->d, lo, hi, e->st));
+  HIPCHK(sphk::launch_build_pos(e->d, e->grid, pos_eps(e), lo, hi, e->st));
   e->launches++;
   for (auto &t : e->tables) {
     HIPCHK(sphk::launch_chunk_bounds(e->d, t.d_start, t.cell_chunk_h[c_lo],
-                                     t.cell_chunk_h[c_hi], t.d_bbox,
+                                     t.cell_chunk_h[c_hi], t.C, t.d_bbox,
                                      t.d_hg2max, e->d_hg2_all, e->st));
     e->launches++;
   }
@@ -264,7 +275,7 @@ int alloc_dev(swsph_engine *e, size_t n) {
                    &d.divv_dt, &d.ax, &d.ay, &d.az, &d.udt, &d.hdt};
   for (float **p : f32) HIPCHK(dalloc(p, n));
   HIPCHK(dalloc(&d.r2b, n));
-  int **i32[] = {&d.nd, &d.redo, &d.ng, &d.nf, &d.lfb, &d.sel};
+  int **i32[] = {&d.nd, &d.redo, &d.ng, &d.nf, &d.lfb, &d.sel, &d.cell};
   for (int **p : i32) HIPCHK(dalloc(p, n));
   HIPCHK(dalloc(&d.tb, n));
   HIPCHK(dalloc(&d.mintb, n));
@@ -666,6 +677,16 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
     return fail(1, "particle count %lld out of range (max 2^27)", (long long)n);
   if (!p->x || !p->v || !p->m || !p->u || !p->time_bin || !p->h)
     return fail(1, "missing particle arrays");
+  // Positions must lie in the box (the reference wraps them when it
+  // rebuilds its cells); the FP32 pre-test bound relies on every particle
+  // lying in its top-level cell.
+  for (int64_t i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) {
+      const double xk = p->x[3 * i + k];
+      if (!(xk >= 0.0 && xk < e->grid.dim[k]))
+        return fail(1, "particle %lld lies outside the periodic box",
+                    (long long)i);
+    }
   HIPCHK(hipStreamSynchronize(e->st));
   free_dev(e);
   e->n = n;
@@ -764,6 +785,10 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
       return rc;
     for (int64_t s = 0; s < n; s++) bd[s] = p->time_bin[e->perm[s]];
     HIPCHK(hipMemcpy(e->d.tb, bd.data(), n, hipMemcpyHostToDevice));
+    std::vector<int> cs(n);
+    for (int64_t s = 0; s < n; s++) cs[s] = cid[e->perm[s]];
+    HIPCHK(hipMemcpy(e->d.cell, cs.data(), sizeof(int) * n,
+                     hipMemcpyHostToDevice));
   }
   HIPCHK(dalloc(&e->d_cell_start, (size_t)e->ncells + 1));
   HIPCHK(hipMemcpy(e->d_cell_start, e->cell_start.data(),
@@ -773,7 +798,7 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
   // and the chunk tables of the loops.
   for (LoopKind k : {LoopKind::Density, LoopKind::Gradient, LoopKind::Force})
     HIPCHK(sphk::launch_build_records(k, e->d, 0, (int)n, e->st));
-  HIPCHK(sphk::launch_build_pos(e->d, 0, (int)n, e->st));
+  HIPCHK(sphk::launch_build_pos(e->d, e->grid, pos_eps(e), 0, (int)n, e->st));
   for (int k = 0; k < 3; k++)
     if (!chunk_table(e, e->loop_c[k], &rc)) return rc;
   HIPCHK(hipStreamSynchronize(e->st));

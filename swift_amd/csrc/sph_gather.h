@@ -15,14 +15,17 @@
 // the largest search radius, then a ballot) and compacts the survivors into
 // LDS.  Survivors are staged 64 particles at a time into LDS and tested in
 // rows of S: stream s tests candidate (row start + s) against its particle
-// g.  A hit sets one bit of the lane's 64-bit row mask; a per-chunk table in
-// LDS remembers where each chunk starts and which image it uses.  After 64
-// rows every lane walks its own set bits, gathers its j record and runs the
-// interaction body, so the expensive bodies run dense although only a few
-// per cent of the candidates are neighbours.  At the end the S partial
-// accumulators of a particle are combined with cross-lane shuffles and
-// written once.  No atomics: every directed interaction is evaluated by one
-// lane.
+// g.  The row test is an FP32 pre-test on cell-relative positions whose
+// radius is widened by a proven bound on its difference from the pair
+// contract (pretest_radius2), so it never rejects a neighbour.  A pass sets
+// one bit of the lane's 64-bit row mask; a per-chunk table in LDS remembers
+// where each chunk starts and which image it uses.  After 64 rows every lane
+// walks its own set bits, gathers its j record, evaluates the exact FP64
+// contract and test, and runs the interaction body, so the expensive bodies
+// run dense although only a few per cent of the candidates are neighbours.
+// At the end the S partial accumulators of a particle are combined with
+// cross-lane shuffles and written once.  No atomics: every directed
+// interaction is evaluated by one lane.
 //
 // The BUILD form (density loop, C = kListChunk) tests against the skinned
 // radius lam2 * (gamma_K h_i)^2 instead, appends every hit to the lane's
@@ -103,6 +106,7 @@ struct GatherLds {
   static constexpr int SLOTS = 64 / RPC;             // chunks per 64 rows
   PosRec lpos[64];
   double ttab[27 * 3];
+  double qorg[27 * 3];  // corner of each neighbour's image
   unsigned int tcode[27];
   int qk0[27], qpre[28], qj0[27], qj1[27], qcell[27];
   unsigned int plist[64];
@@ -116,7 +120,7 @@ template <class L, int C, bool BUILD>
 __device__ __forceinline__ void interact_rows(
     const typename L::Rec *__restrict__ rec, const GatherLds<C> &sh,
     const typename L::I &st, int s, unsigned long long m,
-    typename L::Acc &acc, const NbrList &nl, int i, int &ncnt) {
+    typename L::Acc &acc, const NbrList &nl, int i, float rt2, int &ncnt) {
   constexpr int S = GatherLds<C>::S, RPC = GatherLds<C>::RPC;
   while (m) {
     const int row = __builtin_ctzll(m);
@@ -131,12 +135,14 @@ __device__ __forceinline__ void interact_rows(
     float dx, dy, dz;
     const float r2 = pair_r2(xs, ys, zs, jr.x, jr.y, jr.z, dx, dy, dz);
     if (BUILD) {
-      if (ncnt < nl.Ks)
-        nl.idx[((size_t)ncnt * nl.npad + i) * kListStreams + s] =
-            (sh.tcode[q] << 27) | (unsigned int)j;
-      ncnt++;
-      if (L::test(r2, st, &jr)) L::interact(r2, dx, dy, dz, st, jr, acc);
-    } else {
+      if (r2 < rt2) {
+        if (ncnt < nl.Ks)
+          nl.idx[((size_t)ncnt * nl.npad + i) * kListStreams + s] =
+              (sh.tcode[q] << 27) | (unsigned int)j;
+        ncnt++;
+        if (L::test(r2, st, &jr)) L::interact(r2, dx, dy, dz, st, jr, acc);
+      }
+    } else if (L::test(r2, st, &jr)) {
       L::interact(r2, dx, dy, dz, st, jr, acc);
     }
   }
@@ -184,6 +190,8 @@ __device__ __forceinline__ void run_chunk(
     L::identity(acc);
   // squared radius of the candidate test (skinned when building lists)
   const float rt2 = BUILD ? nl.lam2 * st.hg2 : st.hg2;
+  // pre-test radius; inactive lanes never pass
+  const float rt2m = act ? pretest_radius2(rt2, la.pos_eps) : -1.f;
   const float hg2i = wave_max(act ? rt2 : 0.f);
   int ncnt = 0;  // list entries found by this lane (BUILD)
 
@@ -199,6 +207,11 @@ __device__ __forceinline__ void run_chunk(
       cell_coords(cell_i, la.grid, cx, cy, cz);
       cj = neighbour(la.grid, cx, cy, cz, kStencilOrder[lane], tx, ty, tz);
     }
+    int jx, jy, jz;
+    cell_coords(cj, la.grid, jx, jy, jz);
+    sh.qorg[3 * lane] = (double)jx * (la.grid.dim[0] / la.grid.nc[0]) + tx;
+    sh.qorg[3 * lane + 1] = (double)jy * (la.grid.dim[1] / la.grid.nc[1]) + ty;
+    sh.qorg[3 * lane + 2] = (double)jz * (la.grid.dim[2] / la.grid.nc[2]) + tz;
     const int k0 = ca.cell_chunk[cj], k1 = ca.cell_chunk[cj + 1];
     sh.qk0[lane] = k0;
     sh.qj0[lane] = la.cell_start[cj];
@@ -261,7 +274,14 @@ __device__ __forceinline__ void run_chunk(
           const unsigned int e = sh.plist[pb + c];
           const int q = (int)(e >> 27);
           const int j = sh.qj0[q] + (int)(e & kRowMask) * C + lane % C;
-          if (j < sh.qj1[q]) sh.lpos[lane] = pos[j];
+          PosRec p;
+          if (j < sh.qj1[q]) {
+            p = pos[j];
+          } else {  // past the end of a short chunk: never passes
+            p.x = p.y = p.z = -1e30f;
+            p.hg2m = -1.f;
+          }
+          sh.lpos[lane] = p;
         }
       }
       __syncthreads();
@@ -271,31 +291,30 @@ __device__ __forceinline__ void run_chunk(
         const int q = (int)(e >> 27);
         const int js = sh.qj0[q] + (int)(e & kRowMask) * C;
         const int nj = min(C, sh.qj1[q] - js);
-        // chunk position of the lane's own particle (never its own
-        // neighbour), or -1
-        const int self = sh.qcell[q] == cell_i ? i - js : -1;
-        const double xs = st.x - sh.ttab[3 * q];
-        const double ys = st.y - sh.ttab[3 * q + 1];
-        const double zs = st.z - sh.ttab[3 * q + 2];
+        // the wave's own chunk: lane g's particle is never its own neighbour
+        const bool own = sh.qcell[q] == cell_i && js == i0;
+        // i relative to the neighbour image's corner
+        const float xs = (float)(st.x - sh.qorg[3 * q]);
+        const float ys = (float)(st.y - sh.qorg[3 * q + 1]);
+        const float zs = (float)(st.z - sh.qorg[3 * q + 2]);
         if (lane == 0)
           sh.cinfo[nrow / RPC] = ((unsigned int)q << 27) | (unsigned int)js;
 #pragma unroll
         for (int rr = 0; rr < RPC; rr++) {
           if (rr * S >= nj) break;  // uniform
           const int jl = rr * S + s;
-          // Slots past the end of a short chunk hold stale positions; their
-          // test result is discarded.
-          const PosRec &p = sh.lpos[c * C + jl];
-          float dx, dy, dz;
-          const float r2 = pair_r2(xs, ys, zs, p.x, p.y, p.z, dx, dy, dz);
-          const bool hit = act && jl < nj && jl != self &&
-                           (BUILD ? r2 < rt2 : L::test(r2, st, &p));
+          const PosRec p = sh.lpos[c * C + jl];
+          const float dx = xs - p.x, dy = ys - p.y, dz = zs - p.z;
+          const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+          bool hit = L::kSymmetric ? (r2 < rt2m || r2 < p.hg2m) : r2 < rt2m;
+          if (own) hit = hit && jl != g;
           m |= (unsigned long long)hit << (nrow + rr);
         }
         nrow += RPC;
         if (nrow == 64) {
           __syncthreads();
-          interact_rows<L, C, BUILD>(rec, sh, st, s, m, acc, nl, i, ncnt);
+          interact_rows<L, C, BUILD>(rec, sh, st, s, m, acc, nl, i, rt2,
+                                     ncnt);
           m = 0ull;
           nrow = 0;
         }
@@ -305,7 +324,7 @@ __device__ __forceinline__ void run_chunk(
   }
   if (nrow > 0) {
     __syncthreads();
-    interact_rows<L, C, BUILD>(rec, sh, st, s, m, acc, nl, i, ncnt);
+    interact_rows<L, C, BUILD>(rec, sh, st, s, m, acc, nl, i, rt2, ncnt);
   }
   L::template combine<S>(acc);
   if (act && s == 0) L::store_acc(d, i, acc);
@@ -394,14 +413,28 @@ __global__ __launch_bounds__(64) void k_loop_list(
   const unsigned int *ip = nl.idx + (size_t)i * S + s;
   const size_t stride = (size_t)nl.npad * S;
   const double dxm = la.grid.dim[0], dym = la.grid.dim[1], dzm = la.grid.dim[2];
+  // Software pipeline: list entries are fetched two ahead and records one
+  // ahead of the entry being evaluated, so the gathers overlap the bodies.
+  unsigned int e1 = n > 0 ? ip[0] : 0u;
+  unsigned int e2 = n > 1 ? ip[stride] : 0u;
+  typename L::Rec r1;
+  if (n > 0) r1 = rec[e1 & kRowMask];
   for (int k = 0; k < n; k++) {
-    const unsigned int e = ip[k * stride];
-    const int j = (int)(e & kRowMask);
+    const unsigned int e = e1;
+    const typename L::Rec jr = r1;
+    e1 = e2;
+    if (k + 2 < n) e2 = ip[(size_t)(k + 2) * stride];
+    if (k + 1 < n) r1 = rec[e1 & kRowMask];
+    // image code 9 a + 3 b + c, a, b, c in {0, 1, 2} (translation a - 1,
+    // b - 1, c - 1 box lengths)
     const int tc = (int)(e >> 27);
-    const double xs = st.x - (double)(tc / 9 - 1) * dxm;
-    const double ys = st.y - (double)((tc / 3) % 3 - 1) * dym;
-    const double zs = st.z - (double)(tc % 3 - 1) * dzm;
-    const typename L::Rec &jr = rec[j];
+    const int a = (tc * 57) >> 9;
+    const int rem = tc - 9 * a;
+    const int b = (rem * 11) >> 5;
+    const int c = rem - 3 * b;
+    const double xs = st.x - (double)(a - 1) * dxm;
+    const double ys = st.y - (double)(b - 1) * dym;
+    const double zs = st.z - (double)(c - 1) * dzm;
     float dx, dy, dz;
     const float r2 = pair_r2(xs, ys, zs, jr.x, jr.y, jr.z, dx, dy, dz);
     if (L::test(r2, st, &jr)) L::interact(r2, dx, dy, dz, st, jr, acc);

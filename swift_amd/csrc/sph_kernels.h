@@ -33,11 +33,13 @@ struct __attribute__((aligned(16))) ForceRec {
   float pad[2];
 };
 
-// Position of a candidate for the pair test, and its search radius
-// (gamma_K h)^2 (read by the force test).
+// Candidate record of the FP32 pre-test: the position relative to the
+// corner of the particle's top-level cell, rounded to float, and the
+// particle's search radius (gamma_K h)^2 widened by the pre-test margin
+// (read by the force test).  Inhibited particles are far away with a
+// negative radius.
 struct __attribute__((aligned(16))) PosRec {
-  double x, y, z;
-  float hg2, pad;
+  float x, y, z, hg2m;
 };
 
 // Chunks: runs of <= C consecutive particles of one cell (Morton order inside
@@ -85,6 +87,7 @@ struct DevParts {
   float *r2b;
   int *lfb;
   int *sel;
+  int *cell;  // top-level cell of each particle
 };
 
 // Per-particle neighbour lists (Verlet lists with a skin): the candidates
@@ -115,6 +118,9 @@ struct LoopArgs {
   const int *cell_start;  // [ncells + 1]
   const int *flag;        // subset loops: particles with flag[i] != 0
   int max_active_bin;
+  // Bound on the per-component difference between the FP32 pre-test
+  // distance and the pair contract's distance (see pretest_radius2).
+  float pos_eps;
 };
 
 // Up to two directed cell interactions (i-cell gathers from j-cell, the
@@ -158,13 +164,15 @@ hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
                        const LoopArgs &la, const ChunkArgs &ca,
                        const TaskArgs &ta, hipStream_t st);
 // Positions / search radii of the candidate test for particles [lo, hi).
-hipError_t launch_build_pos(const DevParts &d, int lo, int hi, hipStream_t st);
+hipError_t launch_build_pos(const DevParts &d, const GridDesc &g, float pos_eps,
+                            int lo, int hi, hipStream_t st);
 hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t st);
 // Bounding boxes and largest search radius of chunks [c0, c1); the largest
 // search radius over them is max-reduced into *hg2_all (float bits).
 hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
-                               int c0, int c1, double *bbox, float *hg2max,
-                               unsigned int *hg2_all, hipStream_t st);
+                               int c0, int c1, int C, double *bbox,
+                               float *hg2max, unsigned int *hg2_all,
+                               hipStream_t st);
 hipError_t launch_init_density(const DevParts &d, int lo, int hi,
                                int max_active_bin, float h_max, hipStream_t st);
 // counters[0]: particles sent back for another density round (their

@@ -21,37 +21,52 @@ using namespace sphl;
 namespace {
 
 // ------------------------------------------------------ neighbour records --
-__global__ void k_build_pos(DevParts d, int lo, int hi) {
+__global__ void k_build_pos(DevParts d, GridDesc g, float pos_eps, int lo,
+                            int hi) {
   const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= hi) return;
   const bool inh = d.tb[i] == kBinInhibited;
+  const int c = d.cell[i];
+  const int cz = c % g.nc[2], cy = (c / g.nc[2]) % g.nc[1];
+  const int cx = c / (g.nc[2] * g.nc[1]);
   PosRec p;
-  p.x = inh ? kFar : d.px[i];
-  p.y = inh ? kFar : d.py[i];
-  p.z = inh ? kFar : d.pz[i];
-  p.hg2 = inh ? -1.f : search_radius2(d.h[i]);
-  p.pad = 0.f;
+  p.x = inh ? 1e30f : (float)(d.px[i] - (double)cx * (g.dim[0] / g.nc[0]));
+  p.y = inh ? 1e30f : (float)(d.py[i] - (double)cy * (g.dim[1] / g.nc[1]));
+  p.z = inh ? 1e30f : (float)(d.pz[i] - (double)cz * (g.dim[2] / g.nc[2]));
+  p.hg2m = inh ? -1.f : pretest_radius2(search_radius2(d.h[i]), pos_eps);
   d.pos[i] = p;
 }
 
 // Bounding box (non-inhibited particles) and largest search radius of each
 // chunk in [c0, c1); an empty chunk gets an inverted box that no test passes.
+// A wave takes 64 / C chunks, one lane per particle, and reduces over the C
+// lanes of each chunk.
 __global__ void k_chunk_bounds(DevParts d, const int *chunk_start, int c0,
-                               int c1, double *bbox, float *hg2max,
+                               int c1, int C, double *bbox, float *hg2max,
                                unsigned int *hg2_all) {
-  const int k = c0 + blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int k = c0 + wave * (64 / C) + lane / C;
+  const int g = lane % C;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   float hm = 0.f;
   if (k < c1) {
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int i = chunk_start[k]; i < chunk_start[k + 1]; i++) {
-      if (d.tb[i] == kBinInhibited) continue;
-      const double x[3] = {d.px[i], d.py[i], d.pz[i]};
-      for (int a = 0; a < 3; a++) {
-        lo[a] = fmin(lo[a], x[a]);
-        hi[a] = fmax(hi[a], x[a]);
-      }
-      hm = fmaxf(hm, search_radius2(d.h[i]));
+    const int i = chunk_start[k] + g;
+    if (i < chunk_start[k + 1] && d.tb[i] != kBinInhibited) {
+      lo[0] = hi[0] = d.px[i];
+      lo[1] = hi[1] = d.py[i];
+      lo[2] = hi[2] = d.pz[i];
+      hm = search_radius2(d.h[i]);
     }
+  }
+  for (int off = 1; off < C; off <<= 1) {
+    for (int a = 0; a < 3; a++) {
+      lo[a] = fmin(lo[a], __shfl_xor(lo[a], off));
+      hi[a] = fmax(hi[a], __shfl_xor(hi[a], off));
+    }
+    hm = fmaxf(hm, __shfl_xor(hm, off));
+  }
+  if (k < c1 && g == 0) {
     for (int a = 0; a < 3; a++) {
       bbox[6 * k + a] = lo[a];
       bbox[6 * k + 3 + a] = hi[a];
@@ -59,8 +74,8 @@ __global__ void k_chunk_bounds(DevParts d, const int *chunk_start, int c0,
     hg2max[k] = hm;
   }
   // positive floats order like their bit patterns
-  for (int off = 32; off > 0; off >>= 1) hm = fmaxf(hm, __shfl_down(hm, off));
-  if ((threadIdx.x & 63) == 0) atomicMax(hg2_all, __float_as_uint(hm));
+  for (int off = 32; off >= C; off >>= 1) hm = fmaxf(hm, __shfl_xor(hm, off));
+  if (lane == 0) atomicMax(hg2_all, __float_as_uint(hm));
 }
 
 __global__ void k_build_dens_rec(DevParts d, int lo, int hi) {
@@ -602,19 +617,23 @@ hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t st) {
   return hipGetLastError();
 }
 
-hipError_t launch_build_pos(const DevParts &d, int lo, int hi, hipStream_t st) {
+hipError_t launch_build_pos(const DevParts &d, const GridDesc &g, float pos_eps,
+                            int lo, int hi, hipStream_t st) {
   if (hi <= lo) return hipSuccess;
   hipLaunchKernelGGL(k_build_pos, dim3(nblk(hi - lo, 256)), dim3(256), 0, st, d,
-                     lo, hi);
+                     g, pos_eps, lo, hi);
   return hipGetLastError();
 }
 
 hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
-                               int c0, int c1, double *bbox, float *hg2max,
-                               unsigned int *hg2_all, hipStream_t st) {
+                               int c0, int c1, int C, double *bbox,
+                               float *hg2max, unsigned int *hg2_all,
+                               hipStream_t st) {
   if (c1 <= c0) return hipSuccess;
-  hipLaunchKernelGGL(k_chunk_bounds, dim3(nblk(c1 - c0, 256)), dim3(256), 0, st,
-                     d, chunk_start, c0, c1, bbox, hg2max, hg2_all);
+  if (C != 8 && C != 16 && C != 32 && C != 64) return hipErrorInvalidValue;
+  const int waves = nblk(c1 - c0, 64 / C);
+  hipLaunchKernelGGL(k_chunk_bounds, dim3(nblk(waves, 4)), dim3(256), 0, st, d,
+                     chunk_start, c0, c1, C, bbox, hg2max, hg2_all);
   return hipGetLastError();
 }
 
