@@ -127,16 +127,20 @@ def main() -> int:
         avg = ms[k] / max(nl[k], 1)
         entry = {"avg_ms": avg, "launches_per_step": nl[k] / args.steps,
                  "share_of_step": ms[k] / (1e3 * total)}
+        if k == 0:
+            # every density launch of the pass: the first round and the
+            # ghost's re-runs, charged with the interactions they evaluated
+            entry["interactions_evaluated"] = stats["n_density_all"]
+            entry["tflops"] = (stats["n_density_all"] * FLOP_DENSITY * args.steps
+                               / (ms[0] * 1e-3) / 1e12)
         if k == 1:
             entry["tflops"] = stats["n_gradient"] * FLOP_GRADIENT / (avg * 1e-3) / 1e12
         if k == 2:
             entry["tflops"] = stats["n_force"] * FLOP_FORCE / (avg * 1e-3) / 1e12
         kernels[names[k]] = entry
     if dom == 0:
-        # density launches include the ghost re-runs: charge the converged
-        # pass only (a lower bound on the work done)
-        flops = stats["n_density"] * FLOP_DENSITY * args.steps / (ms[0] * 1e-3)
-        avg_ms = ms[0] / max(nl[0], 1)
+        flops = kernels["density_loop"]["tflops"] * 1e12
+        avg_ms = kernels["density_loop"]["avg_ms"]
     elif dom == 1:
         flops = kernels["gradient_loop"]["tflops"] * 1e12
         avg_ms = kernels["gradient_loop"]["avg_ms"]

@@ -93,6 +93,8 @@ typedef struct swsph_pass_stats {
   int64_t launches;              /* kernel launches issued by the pass */
   int64_t list_rebuilt;          /* batched: particles re-listed by re-runs */
   int64_t list_fallback;         /* batched: gradient via the chunk loop */
+  int64_t n_density_all;         /* batched: density interactions evaluated
+                                    over all ghost rounds (work done) */
 } swsph_pass_stats;
 
 const char *swsph_last_error(void);
@@ -148,6 +150,15 @@ int swsph_do_end_hydro_force(swsph_engine *e, int ci);
  * all cells.  Both leave identical neighbour sets. */
 int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats);
 int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats);
+/* The batched pass replayed from two captured HIP graphs: hydro_init_part,
+ * the first density round and as many ghost / density re-run rounds as the
+ * previous pass needed (rounds with nothing left to redo run empty), then
+ * gradient -> extra ghost -> force -> end force.  The host reads one
+ * convergence counter between the graphs and continues eagerly if more
+ * rounds are needed; graphs are re-captured after any change of particles
+ * or settings.  Results are identical to swsph_pass_batched.  Needs the
+ * neighbour lists disabled.  No counterpart in the reference. */
+int swsph_pass_graph(swsph_engine *e, swsph_pass_stats *stats);
 
 /* Device-to-device save / restore of the state a pass reads and rewrites
  * (h and the (io) fields), on the engine stream: the device-resident
@@ -164,6 +175,17 @@ int swsph_restore_state(swsph_engine *e);
  * summation order. */
 int swsph_set_chunk(swsph_engine *e, int which, int C);
 int swsph_get_chunk(swsph_engine *e, int which, int *C);
+
+/* Schedule of the batched pass's neighbour loops.  mode 0: one wave per
+ * chunk of every cell, launched as one grid (inactive chunks exit at once).
+ * Modes 1 and 2: the particles a loop processes (all active ones, or the
+ * ones the ghost sent back) are compacted on the device into a work queue
+ * of tiles of <= C particles of one cell; mode 1 (default) runs one wave per
+ * tile, mode 2 a persistent grid of `waves` waves (0: enough to fill every
+ * CU) that drains the queue through per-XCD sub-queues with stealing.  The
+ * batched form's answer to the reference's task queue (src/scheduler.c)
+ * over cells of very uneven occupancy; results do not depend on the mode. */
+int swsph_set_schedule(swsph_engine *e, int mode, int waves);
 
 /* Neighbour lists of the batched pass.  With lists enabled, the first
  * density round keeps, per particle, the candidates within `skin` times its

@@ -11,7 +11,9 @@ import os
 import re
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libswsph.so")
+# SWSPH_LIBRARY selects another build of the library (e.g. one compiled with
+# other tuning knobs).
+LIB_PATH = os.environ.get("SWSPH_LIBRARY") or os.path.join(_HERE, "libswsph.so")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "swsph.h")
 _lib = None
 

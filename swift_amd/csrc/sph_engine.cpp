@@ -66,8 +66,9 @@ struct swsph_engine {
   int *d_cell_start = nullptr;
   int4 *d_tiles = nullptr;
   int ntiles = 0;
-  int *d_redo = nullptr;          // ghost counters (see launch_ghost)
-  int *h_redo = nullptr;          // pinned readback
+  unsigned long long *d_redo = nullptr;  // ghost counters (launch_ghost)
+  unsigned long long *h_redo = nullptr;  // pinned readback
+  int64_t density_work = 0;  // density interactions over all rounds
   unsigned long long *d_cnt = nullptr;
   unsigned long long *h_cnt = nullptr;
   hipStream_t st = nullptr;
@@ -102,6 +103,22 @@ struct swsph_engine {
   size_t nl_elems = 0;
   int64_t list_rebuilt = 0;   // particles re-listed in the last pass
   int64_t list_fallback = 0;  // gradient fall-backs in the last pass
+  // Schedule of the batched loops: 0 one wave per chunk over all chunks,
+  // 1 one wave per tile of a device-built tile queue, 2 persistent waves
+  // draining that queue.
+  int schedule = 1;
+  int queue_waves = 0;  // 0: one full-occupancy wave set per CU
+  TileScratch ts{};
+  int tiles_c = 0;  // capacity of the all-active tile table in ts, 0: none
+  // Graph replay of the batched pass: the density phase (with a fixed
+  // number of re-runs) and the gradient-to-end-force phase, captured for
+  // the configuration generation `graph_gen`.
+  uint64_t gen = 1;  // bumped by every change of particles or settings
+  uint64_t graph_gen = 0;
+  int graph_reruns = 0;
+  int last_rounds = 4;  // ghost rounds of the last pass
+  bool capturing = false;
+  hipGraphExec_t g_density = nullptr, g_finish = nullptr;
 };
 
 namespace {
@@ -165,11 +182,14 @@ void free_dev(swsph_engine *e) {
                   e->d.divv_dt, e->d.ax, e->d.ay, e->d.az, e->d.udt, e->d.hdt,
                   e->d.nf, e->d.mintb, e->d.rd, e->d.rg, e->d.rf, e->d.pos,
                   e->d.r2b, e->d.lfb, e->d.sel, e->d.cell, e->nl.idx,
-                  e->nl.cnt,
+                  e->nl.cnt, e->ts.flag, e->ts.scan, e->ts.blk, e->ts.cnt,
+                  e->ts.cscan, e->ts.cblk, e->ts.alist, e->ts.tile_start,
+                  e->ts.tile_cell, e->ts.ntiles, e->ts.heads,
                   e->d_cell_start, e->d_tiles};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   e->d = DevParts{};
+  e->ts = TileScratch{};
   e->nl.idx = nullptr;
   e->nl.cnt = nullptr;
   e->nl_elems = 0;
@@ -283,6 +303,21 @@ int alloc_dev(swsph_engine *e, size_t n) {
   HIPCHK(dalloc(&d.rg, n));
   HIPCHK(dalloc(&d.rf, n));
   HIPCHK(dalloc(&d.pos, n));
+  {
+    TileScratch &t = e->ts;
+    const size_t nc = (size_t)e->ncells;
+    HIPCHK(dalloc(&t.flag, n));
+    HIPCHK(dalloc(&t.scan, n + 1));
+    HIPCHK(dalloc(&t.blk, n / 1024 + 2));
+    HIPCHK(dalloc(&t.cnt, nc));
+    HIPCHK(dalloc(&t.cscan, nc + 1));
+    HIPCHK(dalloc(&t.cblk, nc / 1024 + 2));
+    HIPCHK(dalloc(&t.alist, n));
+    HIPCHK(dalloc(&t.tile_start, n / 8 + nc + 1));  // C >= 8
+    HIPCHK(dalloc(&t.tile_cell, n / 8 + nc + 1));
+    HIPCHK(dalloc(&t.ntiles, 1));
+    HIPCHK(dalloc(&t.heads, 8 * 64));
+  }
   // Zero everything once so that fields never written for inactive
   // particles read as zero.
   const size_t nn = n ? n : 1;
@@ -344,7 +379,7 @@ hipEvent_t next_event(swsph_engine *e) {
 // Timing events around the launches of one loop kind, when enabled.
 int time_begin(swsph_engine *e, hipEvent_t *a) {
   *a = nullptr;
-  if (!e->timing) return 0;
+  if (!e->timing || e->capturing) return 0;
   *a = next_event(e);
   if (!*a) return fail(2, "hipEventCreate failed");
   HIPCHK(hipEventRecord(*a, e->st));
@@ -352,7 +387,7 @@ int time_begin(swsph_engine *e, hipEvent_t *a) {
 }
 
 int time_end(swsph_engine *e, LoopKind k, hipEvent_t a) {
-  if (!e->timing) return 0;
+  if (!e->timing || e->capturing) return 0;
   hipEvent_t b = next_event(e);
   if (!b) return fail(2, "hipEventCreate failed");
   HIPCHK(hipEventRecord(b, e->st));
@@ -372,6 +407,45 @@ int stencil(swsph_engine *e, LoopKind k, bool subset,
   if (flag) la.flag = flag;
   HIPCHK(sphk::launch_stencil(k, subset, e->d, la, chunk_args(t), e->st));
   e->launches++;
+  return time_end(e, k, a);
+}
+
+TileArgs tile_args(const swsph_engine *e) {
+  TileArgs ta;
+  ta.alist = e->ts.alist;
+  ta.tile_start = e->ts.tile_start;
+  ta.tile_cell = e->ts.tile_cell;
+  ta.ntiles = e->ts.ntiles;
+  ta.heads = e->ts.heads;
+  return ta;
+}
+
+// A batched loop through the work queue: tiles of the particles selected by
+// `mode` (0 active, 1 active and flagged, 2 flagged without a usable
+// neighbour list), drained by persistent waves; `build` rebuilds the lists
+// of the selected particles while doing their density.
+int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
+               bool build) {
+  int rc = 0;
+  const int C = build ? kListChunk : e->loop_c[(int)k];
+  const swsph_engine::ChunkTable *t = chunk_table(e, C, &rc);
+  if (!t) return rc;
+  hipEvent_t a;
+  if ((rc = time_begin(e, &a))) return rc;
+  const LoopArgs la = loop_args(e);
+  // The all-active table is kept for the later loops of the pass.
+  if (!(mode == 0 && e->tiles_c == C)) {
+    HIPCHK(sphk::launch_build_tiles(e->d, la, (int)e->n, e->ncells, C, mode,
+                                    flag, e->ts, e->st));
+    e->tiles_c = mode == 0 ? C : 0;
+  } else if (e->schedule == 2) {
+    HIPCHK(sphk::launch_zero_u32((unsigned int *)e->ts.heads, 8 * 64, e->st));
+  }
+  const int max_tiles = (int)(e->n / C) + e->ncells + 1;
+  HIPCHK(sphk::launch_queue(k, build, e->d, la, chunk_args(t), tile_args(e),
+                            e->nl, C, e->schedule == 2, e->queue_waves,
+                            max_tiles, e->st));
+  e->launches += 13;
   return time_end(e, k, a);
 }
 
@@ -496,12 +570,13 @@ void all_neighbours(const swsph_engine *e, int ci, std::vector<int> &out) {
 // needing a new neighbour list, [2] converged without a usable list.
 int read_redo(swsph_engine *e, int *count, int *relist = nullptr,
               int *nolist = nullptr) {
-  HIPCHK(hipMemcpyAsync(e->h_redo, e->d_redo, 3 * sizeof(int),
+  HIPCHK(hipMemcpyAsync(e->h_redo, e->d_redo, 4 * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, e->st));
   HIPCHK(hipStreamSynchronize(e->st));
-  *count = e->h_redo[0];
-  if (relist) *relist = e->h_redo[1];
-  if (nolist) *nolist = e->h_redo[2];
+  *count = (int)e->h_redo[0];
+  if (relist) *relist = (int)e->h_redo[1];
+  if (nolist) *nolist = (int)e->h_redo[2];
+  e->density_work = (int64_t)e->h_redo[3];  // accumulated on the device
   return 0;
 }
 
@@ -600,6 +675,126 @@ int check_h_fits(swsph_engine *e) {
 }  // namespace
 
 // ========================================================================= //
+namespace {
+// ---- phases of the batched pass ------------------------------------------
+// Launches only (no host synchronisation), so that a graph can capture them.
+
+// hydro_init_part on the active particles and the first density round (with
+// lists: a fresh neighbour list for every active particle).
+int phase_start(swsph_engine *e) {
+  int rc = 0;
+  HIPCHK(sphk::launch_init_density(e->d, 0, (int)e->n, e->props.max_active_bin,
+                                   e->props.h_max, e->st));
+  HIPCHK(sphk::launch_zero_u64(e->d_redo + 3, 1, e->st));
+  e->launches += 2;
+  e->tiles_c = 0;  // activity may have changed since the last pass
+  if (e->schedule >= 1)
+    return queue_loop(e, LoopKind::Density, 0, nullptr, e->use_lists);
+  if (e->use_lists) return build_lists(e, false);
+  return stencil(e, LoopKind::Density, false);
+}
+
+// One ghost round over all particles (runner_do_ghost).
+int phase_ghost(swsph_engine *e) {
+  HIPCHK(sphk::launch_ghost(e->d, ghost_args(e), 0, (int)e->n, e->d_redo,
+                            e->st));
+  e->launches += 2;
+  return 0;
+}
+
+// The density re-run of the particles the last ghost round flagged.  With
+// lists it needs the host's counts of that round (redo, relist).
+int phase_rerun(swsph_engine *e, int redo, int relist) {
+  int rc = 0;
+  const bool queue = e->schedule >= 1;
+  if (e->use_lists) {
+    // Re-runs walk the lists of the flagged particles; the few whose
+    // radius outgrew their list go through the chunk loop again.
+    if ((rc = list_loop(e, LoopKind::Density, true, redo))) return rc;
+    if (relist > 0) {
+      e->list_rebuilt += relist;
+      return queue ? queue_loop(e, LoopKind::Density, 2, e->d.redo, true)
+                   : build_lists(e, true);
+    }
+    return 0;
+  }
+  return queue ? queue_loop(e, LoopKind::Density, 1, e->d.redo, false)
+               : stencil(e, LoopKind::Density, true);
+}
+
+// Gradient records and bounds, gradient loop, extra ghost, force records,
+// force loop, end force.
+int phase_finish(swsph_engine *e) {
+  int rc = 0;
+  const int n = (int)e->n;
+  const bool queue = e->schedule >= 1;
+  HIPCHK(sphk::launch_build_records(LoopKind::Gradient, e->d, 0, n, e->st));
+  e->launches++;
+  if ((rc = refresh_bounds(e, 0, e->ncells))) return rc;
+  if (e->use_lists) {
+    if ((rc = list_loop(e, LoopKind::Gradient, false, 0))) return rc;
+    if (e->list_fallback > 0 &&
+        (rc = queue ? queue_loop(e, LoopKind::Gradient, 1, e->d.lfb, false)
+                    : stencil(e, LoopKind::Gradient, true, e->d.lfb)))
+      return rc;
+  } else if ((rc = queue ? queue_loop(e, LoopKind::Gradient, 0, nullptr, false)
+                         : stencil(e, LoopKind::Gradient, false))) {
+    return rc;
+  }
+  HIPCHK(sphk::launch_extra_ghost(e->d, ghost_args(e), 0, n, e->st));
+  HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, 0, n, e->st));
+  e->launches += 2;
+  if ((rc = queue ? queue_loop(e, LoopKind::Force, 0, nullptr, false)
+                  : stencil(e, LoopKind::Force, false)))
+    return rc;
+  HIPCHK(sphk::launch_end_force(e->d, 0, n, e->props.max_active_bin, e->st));
+  e->launches++;
+  return 0;
+}
+
+int finish_stats(swsph_engine *e, swsph_pass_stats *stats, int rounds,
+                 int64_t l0) {
+  int rc = 0;
+  if (stats) {
+    stats->ghost_rounds = rounds;
+    stats->launches = e->launches - l0;
+    stats->list_rebuilt = e->list_rebuilt;
+    stats->list_fallback = e->list_fallback;
+    stats->n_density_all = e->density_work;
+  }
+  if ((rc = count_into(e, stats))) return rc;
+  if ((rc = check_support(e))) return rc;
+  return collect_timing(e);
+}
+
+// Captures the launches of fn on the engine stream into an executable graph.
+template <class F>
+int capture(swsph_engine *e, hipGraphExec_t *out, F fn) {
+  hipGraph_t g = nullptr;
+  HIPCHK(hipStreamBeginCapture(e->st, hipStreamCaptureModeThreadLocal));
+  e->capturing = true;
+  const int rc = fn();
+  e->capturing = false;
+  const hipError_t ec = hipStreamEndCapture(e->st, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (ec != hipSuccess) return fail(2, "graph capture: %s", hipGetErrorString(ec));
+  const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess)
+    return fail(2, "graph instantiation: %s", hipGetErrorString(ei));
+  return 0;
+}
+
+void drop_graphs(swsph_engine *e) {
+  if (e->g_density) (void)hipGraphExecDestroy(e->g_density);
+  if (e->g_finish) (void)hipGraphExecDestroy(e->g_finish);
+  e->g_density = e->g_finish = nullptr;
+}
+}  // namespace
+
 extern "C" {
 
 const char *swsph_last_error(void) { return g_err.c_str(); }
@@ -622,8 +817,10 @@ int swsph_engine_create(const swsph_props *props, swsph_engine **out) {
     delete e;
     return fail(2, "hipStreamCreate failed");
   }
-  if (hipMalloc((void **)&e->d_redo, 4 * sizeof(int)) != hipSuccess ||
-      hipHostMalloc((void **)&e->h_redo, 4 * sizeof(int), 0) != hipSuccess ||
+  if (hipMalloc((void **)&e->d_redo, 4 * sizeof(unsigned long long)) !=
+          hipSuccess ||
+      hipHostMalloc((void **)&e->h_redo, 4 * sizeof(unsigned long long), 0) !=
+          hipSuccess ||
       hipMalloc((void **)&e->d_cnt, 3 * sizeof(unsigned long long)) != hipSuccess ||
       hipHostMalloc((void **)&e->h_cnt, 3 * sizeof(unsigned long long), 0) !=
           hipSuccess ||
@@ -645,6 +842,8 @@ int swsph_engine_destroy(swsph_engine *e) {
   (void)hipStreamSynchronize(e->st);
   free_dev(e);
   for (hipEvent_t ev : e->ev_pool) (void)hipEventDestroy(ev);
+  if (e->g_density) (void)hipGraphExecDestroy(e->g_density);
+  if (e->g_finish) (void)hipGraphExecDestroy(e->g_finish);
   if (e->d_redo) (void)hipFree(e->d_redo);
   if (e->h_redo) (void)hipHostFree(e->h_redo);
   if (e->d_cnt) (void)hipFree(e->d_cnt);
@@ -659,6 +858,7 @@ int swsph_engine_destroy(swsph_engine *e) {
 int swsph_set_props(swsph_engine *e, const swsph_props *props) {
   if (!e || !props) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
+  e->gen++;  // captured graphs no longer match
   int rc = check_props(props);
   if (rc) return rc;
   for (int k = 0; k < 3; k++)
@@ -671,6 +871,7 @@ int swsph_set_props(swsph_engine *e, const swsph_props *props) {
 int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
   if (!e || !p) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
+  e->gen++;  // captured graphs no longer match
   const int64_t n = p->n;
   // Neighbour-loop row tables pack a particle index in 27 bits.
   if (n < 0 || n > ((int64_t)1 << 27))
@@ -964,6 +1165,8 @@ int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
   int rc = 0;
   const int64_t l0 = e->launches;
   std::vector<int> up;
+  e->density_work = 0;
+  HIPCHK(sphk::launch_zero_u64(e->d_redo + 3, 1, e->st));
   HIPCHK(sphk::launch_zero_u32(e->d_hg2_all, 1, e->st));
   for (int c = 0; c < e->ncells; c++) {
     HIPCHK(sphk::launch_init_density(e->d, e->cell_start[c], e->cell_start[c + 1],
@@ -997,78 +1200,89 @@ int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
   if (stats) {
     stats->ghost_rounds = rounds;
     stats->launches = e->launches - l0;
+    stats->list_rebuilt = 0;
+    stats->list_fallback = 0;
+    stats->n_density_all = e->density_work;
   }
   if ((rc = count_into(e, stats))) return rc;
   return check_support(e);
 }
+
 
 int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats) {
   if (!e) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
   int rc = 0;
   const int64_t l0 = e->launches;
-  const int n = (int)e->n;
-  const GhostArgs ga = ghost_args(e);
-  HIPCHK(sphk::launch_init_density(e->d, 0, n, e->props.max_active_bin,
-                                   e->props.h_max, e->st));
-  e->launches++;
-  const bool lists = e->use_lists;
   e->list_rebuilt = 0;
   e->list_fallback = 0;
-  if (lists) {
-    // First round: every active particle gets a fresh list.
-    if ((rc = ensure_lists(e)) || (rc = build_lists(e, false))) return rc;
-  } else if ((rc = stencil(e, LoopKind::Density, false))) {
-    return rc;
-  }
+  e->density_work = 0;
+  if (e->use_lists && (rc = ensure_lists(e))) return rc;
+  if ((rc = phase_start(e))) return rc;
   int redo = 0, it = 0;
   for (; it < e->props.max_iter; it++) {
-    HIPCHK(sphk::launch_ghost(e->d, ga, 0, n, e->d_redo, e->st));
-    e->launches += 2;
+    if ((rc = phase_ghost(e))) return rc;
     int relist = 0, nolist = 0;
     if ((rc = read_redo(e, &redo, &relist, &nolist))) return rc;
     e->list_fallback += nolist;
     if (redo == 0) break;
-    if (lists) {
-      // Re-runs walk the lists of the flagged particles; the few whose
-      // radius outgrew their list go through the chunk loop again.
-      if ((rc = list_loop(e, LoopKind::Density, true, redo))) return rc;
-      if (relist > 0) {
-        e->list_rebuilt += relist;
-        if ((rc = build_lists(e, true))) return rc;
-      }
-    } else if ((rc = stencil(e, LoopKind::Density, true))) {
-      return rc;
-    }
+    if ((rc = phase_rerun(e, redo, relist))) return rc;
   }
   if (redo > 0)
     return fail(3, "Smoothing length failed to converge on %d particles", redo);
-  HIPCHK(sphk::launch_build_records(LoopKind::Gradient, e->d, 0, n, e->st));
-  e->launches++;
-  if ((rc = refresh_bounds(e, 0, e->ncells))) return rc;
-  if (lists) {
-    if ((rc = list_loop(e, LoopKind::Gradient, false, 0))) return rc;
-    if (e->list_fallback > 0 &&
-        (rc = stencil(e, LoopKind::Gradient, true, e->d.lfb)))
+  e->last_rounds = it + 1;
+  if ((rc = phase_finish(e))) return rc;
+  return finish_stats(e, stats, it + 1, l0);
+}
+
+int swsph_pass_graph(swsph_engine *e, swsph_pass_stats *stats) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  int rc = 0;
+  if (e->use_lists)
+    return fail(1, "graph replay needs the neighbour lists disabled");
+  if (e->n == 0) return fail(1, "no particles set");
+  const int64_t l0 = e->launches;
+  e->list_rebuilt = 0;
+  e->list_fallback = 0;
+  e->density_work = 0;
+  // Re-runs captured: as many as the last pass needed.
+  const int reruns = std::max(1, std::min(e->props.max_iter, e->last_rounds) - 1);
+  if (!e->g_density || e->graph_gen != e->gen || e->graph_reruns != reruns) {
+    drop_graphs(e);
+    // chunk tables must exist before capture (they allocate)
+    for (int k = 0; k < 3; k++)
+      if (!chunk_table(e, e->loop_c[k], &rc)) return rc;
+    if ((rc = capture(e, &e->g_density, [&]() -> int {
+           int r = phase_start(e);
+           for (int k = 0; k < reruns && !r; k++) {
+             if ((r = phase_ghost(e))) break;
+             r = phase_rerun(e, 0, 0);
+           }
+           return r ? r : phase_ghost(e);
+         })))
       return rc;
-  } else if ((rc = stencil(e, LoopKind::Gradient, false))) {
-    return rc;
+    if ((rc = capture(e, &e->g_finish, [&]() -> int { return phase_finish(e); })))
+      return rc;
+    e->graph_gen = e->gen;
+    e->graph_reruns = reruns;
   }
-  HIPCHK(sphk::launch_extra_ghost(e->d, ga, 0, n, e->st));
-  HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, 0, n, e->st));
-  e->launches += 2;
-  if ((rc = stencil(e, LoopKind::Force, false))) return rc;
-  HIPCHK(sphk::launch_end_force(e->d, 0, n, e->props.max_active_bin, e->st));
-  e->launches++;
-  if (stats) {
-    stats->ghost_rounds = it + 1;
-    stats->launches = e->launches - l0;
-    stats->list_rebuilt = e->list_rebuilt;
-    stats->list_fallback = e->list_fallback;
+  HIPCHK(hipGraphLaunch(e->g_density, e->st));
+  int redo = 0;
+  if ((rc = read_redo(e, &redo))) return rc;
+  int rounds = reruns + 1;
+  // more rounds than captured: continue eagerly (and capture more next time)
+  while (redo > 0 && rounds < e->props.max_iter) {
+    if ((rc = phase_rerun(e, redo, 0)) || (rc = phase_ghost(e)) ||
+        (rc = read_redo(e, &redo)))
+      return rc;
+    rounds++;
   }
-  if ((rc = count_into(e, stats))) return rc;
-  if ((rc = check_support(e))) return rc;
-  return collect_timing(e);
+  if (redo > 0)
+    return fail(3, "Smoothing length failed to converge on %d particles", redo);
+  e->last_rounds = rounds;
+  HIPCHK(hipGraphLaunch(e->g_finish, e->st));
+  return finish_stats(e, stats, rounds, l0);
 }
 
 int swsph_save_state(swsph_engine *e) {
@@ -1101,6 +1315,7 @@ int swsph_restore_state(swsph_engine *e) {
 int swsph_set_chunk(swsph_engine *e, int which, int C) {
   if (!e) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
+  e->gen++;  // captured graphs no longer match
   if (which < 0 || which > 2) return fail(1, "loop index %d out of range", which);
   if (C != 8 && C != 16 && C != 32 && C != 64)
     return fail(1, "chunk capacity %d not in {8, 16, 32, 64}", C);
@@ -1112,9 +1327,21 @@ int swsph_set_chunk(swsph_engine *e, int which, int C) {
   return 0;
 }
 
+int swsph_set_schedule(swsph_engine *e, int mode, int waves) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  e->gen++;  // captured graphs no longer match
+  if (mode < 0 || mode > 2) return fail(1, "schedule %d not in {0, 1, 2}", mode);
+  if (waves < 0) return fail(1, "negative wave count");
+  e->schedule = mode;
+  e->queue_waves = waves;
+  return 0;
+}
+
 int swsph_set_lists(swsph_engine *e, int enable, float skin, int slots) {
   if (!e) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
+  e->gen++;  // captured graphs no longer match
   if (!(skin >= 1.f && skin <= 2.f))
     return fail(1, "list skin %g outside [1, 2]", (double)skin);
   if (slots < 1 || slots > 4096)

@@ -34,6 +34,8 @@
 // as by the plain form, so both give bit-identical sums.
 #pragma once
 
+#include <type_traits>
+
 #include "sph_loops.h"
 
 namespace sphl {
@@ -72,14 +74,6 @@ __device__ __forceinline__ int neighbour(const GridDesc &g, int cx, int cy,
   return (v[0] * g.nc[1] + v[1]) * g.nc[2] + v[2];
 }
 
-// Map a block to a chunk so that each XCD (blocks are dealt round-robin over
-// the 8 XCDs) works on a contiguous, spatially coherent range of chunks and
-// reuses neighbour data from its own L2.
-__device__ __forceinline__ int xcd_remap(int b, int n) {
-  const int per = (n + 7) >> 3;
-  return (b & 7) * per + (b >> 3);
-}
-
 __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int off = 1; off < 64; off <<= 1) v = fmaxf(v, __shfl_xor(v, off));
@@ -87,6 +81,58 @@ __device__ __forceinline__ float wave_max(float v) {
 }
 
 constexpr unsigned int kRowMask = (1u << 27) - 1u;
+
+// Waves per block of the batched kernels.  Each wave works on its own chunk
+// or tile with its own LDS.  Several waves per block lift the per-CU
+// workgroup limit off the occupancy, but measured slower on the 128^3 box
+// than one wave per block.
+#ifndef SWSPH_WAVES_PER_BLOCK
+#define SWSPH_WAVES_PER_BLOCK 1
+#endif
+constexpr int kWaves = SWSPH_WAVES_PER_BLOCK;
+
+// Launch bounds of the batched neighbour kernels: block size and, unless
+// SWSPH_NO_OCC_TARGET is defined, the loop's occupancy target.
+#ifdef SWSPH_NO_OCC_TARGET
+#define SWSPH_LOOP_BOUNDS(L) __launch_bounds__(64 * kWaves)
+#else
+#define SWSPH_LOOP_BOUNDS(L) __launch_bounds__(64 * kWaves, L::kWavesPerSimd)
+#endif
+
+// A value every lane of the wave holds (e.g. read from LDS at a common
+// address), moved to a scalar register so that branches on it stay uniform.
+__device__ __forceinline__ int uniform(int v) {
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Orders LDS accesses of one wave around a point where lanes exchange data
+// through LDS (a wave's LDS operations execute in issue order; this stops
+// the compiler from moving them across).
+__device__ __forceinline__ void wave_sync() {
+#if SWSPH_WAVES_PER_BLOCK == 1
+  __syncthreads();  // the block is the wave
+#else
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#endif
+}
+
+// Block b of a batched grid over n work items, wave w of it: each XCD (the
+// blocks are dealt round-robin over the 8 XCDs) works on a contiguous,
+// spatially coherent range of items and reuses neighbour data from its own
+// L2.  Returns n for the waves past the end.
+__device__ __forceinline__ int xcd_item(int b, int w, int n) {
+  const int per = (n + 7) >> 3;
+  const int k = (b >> 3) * kWaves + w;
+  return k < per ? min(n, (b & 7) * per + k) : n;
+}
+
+// Blocks of a batched grid over n items (see xcd_item).
+__host__ __device__ inline int xcd_blocks(int n) {
+  const int per = (n + 7) >> 3;
+  return 8 * ((per + kWaves - 1) / kWaves);
+}
 
 // Image code 9 (sx + 1) + 3 (sy + 1) + (sz + 1) of a translation.
 __device__ __forceinline__ unsigned int image_code(double tx, double ty,
@@ -148,32 +194,44 @@ __device__ __forceinline__ void interact_rows(
   }
 }
 
-// One i-chunk against a list of neighbour cells.  `stencil` selects the 27
-// neighbours of the i-chunk's cell; otherwise the single cell `cj0` with
-// translation (tx0, ty0, tz0).
-template <class L, int C, bool SUBSET, bool BUILD>
+// One i-chunk (TILE: one tile of the work queue) against a list of
+// neighbour cells.  `stencil` selects the 27 neighbours of the i-chunk's
+// cell; otherwise the single cell `cj0` with translation (tx0, ty0, tz0).
+template <class L, int C, bool SUBSET, bool BUILD, bool TILE = false,
+          int W = kWaves>
 __device__ __forceinline__ void run_chunk(
     const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
     const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
     int ich, bool stencil, int cj0, double tx0, double ty0, double tz0,
-    const NbrList &nl) {
+    const NbrList &nl, const TileArgs *tq = nullptr, int tile = 0) {
   using Sh = GatherLds<C>;
   constexpr int S = Sh::S, RPC = Sh::RPC;
   constexpr int G = 64 / C;  // chunks per staging group
   static_assert(!BUILD || (C == kListChunk && !L::kSymmetric),
                 "lists are built by the density loop in list tiles");
-  __shared__ Sh sh;
-  const int lane = threadIdx.x;
+  __shared__ Sh sh_all[W];
+  Sh &sh = sh_all[W == 1 ? 0 : threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
   const int g = lane % C;
   const int s = lane / C;
-  const int i0 = ca.chunk_start[ich], i1 = ca.chunk_start[ich + 1];
-  const int i = i0 + g;
-  bool act = i < i1;
-  if (act) {
-    act = is_active(d.tb[i], la.max_active_bin);
-    if (SUBSET) act = act && la.flag[i];
-    // rebuild only the lists that do not cover the current radius
-    if (BUILD && SUBSET) act = act && !(search_radius2(d.h[i]) <= d.r2b[i]);
+  int i, cell_i;
+  bool act;
+  if (TILE) {
+    // the tile builder applied the loop's predicate
+    const int ts = uniform(tq->tile_start[tile]);
+    act = g < uniform(tq->tile_start[tile + 1]) - ts;
+    i = act ? tq->alist[ts + g] : 0;
+    cell_i = uniform(tq->tile_cell[tile]);
+  } else {
+    i = ca.chunk_start[ich] + g;
+    act = i < ca.chunk_start[ich + 1];
+    cell_i = ca.chunk_cell[ich];
+    if (act) {
+      act = is_active(d.tb[i], la.max_active_bin);
+      if (SUBSET) act = act && la.flag[i];
+      // rebuild only the lists that do not cover the current radius
+      if (BUILD && SUBSET) act = act && !(search_radius2(d.h[i]) <= d.r2b[i]);
+    }
   }
   if (!__any(act)) return;  // uniform
   typename L::I st;
@@ -196,7 +254,6 @@ __device__ __forceinline__ void run_chunk(
   int ncnt = 0;  // list entries found by this lane (BUILD)
 
   // Neighbour table: lane q < nq describes neighbour q.
-  const int cell_i = ca.chunk_cell[ich];
   const int nq = stencil ? 27 : 1;
   int nchq = 0;
   if (lane < nq) {
@@ -231,11 +288,34 @@ __device__ __forceinline__ void run_chunk(
   }
   if (lane < nq) sh.qpre[lane] = incl - nchq;
   const int total = __shfl(incl, nq - 1);
-  // i-chunk box
-  const double *bb = ca.bbox + 6 * ich;
-  const double ilx = bb[0], ily = bb[1], ilz = bb[2];
-  const double ihx = bb[3], ihy = bb[4], ihz = bb[5];
-  __syncthreads();
+  // i-chunk box (a tile's box comes from its lanes)
+  double ilx, ily, ilz, ihx, ihy, ihz;
+  if (TILE) {
+    ilx = act ? st.x : 1e300;
+    ily = act ? st.y : 1e300;
+    ilz = act ? st.z : 1e300;
+    ihx = act ? st.x : -1e300;
+    ihy = act ? st.y : -1e300;
+    ihz = act ? st.z : -1e300;
+#pragma unroll
+    for (int off = 1; off < C; off <<= 1) {
+      ilx = fmin(ilx, __shfl_xor(ilx, off));
+      ily = fmin(ily, __shfl_xor(ily, off));
+      ilz = fmin(ilz, __shfl_xor(ilz, off));
+      ihx = fmax(ihx, __shfl_xor(ihx, off));
+      ihy = fmax(ihy, __shfl_xor(ihy, off));
+      ihz = fmax(ihz, __shfl_xor(ihz, off));
+    }
+  } else {
+    const double *bb = ca.bbox + 6 * ich;
+    ilx = bb[0];
+    ily = bb[1];
+    ilz = bb[2];
+    ihx = bb[3];
+    ihy = bb[4];
+    ihz = bb[5];
+  }
+  wave_sync();
 
   unsigned long long m = 0ull;  // row hits of this lane
   int nrow = 0;                 // rows recorded (uniform, multiple of RPC)
@@ -265,7 +345,7 @@ __device__ __forceinline__ void run_chunk(
     if (pm == 0ull) continue;  // uniform
     const int np = __popcll(pm);
     if (pass) sh.plist[__popcll(pm & ((1ull << lane) - 1ull))] = pe;
-    __syncthreads();
+    wave_sync();
     for (int pb = 0; pb < np; pb += G) {
       // Stage the candidates of up to G surviving chunks.
       {
@@ -284,21 +364,27 @@ __device__ __forceinline__ void run_chunk(
           sh.lpos[lane] = p;
         }
       }
-      __syncthreads();
+      wave_sync();
       for (int c = 0; c < G; c++) {
         if (pb + c >= np) break;  // uniform
-        const unsigned int e = sh.plist[pb + c];
+        const unsigned int e = (unsigned int)uniform((int)sh.plist[pb + c]);
         const int q = (int)(e >> 27);
-        const int js = sh.qj0[q] + (int)(e & kRowMask) * C;
-        const int nj = min(C, sh.qj1[q] - js);
-        // the wave's own chunk: lane g's particle is never its own neighbour
-        const bool own = sh.qcell[q] == cell_i && js == i0;
+        const int js = uniform(sh.qj0[q]) + (int)(e & kRowMask) * C;
+        const int nj = min(C, uniform(sh.qj1[q]) - js);
+        // chunk slot of the lane's own particle (a particle is not its own
+        // neighbour); the check runs only on chunks holding i-particles
+        const int selfl = uniform(sh.qcell[q]) == cell_i ? i - js : -1;
+        const bool selfin = __any(act && selfl >= 0 && selfl < nj);
         // i relative to the neighbour image's corner
         const float xs = (float)(st.x - sh.qorg[3 * q]);
         const float ys = (float)(st.y - sh.qorg[3 * q + 1]);
         const float zs = (float)(st.z - sh.qorg[3 * q + 2]);
         if (lane == 0)
           sh.cinfo[nrow / RPC] = ((unsigned int)q << 27) | (unsigned int)js;
+        // hits of this chunk's rows, bit rr for row rr
+        using Nib = typename std::conditional<(RPC > 32), unsigned long long,
+                                              unsigned int>::type;
+        Nib nib = 0;
 #pragma unroll
         for (int rr = 0; rr < RPC; rr++) {
           if (rr * S >= nj) break;  // uniform
@@ -306,24 +392,25 @@ __device__ __forceinline__ void run_chunk(
           const PosRec p = sh.lpos[c * C + jl];
           const float dx = xs - p.x, dy = ys - p.y, dz = zs - p.z;
           const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-          bool hit = L::kSymmetric ? (r2 < rt2m || r2 < p.hg2m) : r2 < rt2m;
-          if (own) hit = hit && jl != g;
-          m |= (unsigned long long)hit << (nrow + rr);
+          const bool pass =
+              L::kSymmetric ? (r2 < rt2m || r2 < p.hg2m) : r2 < rt2m;
+          nib |= (pass && (!selfin || jl != selfl)) ? ((Nib)1 << rr) : (Nib)0;
         }
+        m |= (unsigned long long)nib << nrow;
         nrow += RPC;
         if (nrow == 64) {
-          __syncthreads();
+          wave_sync();
           interact_rows<L, C, BUILD>(rec, sh, st, s, m, acc, nl, i, rt2,
                                      ncnt);
           m = 0ull;
           nrow = 0;
         }
       }
-      __syncthreads();
+      wave_sync();
     }
   }
   if (nrow > 0) {
-    __syncthreads();
+    wave_sync();
     interact_rows<L, C, BUILD>(rec, sh, st, s, m, acc, nl, i, rt2, ncnt);
   }
   L::template combine<S>(acc);
@@ -341,12 +428,12 @@ __device__ __forceinline__ void run_chunk(
 
 // Batched: every chunk against its 27-cell stencil.
 template <class L, int C, bool SUBSET, bool BUILD>
-__global__ __launch_bounds__(64) void k_loop_stencil(
+__global__ SWSPH_LOOP_BOUNDS(L) void k_loop_stencil(
     DevParts d, LoopArgs la, ChunkArgs ca,
     const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
     NbrList nl) {
-  const int ich = xcd_remap(blockIdx.x, ca.nchunks);
-  if (ich >= ca.nchunks) return;
+  const int ich = xcd_item(blockIdx.x, threadIdx.x >> 6, ca.nchunks);
+  if (ich >= ca.nchunks) return;  // uniform per wave
   run_chunk<L, C, SUBSET, BUILD>(d, la, ca, rec, pos, ich, true, 0, 0.0, 0.0,
                                  0.0, nl);
 }
@@ -366,33 +453,81 @@ __global__ __launch_bounds__(64) void k_loop_task(
   }
   const int ich = ca.cell_chunk[ta.icell[q]] + b;
   const NbrList none{};
-  run_chunk<L, C, SUBSET, false>(d, la, ca, rec, pos, ich, false, ta.jcell[q],
-                                 ta.t[3 * q], ta.t[3 * q + 1], ta.t[3 * q + 2],
-                                 none);
+  run_chunk<L, C, SUBSET, false, false, 1>(d, la, ca, rec, pos, ich, false,
+                                           ta.jcell[q], ta.t[3 * q],
+                                           ta.t[3 * q + 1], ta.t[3 * q + 2],
+                                           none);
+}
+
+// Persistent form: a fixed grid of waves drains the tile queue.  A wave
+// starts on the queue of its XCD (blocks are dealt round-robin over the 8
+// XCDs), takes kGrab tiles per atomic, and moves on to the next queue when
+// its own is exhausted; every queue is visited at most once after it runs
+// dry, so every wave exits.
+constexpr int kGrab = 2;
+template <class L, int C, bool BUILD>
+__global__ SWSPH_LOOP_BOUNDS(L) void k_loop_queue(
+    DevParts d, LoopArgs la, ChunkArgs ca,
+    const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
+    NbrList nl, TileArgs tq) {
+  const int nt = *tq.ntiles;
+  const int per = (nt + 7) >> 3;
+  int q = blockIdx.x & 7;
+  for (int tries = 0; tries < 8;) {
+    int base = 0;
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(&tq.heads[64 * q], kGrab);
+    base = __shfl(base, 0);
+    const int lo = q * per + base;
+    const int hi = min(nt, (q + 1) * per);
+    if (lo >= hi) {  // uniform: this queue is exhausted
+      q = (q + 1) & 7;
+      tries++;
+      continue;
+    }
+    const int end = min(hi, lo + kGrab);
+    for (int t = lo; t < end; t++) {
+      wave_sync();
+      run_chunk<L, C, false, BUILD, true>(d, la, ca, rec, pos, 0, true, 0, 0.0,
+                                          0.0, 0.0, nl, &tq, t);
+    }
+  }
+}
+
+// Grid form of the tile queue: one wave per tile over a grid sized for the
+// largest possible tile count; waves past the device-side count exit.
+template <class L, int C, bool BUILD>
+__global__ SWSPH_LOOP_BOUNDS(L) void k_loop_tiles(
+    DevParts d, LoopArgs la, ChunkArgs ca,
+    const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
+    NbrList nl, TileArgs tq) {
+  const int t = xcd_item(blockIdx.x, threadIdx.x >> 6, *tq.ntiles);
+  if (t >= *tq.ntiles) return;  // uniform per wave
+  run_chunk<L, C, false, BUILD, true>(d, la, ca, rec, pos, 0, true, 0, 0.0, 0.0,
+                                      0.0, nl, &tq, t);
 }
 
 // List loop: lane g + kListChunk * s walks stream s of the neighbour list of
-// particle g of its tile (COMPACT: of d.sel[blockIdx.x * kListChunk + g]),
+// particle g of its tile (COMPACT: of d.sel[wave * kListChunk + g]),
 // re-evaluates the pair contract for each entry and interacts with the ones
 // that pass the loop's own test.  Particles whose list does not cover their
 // search radius are left to the chunk loop.
 template <class L, bool COMPACT>
-__global__ __launch_bounds__(64) void k_loop_list(
+__global__ SWSPH_LOOP_BOUNDS(L) void k_loop_list(
     DevParts d, LoopArgs la, ChunkArgs ca, NbrList nl,
     const typename L::Rec *__restrict__ rec, int nsel) {
   constexpr int C = kListChunk, S = kListStreams;
   static_assert(!L::kSymmetric, "lists hold gather neighbours only");
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x & 63;
   const int g = lane % C;
   const int s = lane / C;
   int i = 0;
   bool act;
   if (COMPACT) {
-    const int k = blockIdx.x * C + g;
+    const int k = (blockIdx.x * kWaves + (threadIdx.x >> 6)) * C + g;
     act = k < nsel;
     if (act) i = d.sel[k];
   } else {
-    const int ich = xcd_remap(blockIdx.x, ca.nchunks);
+    const int ich = xcd_item(blockIdx.x, threadIdx.x >> 6, ca.nchunks);
     if (ich >= ca.nchunks) return;
     i = ca.chunk_start[ich] + g;
     act = i < ca.chunk_start[ich + 1];

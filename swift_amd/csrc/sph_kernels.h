@@ -54,6 +54,21 @@ struct ChunkArgs {
   int C;                    // chunk capacity (8, 16, 32 or 64)
 };
 
+// Work queue of the persistent loops: tiles of <= C particles of one cell,
+// cut from a compact, cell-ordered list of the particles a loop processes
+// (all active ones, or the flagged ones), built on the device.  Tile t
+// holds alist[tile_start[t] .. tile_start[t + 1]).  Waves take tiles from
+// eight queues (one per XCD, each over a contiguous range of tiles) through
+// the counters heads[64 q], and steal from the other queues when their own
+// runs dry.
+struct TileArgs {
+  const int *alist;
+  const int *tile_start;  // [ntiles + 1]
+  const int *tile_cell;   // [ntiles]
+  const int *ntiles;      // device-side count
+  int *heads;             // [8 * 64]
+};
+
 // Structure-of-arrays particle store in HBM, sorted by top-level cell.
 struct DevParts {
   double *px, *py, *pz;
@@ -142,6 +157,20 @@ struct GhostArgs {
 
 enum class LoopKind { Density, Gradient, Force };
 
+// Scratch of the tile builder, sized for n particles and ncells cells.
+struct TileScratch {
+  int *flag;        // [n]
+  int *scan;        // [n + 1]
+  int *blk;         // [n / 1024 + 2]
+  int *cnt;         // [ncells]
+  int *cscan;       // [ncells + 1]
+  int *cblk;        // [ncells / 1024 + 2]
+  int *alist;       // [n]
+  int *tile_start;  // [n / C + ncells + 1] for the smallest C in use
+  int *tile_cell;
+  int *ntiles;      // [1]
+  int *heads;       // [8 * 64]
+};
 namespace sphk {
 hipError_t launch_stencil(LoopKind kind, bool subset, const DevParts &d,
                           const LoopArgs &la, const ChunkArgs &ca,
@@ -160,6 +189,20 @@ hipError_t launch_list_loop(LoopKind kind, bool compact, const DevParts &d,
                             const LoopArgs &la, const ChunkArgs &ca,
                             const NbrList &nl, int nsel, hipStream_t st);
 hipError_t launch_fill_f32(float *p, float v, int n, hipStream_t st);
+
+// Tiles of <= C particles over the active particles (mode 0) or the active
+// particles with flag[i] != 0 (mode 1); all device side, no host sync.
+hipError_t launch_build_tiles(const DevParts &d, const LoopArgs &la, int n,
+                              int ncells, int C, int mode, const int *flag,
+                              const TileScratch &ts, hipStream_t st);
+// A neighbour loop over the tiles: by a persistent grid of `waves` waves
+// (0: full occupancy on every CU) when `persistent`, else by one wave per
+// tile over a grid for `max_tiles` tiles.
+hipError_t launch_queue(LoopKind kind, bool build_lists, const DevParts &d,
+                        const LoopArgs &la, const ChunkArgs &ca,
+                        const TileArgs &ta, const NbrList &nl, int C,
+                        bool persistent, int waves, int max_tiles,
+                        hipStream_t st);
 hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
                        const LoopArgs &la, const ChunkArgs &ca,
                        const TaskArgs &ta, hipStream_t st);
@@ -167,6 +210,7 @@ hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
 hipError_t launch_build_pos(const DevParts &d, const GridDesc &g, float pos_eps,
                             int lo, int hi, hipStream_t st);
 hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t st);
+hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st);
 // Bounding boxes and largest search radius of chunks [c0, c1); the largest
 // search radius over them is max-reduced into *hg2_all (float bits).
 hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
@@ -178,9 +222,11 @@ hipError_t launch_init_density(const DevParts &d, int lo, int hi,
 // counters[0]: particles sent back for another density round (their
 // indices are appended to d.sel); [1]: of those, the ones whose list no
 // longer covers the new radius; [2]: converged particles whose list does
-// not cover their final radius (flagged in d.lfb).
+// not cover their final radius (flagged in d.lfb); [3]: density
+// interactions evaluated for the processed particles, accumulated over the
+// rounds (the pass zeroes it).  [0..2] are zeroed by each launch.
 hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,
-                        int *counters, hipStream_t st);
+                        unsigned long long *counters, hipStream_t st);
 hipError_t launch_extra_ghost(const DevParts &d, const GhostArgs &ga, int lo,
                               int hi, hipStream_t st);
 hipError_t launch_end_force(const DevParts &d, int lo, int hi,

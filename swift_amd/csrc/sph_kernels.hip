@@ -40,10 +40,13 @@ __global__ void k_build_pos(DevParts d, GridDesc g, float pos_eps, int lo,
 // Bounding box (non-inhibited particles) and largest search radius of each
 // chunk in [c0, c1); an empty chunk gets an inverted box that no test passes.
 // A wave takes 64 / C chunks, one lane per particle, and reduces over the C
-// lanes of each chunk.
-__global__ void k_chunk_bounds(DevParts d, const int *chunk_start, int c0,
-                               int c1, int C, double *bbox, float *hg2max,
-                               unsigned int *hg2_all) {
+// lanes of each chunk; the largest radius of the block goes to *hg2_all with
+// one atomic per block.
+constexpr int kBoundsBlock = 1024;
+__global__ __launch_bounds__(kBoundsBlock) void k_chunk_bounds(
+    DevParts d, const int *chunk_start, int c0, int c1, int C, double *bbox,
+    float *hg2max, unsigned int *hg2_all) {
+  __shared__ float wmax[kBoundsBlock / 64];
   const int lane = threadIdx.x & 63;
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int k = c0 + wave * (64 / C) + lane / C;
@@ -73,9 +76,14 @@ __global__ void k_chunk_bounds(DevParts d, const int *chunk_start, int c0,
     }
     hg2max[k] = hm;
   }
-  // positive floats order like their bit patterns
   for (int off = 32; off >= C; off >>= 1) hm = fmaxf(hm, __shfl_xor(hm, off));
-  if (lane == 0) atomicMax(hg2_all, __float_as_uint(hm));
+  if (lane == 0) wmax[threadIdx.x >> 6] = hm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < kBoundsBlock / 64; k++) hm = fmaxf(hm, wmax[k]);
+    // positive floats order like their bit patterns
+    atomicMax(hg2_all, __float_as_uint(hm));
+  }
 }
 
 __global__ void k_build_dens_rec(DevParts d, int lo, int hi) {
@@ -320,15 +328,22 @@ __device__ bool ghost_one(const DevParts &d, const GhostArgs &ga, int i,
 // Counts are reduced over the block first: one atomic per block and
 // counter keeps the same-address traffic small.
 constexpr int kGhostBlock = 1024;
-__global__ __launch_bounds__(kGhostBlock) void k_ghost(DevParts d, GhostArgs ga,
-                                                       int lo, int hi,
-                                                       int *counters) {
+__global__ __launch_bounds__(kGhostBlock) void k_ghost(
+    DevParts d, GhostArgs ga, int lo, int hi, unsigned long long *counters) {
   constexpr int W = kGhostBlock / 64;
   __shared__ int wcnt[3][W];
+  __shared__ unsigned long long wwork[W];
   __shared__ int base;
   const int i = lo + blockIdx.x * kGhostBlock + threadIdx.x;
   bool again = false, relist = false;
-  if (i < hi && d.redo[i]) again = ghost_one(d, ga, i, relist);
+  // density interactions evaluated for this particle in the last round
+  unsigned long long work = 0ull;
+  if (i < hi && d.redo[i]) {
+    work = (unsigned long long)d.nd[i];
+    again = ghost_one(d, ga, i, relist);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) work += __shfl_xor(work, off);
   const unsigned long long ba = __ballot(again);
   const unsigned long long bb = __ballot(again && relist);
   const unsigned long long bc = __ballot(!again && relist);
@@ -337,14 +352,19 @@ __global__ __launch_bounds__(kGhostBlock) void k_ghost(DevParts d, GhostArgs ga,
     wcnt[0][w] = __popcll(ba);
     wcnt[1][w] = __popcll(bb);
     wcnt[2][w] = __popcll(bc);
+    wwork[w] = work;
   }
   __syncthreads();
   if (threadIdx.x < 3) {
-    int t = 0;
-    for (int k = 0; k < W; k++) t += wcnt[threadIdx.x][k];
-    int b0 = 0;
+    unsigned long long t = 0;
+    for (int k = 0; k < W; k++) t += (unsigned long long)wcnt[threadIdx.x][k];
+    unsigned long long b0 = 0;
     if (t) b0 = atomicAdd(&counters[threadIdx.x], t);
-    if (threadIdx.x == 0) base = b0;
+    if (threadIdx.x == 0) base = (int)b0;
+  } else if (threadIdx.x == 3) {
+    unsigned long long t = 0;
+    for (int k = 0; k < W; k++) t += wwork[k];
+    if (t) atomicAdd(&counters[3], t);
   }
   __syncthreads();
   if (again) {
@@ -469,6 +489,124 @@ __global__ void k_count(DevParts d, int n, int max_active_bin,
   }
 }
 
+// ------------------------------------------------------------ tile builder --
+// Device-wide exclusive scans (block sums, one-block scan of the sums,
+// block-local scans) and the compaction / tiling of the particles a loop
+// processes.  Everything stays on the device.
+constexpr int kScanBlock = 1024;
+
+// Exclusive scan of v over the block; *total receives the block sum.
+__device__ __forceinline__ int block_excl_scan(int v, int *wsum, int *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int incl = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(incl, off);
+    if (lane >= off) incl += t;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  if (w == 0) {
+    const int x = lane < kScanBlock / 64 ? wsum[lane] : 0;
+    int xi = x;
+#pragma unroll
+    for (int off = 1; off < kScanBlock / 64; off <<= 1) {
+      const int t = __shfl_up(xi, off);
+      if (lane >= off) xi += t;
+    }
+    if (lane < kScanBlock / 64) wsum[lane] = xi - x;
+    if (lane == kScanBlock / 64 - 1) wsum[kScanBlock / 64] = xi;
+  }
+  __syncthreads();
+  *total = wsum[kScanBlock / 64];
+  return wsum[w] + incl - v;
+}
+
+__device__ __forceinline__ bool tile_pred(const DevParts &d, const LoopArgs &la,
+                                          int mode, const int *flag, int i) {
+  if (!is_active(d.tb[i], la.max_active_bin)) return false;
+  if (mode >= 1 && !flag[i]) return false;
+  // mode 2: only particles whose neighbour list does not cover their radius
+  if (mode == 2 && search_radius2(d.h[i]) <= d.r2b[i]) return false;
+  return true;
+}
+
+__global__ void k_sel_flag(DevParts d, LoopArgs la, int mode, const int *flag,
+                           int n, int *f) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) f[i] = tile_pred(d, la, mode, flag, i) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_up(const int *in, int n,
+                                                        int *blk) {
+  __shared__ int wsum[kScanBlock / 64 + 1];
+  const int i = blockIdx.x * kScanBlock + threadIdx.x;
+  int total;
+  (void)block_excl_scan(i < n ? in[i] : 0, wsum, &total);
+  if (threadIdx.x == 0) blk[blockIdx.x] = total;
+}
+
+// One block: exclusive scan of blk[0..nb) in place, the total to *total_out.
+__global__ __launch_bounds__(kScanBlock) void k_scan_blocks(int *blk, int nb,
+                                                            int *total_out) {
+  __shared__ int wsum[kScanBlock / 64 + 1];
+  const int per = (nb + kScanBlock - 1) / kScanBlock;
+  const int a = threadIdx.x * per, b = min(nb, a + per);
+  int sum = 0;
+  for (int k = a; k < b; k++) sum += blk[k];
+  int total;
+  int run = block_excl_scan(sum, wsum, &total);
+  for (int k = a; k < b; k++) {
+    const int v = blk[k];
+    blk[k] = run;
+    run += v;
+  }
+  if (threadIdx.x == 0) *total_out = total;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_down(const int *in, int n,
+                                                          const int *blk,
+                                                          int *out) {
+  __shared__ int wsum[kScanBlock / 64 + 1];
+  const int i = blockIdx.x * kScanBlock + threadIdx.x;
+  int total;
+  const int ex = block_excl_scan(i < n ? in[i] : 0, wsum, &total);
+  if (i < n) out[i] = blk[blockIdx.x] + ex;
+}
+
+__global__ void k_sel_scatter(const int *f, const int *S, int n, int *alist) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && f[i]) alist[S[i]] = i;
+}
+
+__global__ void k_tile_count(const int *cell_start, const int *S, int ncells,
+                             int C, int *cnt) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncells) return;
+  const int k = S[cell_start[c + 1]] - S[cell_start[c]];
+  cnt[c] = (k + C - 1) / C;
+}
+
+__global__ void k_tile_fill(const int *cell_start, const int *S, const int *T,
+                            int ncells, int C, int n, int *tile_start,
+                            int *tile_cell, int *ntiles) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < ncells) {
+    const int base = S[cell_start[c]];
+    const int k = S[cell_start[c + 1]] - base;
+    const int t0 = T[c];
+    for (int t = 0; t * C < k; t++) {
+      tile_start[t0 + t] = base + t * C;
+      tile_cell[t0 + t] = c;
+    }
+  }
+  if (c == 0) {
+    const int nt = T[ncells];
+    tile_start[nt] = S[n];
+    *ntiles = nt;
+  }
+}
+
 }  // namespace
 
 // --------------------------------------------------------------- launchers --
@@ -479,11 +617,10 @@ static inline int nblk(int n, int b) { return (n + b - 1) / b; }
 template <class L, bool SUB, int C>
 void stencil_c(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
                const typename L::Rec *rec, hipStream_t st) {
-  const int per = (ca.nchunks + 7) >> 3;
   const NbrList none{};
-  hipLaunchKernelGGL((k_loop_stencil<L, C, SUB, false>), dim3(per * 8),
-                     dim3(64), 0, st, d, la, ca, rec, (const PosRec *)d.pos,
-                     none);
+  hipLaunchKernelGGL((k_loop_stencil<L, C, SUB, false>),
+                     dim3(xcd_blocks(ca.nchunks)), dim3(64 * kWaves), 0, st, d,
+                     la, ca, rec, (const PosRec *)d.pos, none);
 }
 
 hipError_t launch_build_lists(bool subset, const DevParts &d,
@@ -491,14 +628,14 @@ hipError_t launch_build_lists(bool subset, const DevParts &d,
                               const NbrList &nl, hipStream_t st) {
   if (ca.nchunks <= 0) return hipSuccess;
   if (ca.C != kListChunk) return hipErrorInvalidValue;
-  const int per = (ca.nchunks + 7) >> 3;
+  const dim3 grid(xcd_blocks(ca.nchunks)), blk(64 * kWaves);
   if (subset)
     hipLaunchKernelGGL((k_loop_stencil<DensityLoop, kListChunk, true, true>),
-                       dim3(per * 8), dim3(64), 0, st, d, la, ca, d.rd,
+                       grid, blk, 0, st, d, la, ca, d.rd,
                        (const PosRec *)d.pos, nl);
   else
     hipLaunchKernelGGL((k_loop_stencil<DensityLoop, kListChunk, false, true>),
-                       dim3(per * 8), dim3(64), 0, st, d, la, ca, d.rd,
+                       grid, blk, 0, st, d, la, ca, d.rd,
                        (const PosRec *)d.pos, nl);
   return hipGetLastError();
 }
@@ -508,12 +645,12 @@ void list_loop(bool compact, const DevParts &d, const LoopArgs &la,
                const ChunkArgs &ca, const NbrList &nl,
                const typename L::Rec *rec, int nsel, hipStream_t st) {
   if (compact) {
-    hipLaunchKernelGGL((k_loop_list<L, true>), dim3(nblk(nsel, kListChunk)),
-                       dim3(64), 0, st, d, la, ca, nl, rec, nsel);
+    hipLaunchKernelGGL((k_loop_list<L, true>),
+                       dim3(nblk(nsel, kListChunk * kWaves)), dim3(64 * kWaves),
+                       0, st, d, la, ca, nl, rec, nsel);
   } else {
-    const int per = (ca.nchunks + 7) >> 3;
-    hipLaunchKernelGGL((k_loop_list<L, false>), dim3(per * 8), dim3(64), 0, st,
-                       d, la, ca, nl, rec, 0);
+    hipLaunchKernelGGL((k_loop_list<L, false>), dim3(xcd_blocks(ca.nchunks)),
+                       dim3(64 * kWaves), 0, st, d, la, ca, nl, rec, 0);
   }
 }
 
@@ -533,6 +670,118 @@ hipError_t launch_list_loop(LoopKind kind, bool compact, const DevParts &d,
       return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_build_tiles(const DevParts &d, const LoopArgs &la, int n,
+                              int ncells, int C, int mode, const int *flag,
+                              const TileScratch &ts, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_int, dim3(8), dim3(64), 0, st, ts.heads, 8 * 64);
+  if (n <= 0) {
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, ts.ntiles, 1);
+    return hipGetLastError();
+  }
+  const int nb = nblk(n, kScanBlock), ncb = nblk(ncells, kScanBlock);
+  hipLaunchKernelGGL(k_sel_flag, dim3(nblk(n, 256)), dim3(256), 0, st, d, la,
+                     mode, flag, n, ts.flag);
+  hipLaunchKernelGGL(k_scan_up, dim3(nb), dim3(kScanBlock), 0, st,
+                     (const int *)ts.flag, n, ts.blk);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanBlock), 0, st, ts.blk,
+                     nb, ts.scan + n);
+  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanBlock), 0, st,
+                     (const int *)ts.flag, n, (const int *)ts.blk, ts.scan);
+  hipLaunchKernelGGL(k_sel_scatter, dim3(nblk(n, 256)), dim3(256), 0, st,
+                     (const int *)ts.flag, (const int *)ts.scan, n, ts.alist);
+  hipLaunchKernelGGL(k_tile_count, dim3(nblk(ncells, 256)), dim3(256), 0, st,
+                     la.cell_start, (const int *)ts.scan, ncells, C, ts.cnt);
+  hipLaunchKernelGGL(k_scan_up, dim3(ncb), dim3(kScanBlock), 0, st,
+                     (const int *)ts.cnt, ncells, ts.cblk);
+  hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(kScanBlock), 0, st, ts.cblk,
+                     ncb, ts.cscan + ncells);
+  hipLaunchKernelGGL(k_scan_down, dim3(ncb), dim3(kScanBlock), 0, st,
+                     (const int *)ts.cnt, ncells, (const int *)ts.cblk,
+                     ts.cscan);
+  hipLaunchKernelGGL(k_tile_fill, dim3(nblk(ncells, 256)), dim3(256), 0, st,
+                     la.cell_start, (const int *)ts.scan, (const int *)ts.cscan,
+                     ncells, C, n, ts.tile_start, ts.tile_cell, ts.ntiles);
+  return hipGetLastError();
+}
+
+// Resident grid of a persistent kernel: blocks per CU at full occupancy
+// times the CU count (queried once per kernel instance).
+template <class L, int C, bool B>
+static int resident_blocks() {
+  static int cached = 0;
+  if (cached == 0) {
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per, k_loop_queue<L, C, B>, 64 * kWaves, 0) != hipSuccess ||
+        per <= 0)
+      return 0;
+    cached = ncu * per * kWaves;
+  }
+  return cached;
+}
+
+template <class L, int C, bool B>
+hipError_t queue_c(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
+                   const TileArgs &ta, const NbrList &nl,
+                   const typename L::Rec *rec, bool persistent, int waves,
+                   int max_tiles, hipStream_t st) {
+  if (!persistent) {
+    hipLaunchKernelGGL((k_loop_tiles<L, C, B>), dim3(xcd_blocks(max_tiles)),
+                       dim3(64 * kWaves), 0, st, d, la, ca, rec,
+                       (const PosRec *)d.pos, nl, ta);
+    return hipGetLastError();
+  }
+  auto kern = k_loop_queue<L, C, B>;
+  if (waves <= 0) waves = resident_blocks<L, C, B>();
+  if (waves <= 0) return hipErrorInvalidValue;
+  // `waves` counts waves; blocks hold kWaves of them
+  hipLaunchKernelGGL(kern, dim3((waves + kWaves - 1) / kWaves),
+                     dim3(64 * kWaves), 0, st, d, la, ca, rec,
+                     (const PosRec *)d.pos, nl, ta);
+  return hipGetLastError();
+}
+
+template <class L>
+hipError_t queue_any(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
+                     const TileArgs &ta, const NbrList &nl,
+                     const typename L::Rec *rec, int C, bool p, int waves,
+                     int mt, hipStream_t st) {
+  switch (C) {
+    case 8: return queue_c<L, 8, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 16: return queue_c<L, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 32: return queue_c<L, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 64: return queue_c<L, 64, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_queue(LoopKind kind, bool build_lists, const DevParts &d,
+                        const LoopArgs &la, const ChunkArgs &ca,
+                        const TileArgs &ta, const NbrList &nl, int C,
+                        bool persistent, int waves, int max_tiles,
+                        hipStream_t st) {
+  switch (kind) {
+    case LoopKind::Density:
+      if (build_lists) {
+        if (C != kListChunk) return hipErrorInvalidValue;
+        return queue_c<DensityLoop, kListChunk, true>(
+            d, la, ca, ta, nl, d.rd, persistent, waves, max_tiles, st);
+      }
+      return queue_any<DensityLoop>(d, la, ca, ta, nl, d.rd, C, persistent,
+                                    waves, max_tiles, st);
+    case LoopKind::Gradient:
+      return queue_any<GradientLoop>(d, la, ca, ta, nl, d.rg, C, persistent,
+                                     waves, max_tiles, st);
+    case LoopKind::Force:
+      return queue_any<ForceLoop>(d, la, ca, ta, nl, d.rf, C, persistent,
+                                  waves, max_tiles, st);
+  }
+  return hipErrorInvalidValue;
 }
 
 hipError_t launch_fill_f32(float *p, float v, int n, hipStream_t st) {
@@ -611,6 +860,11 @@ hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
   return hipErrorInvalidValue;
 }
 
+hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_u64, dim3(nblk(n, 64)), dim3(64), 0, st, p, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_int, dim3(nblk(n, 64)), dim3(64), 0, st, (int *)p,
                      n);
@@ -632,8 +886,9 @@ hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
   if (c1 <= c0) return hipSuccess;
   if (C != 8 && C != 16 && C != 32 && C != 64) return hipErrorInvalidValue;
   const int waves = nblk(c1 - c0, 64 / C);
-  hipLaunchKernelGGL(k_chunk_bounds, dim3(nblk(waves, 4)), dim3(256), 0, st, d,
-                     chunk_start, c0, c1, C, bbox, hg2max, hg2_all);
+  hipLaunchKernelGGL(k_chunk_bounds, dim3(nblk(waves, kBoundsBlock / 64)),
+                     dim3(kBoundsBlock), 0, st, d, chunk_start, c0, c1, C, bbox,
+                     hg2max, hg2_all);
   return hipGetLastError();
 }
 
@@ -664,8 +919,9 @@ hipError_t launch_init_density(const DevParts &d, int lo, int hi,
 }
 
 hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,
-                        int *counters, hipStream_t st) {
-  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, counters, 3);
+                        unsigned long long *counters, hipStream_t st) {
+  // counters[3] accumulates over the rounds of a pass (zeroed by the pass)
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, counters, 3);
   if (hi > lo)
     hipLaunchKernelGGL(k_ghost, dim3(nblk(hi - lo, kGhostBlock)),
                        dim3(kGhostBlock), 0, st, d, ga, lo, hi, counters);

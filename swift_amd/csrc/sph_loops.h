@@ -6,6 +6,17 @@
 #include "sph_device.h"
 #include "sph_kernels.h"
 
+// Occupancy targets (waves per SIMD) of the batched neighbour kernels.
+#ifndef SWSPH_OCC_DENSITY
+#define SWSPH_OCC_DENSITY 7
+#endif
+#ifndef SWSPH_OCC_GRADIENT
+#define SWSPH_OCC_GRADIENT 7
+#endif
+#ifndef SWSPH_OCC_FORCE
+#define SWSPH_OCC_FORCE 5
+#endif
+
 namespace sphl {
 using namespace sph;
 
@@ -27,6 +38,8 @@ struct DensI {
 };
 
 struct DensityLoop {
+  // occupancy target of the batched kernels (waves per SIMD)
+  static constexpr int kWavesPerSimd = SWSPH_OCC_DENSITY;
   using Rec = DensRec;
   using Acc = DensAcc;
   using I = DensI;
@@ -125,6 +138,8 @@ struct GradI {
 };
 
 struct GradientLoop {
+  // occupancy target of the batched kernels (waves per SIMD)
+  static constexpr int kWavesPerSimd = SWSPH_OCC_GRADIENT;
   using Rec = GradRec;
   using Acc = GradAcc;
   using I = GradI;
@@ -204,6 +219,8 @@ struct ForceI {
 };
 
 struct ForceLoop {
+  // occupancy target of the batched kernels (waves per SIMD)
+  static constexpr int kWavesPerSimd = SWSPH_OCC_FORCE;
   using Rec = ForceRec;
   using Acc = ForceAcc;
   using I = ForceI;

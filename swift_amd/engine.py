@@ -53,7 +53,7 @@ class PassStats(C.Structure):
     _fields_ = [("ghost_rounds", C.c_int), ("n_density", C.c_int64),
                 ("n_gradient", C.c_int64), ("n_force", C.c_int64),
                 ("launches", C.c_int64), ("list_rebuilt", C.c_int64),
-                ("list_fallback", C.c_int64)]
+                ("list_fallback", C.c_int64), ("n_density_all", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -195,6 +195,12 @@ class Engine:
         check(lib().swsph_pass_batched(self._h, C.byref(s)), "swsph_pass_batched")
         return s.as_dict()
 
+    def pass_graph(self) -> dict:
+        """The batched pass replayed from captured HIP graphs."""
+        s = PassStats()
+        check(lib().swsph_pass_graph(self._h, C.byref(s)), "swsph_pass_graph")
+        return s.as_dict()
+
     LOOPS = {"density": 0, "gradient": 1, "force": 2}
 
     def set_chunk(self, loop: str, C: int) -> None:
@@ -208,6 +214,13 @@ class Engine:
         check(lib().swsph_get_chunk(self._h, self.LOOPS[loop], C.byref(c)),
               "swsph_get_chunk")
         return c.value
+
+    def set_schedule(self, mode: int = 1, waves: int = 0) -> None:
+        """Batched loops: 0 one wave per chunk (grid), 1 one wave per tile
+        of a device-built tile queue, 2 persistent waves draining that queue
+        (`waves` 0: fill every CU)."""
+        check(lib().swsph_set_schedule(self._h, int(mode), int(waves)),
+              "swsph_set_schedule")
 
     def set_lists(self, enable: bool = True, skin: float = 1.2,
                   slots: int = 48) -> None:

@@ -72,6 +72,8 @@ def test_modulated(orc, mode):
     _check(orc, props, st, res)
     _h_converged(orc, props, st, res)
     assert stats["ghost_rounds"] >= 2
+    # work counter: every round's density interactions, the last included
+    assert stats["n_density_all"] > stats["n_density"]
 
 
 def test_plain_matches_batched():
@@ -273,3 +275,96 @@ def test_lists_clustered(orc):
     for k in ref:
         assert np.array_equal(ref[k], res[k]), k
     _check(orc, props, st, res)
+
+
+def _run_sched(st, props, mode, waves=0, lists=False):
+    from swift_amd import Engine
+    with Engine(props) as eng:
+        eng.set_schedule(mode, waves)
+        eng.set_lists(enable=lists)
+        eng.set_particles(st)
+        stats = eng.pass_batched()
+        res = eng.results()
+    return res, stats
+
+
+@pytest.mark.parametrize("case", ["modulated", "clustered", "half_active"])
+def test_queue_matches_grid(orc, case):
+    """The work-queue schedules (device-built tiles of the active or re-run
+    particles, one wave per tile or persistent waves) and the
+    one-wave-per-chunk grid visit every particle's neighbours in the same
+    order per stream: all fields are bit-identical, including with a tiny
+    persistent grid (8 waves), where every wave drains many tiles and steals
+    from the other queues."""
+    from swift_amd import HydroProps, ics
+    if case == "clustered":
+        st = ics.clustered_box(12000, seed=9, n_spheres=6, a_min=0.01, a_max=0.04)
+    else:
+        st = ics.modulated_box(16, seed=23, amp=0.5)
+    props = HydroProps(ncell=(4, 4, 4))
+    if case == "half_active":
+        base, _ = _run_sched(st, props, 0)
+        for k in ("h", "rho", "pressure", "soundspeed", "f", "balsara",
+                  "alpha_visc", "alpha_diff"):
+            st[k] = base[k].astype(st[k].dtype)
+        ics.set_active_fraction(st, 0.5, seed=2)
+    grid, s0 = _run_sched(st, props, 0)
+    queue, s1 = _run_sched(st, props, 1)
+    pers, s2 = _run_sched(st, props, 2)
+    small, s3 = _run_sched(st, props, 2, waves=8)
+    for k in grid:
+        assert np.array_equal(grid[k], queue[k]), k
+        assert np.array_equal(grid[k], pers[k]), k
+        assert np.array_equal(grid[k], small[k]), k
+    assert s0["n_density"] == s1["n_density"] == s2["n_density"] == s3["n_density"]
+    _check(orc, props, st, queue)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_graph_replay_matches_eager(orc, mode):
+    """The batched pass replayed from captured HIP graphs gives the eager
+    pass's results bit for bit, on every replay, with eager passes and
+    synchronisations in between; a start that needs more ghost rounds than
+    were captured continues eagerly and still matches."""
+    from swift_amd import Engine, HydroProps, ics
+    st = ics.modulated_box(16, seed=31, amp=0.5)
+    props = HydroProps(ncell=(4, 4, 4))
+    with Engine(props) as eng:
+        eng.set_schedule(mode)
+        eng.set_particles(st)
+        eng.save_state()
+        eng.restore_state()
+        s0 = eng.pass_batched()
+        ref = eng.results()
+        for rep in range(3):
+            eng.restore_state()
+            s = eng.pass_graph()
+            res = eng.results()
+            for k in ref:
+                assert np.array_equal(ref[k], res[k]), (rep, k)
+            assert s["ghost_rounds"] == s0["ghost_rounds"]
+            assert s["n_density_all"] == s0["n_density_all"]
+            eng.restore_state()
+            eng.pass_batched()
+            eng.sync()
+    _check(orc, props, st, res)
+    # a poor initial h needs more rounds than the graph captured
+    st2 = {k: np.array(v, copy=True) for k, v in st.items()}
+    st2["h"] = (st2["h"] * np.float32(0.55)).astype(np.float32)
+    with Engine(props) as eng:
+        eng.set_schedule(mode)
+        eng.set_particles(st2)
+        eng.save_state()
+        g = eng.pass_graph()
+        gres = eng.results()
+        eng.restore_state()
+        e = eng.pass_batched()
+        eres = eng.results()
+        eng.restore_state()
+        g2 = eng.pass_graph()  # re-captured with the longer round count
+        g2res = eng.results()
+    assert e["ghost_rounds"] > 4
+    assert g["ghost_rounds"] == e["ghost_rounds"] == g2["ghost_rounds"]
+    for k in eres:
+        assert np.array_equal(eres[k], gres[k]), k
+        assert np.array_equal(eres[k], g2res[k]), k
