@@ -64,6 +64,88 @@ def _read_traffic(path, kernel_key):
     return None
 
 
+def serving_bench(duration_s: float, rate: float, seed: int = 0,
+                  max_inflight: int = 8, pool_size: int = 24) -> dict:
+    """Poisson stream of independent boxes through the request server:
+    N log-uniform in [16^3, 96^3] (rounded to a cube), each a modulated box
+    with its own seed and amplitude U[0, 0.8], one density+force step per
+    request.  Boxes come from a pool generated beforehand (generating one
+    per request would time numpy, not the server)."""
+    import threading
+    from swift_amd import HydroProps, Server, ics
+    rng = np.random.default_rng(seed)
+    pool = []
+    for k in range(pool_size):
+        N = float(np.exp(rng.uniform(np.log(16 ** 3), np.log(96 ** 3))))
+        n = int(round(N ** (1.0 / 3.0)))
+        amp = float(rng.uniform(0.0, 0.8))
+        st = ics.modulated_box(n, seed=1000 + k, amp=amp)
+        # H <= cell width at the lowest density (rho = 1 - amp)
+        props = HydroProps(ncell=(max(3, n // 5),) * 3)
+        pool.append((props, st))
+    print(f"bench: serving pool of {pool_size} boxes ready", file=sys.stderr,
+          flush=True)
+    infos = []
+    ids = []
+    lock = threading.Lock()
+    fields = ["h", "rho", "pressure", "soundspeed", "v_sig", "alpha_visc",
+              "alpha_diff", "a_hydro", "u_dt", "h_dt", "n_density", "n_force",
+              "min_ngb_time_bin"]
+    with Server(max_inflight=max_inflight, fields=fields) as srv:
+        done = threading.Event()
+
+        def collect():
+            k = 0
+            while True:
+                with lock:
+                    if k < len(ids):
+                        rid = ids[k]
+                    elif done.is_set():
+                        return
+                    else:
+                        rid = None
+                if rid is None:
+                    time.sleep(0.0005)
+                    continue
+                infos.append(srv.wait(rid))
+                srv.release(rid)
+                k += 1
+
+        th = threading.Thread(target=collect)
+        th.start()
+        t0 = time.perf_counter()
+        t_next = t0
+        while True:
+            t_next += rng.exponential(1.0 / rate)
+            if t_next - t0 > duration_s:
+                break
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            props, st = pool[int(rng.integers(pool_size))]
+            rid = srv.submit(props, st)
+            with lock:
+                ids.append(rid)
+        done.set()
+        th.join()
+        wall = time.perf_counter() - t0
+    lat = [i["t_done_ms"] - i["t_submit_ms"] for i in infos]
+    queue = [i["t_admit_ms"] - i["t_submit_ms"] for i in infos]
+    stages = {"upload_ms": [i["t_uploaded_ms"] - i["t_start_ms"] for i in infos],
+              "pass_ms": [i["t_passed_ms"] - i["t_uploaded_ms"] for i in infos],
+              "fetch_ms": [i["t_done_ms"] - i["t_passed_ms"] for i in infos]}
+    inter = sum(i["n_density"] + i["n_force"] for i in infos)
+    return {"workload": f"Poisson stream, mean {rate:g} req/s for {duration_s:g} s, "
+                        "N log-uniform in [16^3, 96^3], modulated boxes with "
+                        "amplitude U[0, 0.8], one density+force step each",
+            "requests": len(infos), "max_inflight": max_inflight,
+            "p50_latency_ms": _pct(lat, 50), "p99_latency_ms": _pct(lat, 99),
+            "p50_queue_ms": _pct(queue, 50), "p99_queue_ms": _pct(queue, 99),
+            "mean_stage_ms": {k: float(np.mean(v)) for k, v in stages.items()},
+            "requests_per_s": len(infos) / wall,
+            "interactions_per_s": inter / wall, "wall_s": wall}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,6 +160,13 @@ def main() -> int:
                     help="host threads for the CPU baseline (0: min(16, cores))")
     ap.add_argument("--cpu-n", type=int, default=0,
                     help="particles per axis of the CPU sample (0: same as --n)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default schedule and tile shapes")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="1: replay each step from captured HIP graphs")
+    ap.add_argument("--serving-s", type=float, default=5.0,
+                    help="seconds of the serving stream (0: skip)")
+    ap.add_argument("--serving-rate", type=float, default=200.0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -93,15 +182,19 @@ def main() -> int:
     eng = Engine(props)
     eng.set_particles(st)
     eng.save_state()
+    tuning = None
+    if not args.no_tune:
+        tuning = eng.autotune(repeats=2)
+        print(f"bench: autotuned {tuning}", file=sys.stderr, flush=True)
+    run = eng.pass_graph if args.graph else eng.pass_batched
 
     def step():
         eng.restore_state()
-        return eng.pass_batched()
+        return run()
 
     for _ in range(args.warmup):
         step()
     print(f"bench: warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
-    eng.set_timing(True)
     eng.sync()
     times = []
     stats = None
@@ -112,14 +205,27 @@ def main() -> int:
         times.append(time.perf_counter() - ts)
     eng.sync()
     total = time.perf_counter() - t0
-    ms, nl = eng.timing()
     print(f"bench: {args.steps} timed steps in {total:.3f} s", file=sys.stderr,
           flush=True)
     inter = stats["n_density"] + stats["n_force"]
     value = args.steps * inter / total
 
-    # Roofline of the dominant neighbour-loop kernel (HIP events on the
-    # engine stream over the timed region).
+    # Per-loop kernel times: HIP events recorded on the engine stream around
+    # every neighbour-loop launch (tile build included) of the same number of
+    # eager passes with the same kernels (a captured graph carries no
+    # per-launch events).
+    eng.set_timing(True)
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        eng.restore_state()
+        eager_stats = eng.pass_batched()
+    eng.sync()
+    eager_total = time.perf_counter() - t1
+    ms, nl = eng.timing()
+    eng.set_timing(False)
+    assert eager_stats["n_density"] == stats["n_density"]
+
+    # Roofline of the dominant neighbour-loop kernel.
     names = ["density_loop", "gradient_loop", "force_loop"]
     dom = int(np.argmax(ms))
     kernels = {}
@@ -182,6 +288,9 @@ def main() -> int:
                                   "force": stats["n_force"],
                                   "gradient": stats["n_gradient"]},
         "ghost_rounds": stats["ghost_rounds"],
+        "pass": "graph replay" if args.graph else "eager batched",
+        "eager_ms_per_step": 1e3 * eager_total / args.steps,
+        "tuning": tuning,
         "launches_per_step": stats["launches"],
         "kernels": kernels,
         "roofline": roofline,
@@ -208,6 +317,11 @@ def main() -> int:
         print(f"bench: plain pass {np.mean(ptimes):.2f} s", file=sys.stderr,
               flush=True)
     eng.close()
+
+    if args.serving_s > 0:
+        out["serving"] = serving_bench(args.serving_s, args.serving_rate)
+        print(f"bench: serving {out['serving']['requests']} requests",
+              file=sys.stderr, flush=True)
 
     if not args.no_cpu:
         from oracle import oracle as orc

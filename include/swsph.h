@@ -187,6 +187,22 @@ int swsph_get_chunk(swsph_engine *e, int which, int *C);
  * over cells of very uneven occupancy; results do not depend on the mode. */
 int swsph_set_schedule(swsph_engine *e, int mode, int waves);
 
+/* Autotuner of the batched pass: times every schedule (0, 1, 2 of
+ * swsph_set_schedule) with chunk capacities 8, 16 and 32 on the engine's
+ * current particles (`repeats` timed passes each, HIP events per loop),
+ * keeps the schedule with the smallest summed loop time and, under it, the
+ * fastest capacity of each loop, and applies them.  The pass state (h and
+ * the (io) fields) is saved before and restored after, so results are those
+ * of an untuned engine.  No counterpart in the reference. */
+typedef struct swsph_tune_result {
+  int schedule;       /* chosen schedule */
+  int chunk[3];       /* chosen chunk capacity of each loop */
+  double loop_ms[3];  /* loop times per pass of the choice (ms) */
+  double pass_ms;     /* wall time per pass, chosen schedule, capacity 16 */
+  int trials;         /* configurations timed */
+} swsph_tune_result;
+int swsph_autotune(swsph_engine *e, int repeats, swsph_tune_result *out);
+
 /* Neighbour lists of the batched pass.  With lists enabled, the first
  * density round keeps, per particle, the candidates within `skin` times its
  * search radius (`slots` entries per candidate stream, 4 streams); the
@@ -198,6 +214,53 @@ int swsph_set_schedule(swsph_engine *e, int mode, int waves);
  * candidates; neighbour sets and results do not depend on the setting.
  * Defaults: disabled, skin 1.2, 48 slots. */
 int swsph_set_lists(swsph_engine *e, int enable, float skin, int slots);
+
+/* ---- serving -------------------------------------------------------------
+ * A request server for independent periodic boxes (no counterpart in the
+ * reference, whose engine steps one box).  Each request is one batched
+ * pass (density -> ghost -> gradient -> force) of one box.  Requests are
+ * admitted first come, first served while a slot (of max_inflight) is free
+ * and the device bytes of the admitted boxes stay within hbm_budget_bytes
+ * (<= 0: 80 % of the free HBM at creation); every admitted request runs on
+ * its own engine and HIP stream, so the work of concurrent requests
+ * overlaps on the GPU, and a finished request frees its slot for the next
+ * at once.  Each slot keeps its engine between requests and reuses its
+ * device buffers when the next box fits (the budget counts the admitted
+ * requests, not this retained capacity).  A box submitted with keep != 0 stays resident after its step
+ * (its id is the request's id) and is stepped again with
+ * swsph_server_step without a new upload.  Results stay on the host until
+ * swsph_server_release.  Errors: swsph_server_last_error(). */
+typedef struct swsph_server swsph_server;
+typedef struct swsph_server_opts {
+  double hbm_budget_bytes;
+  int max_inflight;     /* concurrent requests (1..64) */
+  int use_graphs;       /* replay each pass from captured HIP graphs */
+  unsigned int fields;  /* bit f: return swsph_field f (0: all) */
+} swsph_server_opts;
+typedef struct swsph_request_info {
+  int64_t box;           /* resident box id (0: not kept) */
+  int ghost_rounds;
+  int64_t n_density, n_force;
+  double t_submit_ms, t_admit_ms, t_start_ms, t_done_ms;  /* steady clock */
+  double t_uploaded_ms;  /* engine ready and particles on the device */
+  double t_passed_ms;    /* pass finished, before the results are copied */
+} swsph_request_info;
+const char *swsph_server_last_error(void);
+int swsph_server_create(const swsph_server_opts *opts, swsph_server **out);
+/* Finishes the admitted requests; requests still queued fail. */
+int swsph_server_destroy(swsph_server *s);
+/* Copies the particles and queues one step; returns at once. */
+int swsph_server_submit(swsph_server *s, const swsph_props *props,
+                        const swsph_host_parts *parts, int keep, int64_t *id);
+int swsph_server_step(swsph_server *s, int64_t box, int64_t *id);
+/* Blocks until the request is done; status 3 if it failed. */
+int swsph_server_wait(swsph_server *s, int64_t id, swsph_request_info *info);
+/* One result field (as swsph_get_field) of a finished request. */
+int swsph_server_get(swsph_server *s, int64_t id, int field, void *host_out);
+int swsph_server_release(swsph_server *s, int64_t id);
+int swsph_server_close_box(swsph_server *s, int64_t box);
+int swsph_server_stats(swsph_server *s, double *bytes_in_use, int *inflight,
+                       int *queued);
 
 /* Interaction counters summed over active particles (device reduction). */
 int swsph_count_interactions(swsph_engine *e, int64_t out3[3]);

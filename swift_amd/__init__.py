@@ -7,8 +7,9 @@ task entry points plus synthetic initial conditions.
 """
 from ._lib import SwsphError, lib, LIB_PATH
 from .engine import Engine, FIELDS
+from .server import Server
 from .ics import HydroProps
 from . import ics
 
-__all__ = ["Engine", "FIELDS", "HydroProps", "SwsphError", "ics", "lib",
+__all__ = ["Engine", "FIELDS", "HydroProps", "Server", "SwsphError", "ics", "lib",
            "LIB_PATH"]

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -86,6 +87,9 @@ struct swsph_engine {
   struct ChunkTable {
     int C = 0;
     int nchunks = 0;
+    bool valid = false;    // built for the current particles
+    int cap_chunks = 0;    // device capacity (chunks, cells)
+    int cap_cells = 0;
     std::vector<int> cell_chunk_h;  // [ncells + 1]
     int *d_start = nullptr, *d_cell_chunk = nullptr, *d_chunk_cell = nullptr;
     double *d_bbox = nullptr;
@@ -110,6 +114,12 @@ struct swsph_engine {
   int queue_waves = 0;  // 0: one full-occupancy wave set per CU
   TileScratch ts{};
   int tiles_c = 0;  // capacity of the all-active tile table in ts, 0: none
+  // Device capacity (particles, cells): buffers are reused by later
+  // swsph_set_particles calls that fit, without reallocation.
+  size_t cap_n = 0;
+  int cap_cells = 0;
+  bool saved_valid = false;  // swsph_save_state holds this particle set
+  bool need_particles = false;  // grid changed since the last upload
   // Graph replay of the batched pass: the density phase (with a fixed
   // number of re-runs) and the gradient-to-end-force phase, captured for
   // the configuration generation `graph_gen`.
@@ -119,6 +129,7 @@ struct swsph_engine {
   int last_rounds = 4;  // ghost rounds of the last pass
   bool capturing = false;
   hipGraphExec_t g_density = nullptr, g_finish = nullptr;
+  std::vector<float *> tune_saved;  // pass state kept by the autotuner
 };
 
 namespace {
@@ -195,7 +206,10 @@ void free_dev(swsph_engine *e) {
   e->nl_elems = 0;
   e->d_cell_start = nullptr;
   e->d_tiles = nullptr;
+  e->cap_n = 0;
+  e->cap_cells = 0;
   free_saved(e);
+  e->saved_valid = false;
   for (auto &t : e->tables) {
     void *tp[] = {t.d_start, t.d_cell_chunk, t.d_chunk_cell, t.d_bbox,
                   t.d_hg2max};
@@ -210,15 +224,25 @@ void free_dev(swsph_engine *e) {
 int refresh_bounds(swsph_engine *e, int c_lo, int c_hi);
 
 swsph_engine::ChunkTable *chunk_table(swsph_engine *e, int C, int *rc) {
-  for (auto &t : e->tables)
-    if (t.C == C) return &t;
   *rc = 0;
+  swsph_engine::ChunkTable *tp = nullptr;
+  for (auto &t : e->tables)
+    if (t.C == C) tp = &t;
+  if (tp && tp->valid) return tp;
   if (C != 8 && C != 16 && C != 32 && C != 64) {
     *rc = fail(1, "chunk capacity %d not in {8, 16, 32, 64}", C);
     return nullptr;
   }
-  swsph_engine::ChunkTable t;
-  t.C = C;
+  if (!tp) {
+    if (e->tables.size() == e->tables.capacity()) {
+      *rc = fail(2, "chunk table limit reached");
+      return nullptr;
+    }
+    e->tables.emplace_back();
+    tp = &e->tables.back();
+    tp->C = C;
+  }
+  swsph_engine::ChunkTable &t = *tp;
   std::vector<int> start, cell_of;
   t.cell_chunk_h.assign(e->ncells + 1, 0);
   for (int c = 0; c < e->ncells; c++) {
@@ -231,25 +255,42 @@ swsph_engine::ChunkTable *chunk_table(swsph_engine *e, int C, int *rc) {
   t.nchunks = (int)start.size();
   t.cell_chunk_h[e->ncells] = t.nchunks;
   start.push_back((int)e->n);
-  auto up = [&](int **dst, const std::vector<int> &v) -> int {
-    HIPCHK(dalloc(dst, std::max<size_t>(1, v.size())));
+  // (re)allocate only when the capacity does not suffice
+  if (t.nchunks > t.cap_chunks || e->ncells > t.cap_cells) {
+    void *tp2[] = {t.d_start, t.d_cell_chunk, t.d_chunk_cell, t.d_bbox,
+                   t.d_hg2max};
+    for (void *q : tp2)
+      if (q) (void)hipFree(q);
+    t.d_start = t.d_cell_chunk = t.d_chunk_cell = nullptr;
+    t.d_bbox = nullptr;
+    t.d_hg2max = nullptr;
+    const size_t nk = (size_t)std::max(1, t.nchunks);
+    if (dalloc(&t.d_start, nk + 1) != hipSuccess ||
+        dalloc(&t.d_cell_chunk, (size_t)e->ncells + 1) != hipSuccess ||
+        dalloc(&t.d_chunk_cell, nk) != hipSuccess ||
+        dalloc(&t.d_bbox, 6 * nk) != hipSuccess ||
+        dalloc(&t.d_hg2max, nk) != hipSuccess) {
+      *rc = fail(2, "allocation of chunk table failed");
+      return nullptr;
+    }
+    t.cap_chunks = t.nchunks;
+    t.cap_cells = e->ncells;
+  }
+  auto up = [&](int *dst, const std::vector<int> &v) -> int {
     if (!v.empty())
-      HIPCHK(hipMemcpy(*dst, v.data(), sizeof(int) * v.size(),
-                       hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpyAsync(dst, v.data(), sizeof(int) * v.size(),
+                            hipMemcpyHostToDevice, e->st));
     return 0;
   };
-  if ((*rc = up(&t.d_start, start)) || (*rc = up(&t.d_cell_chunk, t.cell_chunk_h)) ||
-      (*rc = up(&t.d_chunk_cell, cell_of)))
+  if ((*rc = up(t.d_start, start)) || (*rc = up(t.d_cell_chunk, t.cell_chunk_h)) ||
+      (*rc = up(t.d_chunk_cell, cell_of)))
     return nullptr;
-  if (hipMalloc((void **)&t.d_bbox, sizeof(double) * 6 * std::max(1, t.nchunks)) !=
-          hipSuccess ||
-      hipMalloc((void **)&t.d_hg2max, sizeof(float) * std::max(1, t.nchunks)) !=
-          hipSuccess) {
-    *rc = fail(2, "allocation of chunk table failed");
+  // the host vectors must outlive the asynchronous copies
+  if (hipStreamSynchronize(e->st) != hipSuccess) {
+    *rc = fail(2, "chunk table upload failed");
     return nullptr;
   }
-  e->tables.push_back(t);
-  swsph_engine::ChunkTable *tp = &e->tables.back();
+  t.valid = true;
   if ((*rc = refresh_bounds(e, 0, e->ncells))) return nullptr;
   return tp;
 }
@@ -275,6 +316,7 @@ int refresh_bounds(swsph_engine *e, int c_lo, int c_hi) {
   HIPCHK(sphk::launch_build_pos(e->d, e->grid, pos_eps(e), lo, hi, e->st));
   e->launches++;
   for (auto &t : e->tables) {
+    if (!t.valid) continue;
     HIPCHK(sphk::launch_chunk_bounds(e->d, t.d_start, t.cell_chunk_h[c_lo],
                                      t.cell_chunk_h[c_hi], t.C, t.d_bbox,
                                      t.d_hg2max, e->d_hg2_all, e->st));
@@ -283,8 +325,10 @@ int refresh_bounds(swsph_engine *e, int c_lo, int c_hi) {
   return 0;
 }
 
+// Device buffers for up to n particles and e->ncells cells.
 int alloc_dev(swsph_engine *e, size_t n) {
   DevParts &d = e->d;
+  HIPCHK(dalloc(&e->d_cell_start, (size_t)e->ncells + 1));
   HIPCHK(dalloc(&d.px, n));
   HIPCHK(dalloc(&d.py, n));
   HIPCHK(dalloc(&d.pz, n));
@@ -318,13 +362,29 @@ int alloc_dev(swsph_engine *e, size_t n) {
     HIPCHK(dalloc(&t.ntiles, 1));
     HIPCHK(dalloc(&t.heads, 8 * 64));
   }
-  // Zero everything once so that fields never written for inactive
-  // particles read as zero.
+  e->cap_n = n;
+  e->cap_cells = e->ncells;
+  return 0;
+}
+
+// Zero the particle fields of a new particle set of n (so that fields never
+// written for inactive particles read as zero) and forget its lists.
+int reset_dev(swsph_engine *e, size_t n) {
+  DevParts &d = e->d;
+  float *f32[] = {d.vx, d.vy, d.vz, d.m, d.u, d.h, d.rho, d.rho_dh,
+                  d.wc, d.wc_dh, d.divv, d.rotx, d.roty, d.rotz,
+                  d.left, d.right, d.P, d.c, d.f, d.bal, d.vsig,
+                  d.lapu, d.avmax, d.alpha, d.adiff, d.divv_prev,
+                  d.divv_dt, d.ax, d.ay, d.az, d.udt, d.hdt};
+  int *i32[] = {d.nd, d.redo, d.ng, d.nf, d.lfb, d.sel, d.cell};
   const size_t nn = n ? n : 1;
-  for (float **p : f32) HIPCHK(hipMemsetAsync(*p, 0, sizeof(float) * nn, e->st));
-  for (int **p : i32) HIPCHK(hipMemsetAsync(*p, 0, sizeof(int) * nn, e->st));
+  for (float *p : f32) HIPCHK(hipMemsetAsync(p, 0, sizeof(float) * nn, e->st));
+  for (int *p : i32) HIPCHK(hipMemsetAsync(p, 0, sizeof(int) * nn, e->st));
   HIPCHK(hipMemsetAsync(d.mintb, 0, nn, e->st));
   HIPCHK(sphk::launch_fill_f32(d.r2b, -1.f, (int)nn, e->st));  // no lists yet
+  for (auto &t : e->tables) t.valid = false;
+  e->saved_valid = false;
+  e->tiles_c = 0;
   // The uploads that follow use blocking copies on the null stream, which
   // does not order against this non-blocking stream.
   HIPCHK(hipStreamSynchronize(e->st));
@@ -363,6 +423,8 @@ bool adjacent(const swsph_engine *e, int ci, int cj, double t[3]) {
 
 int cell_ok(const swsph_engine *e, int c) {
   if (e->n == 0 && e->cell_start.empty()) return fail(1, "no particles set");
+  if (e->need_particles)
+    return fail(1, "the grid changed: call swsph_set_particles first");
   if (c < 0 || c >= e->ncells) return fail(1, "cell index %d out of range", c);
   return 0;
 }
@@ -451,7 +513,7 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
 
 // Neighbour-list storage for the current particle count and slot count.
 int ensure_lists(swsph_engine *e) {
-  const size_t npad = (size_t)std::max<int64_t>(1, e->n);
+  const size_t npad = std::max<size_t>(1, e->cap_n);
   const size_t elems = (size_t)e->list_slots * npad * kListStreams;
   if (e->nl.idx && e->nl_elems == elems) return 0;
   if (e->nl.idx) (void)hipFree(e->nl.idx);
@@ -793,6 +855,47 @@ void drop_graphs(swsph_engine *e) {
   if (e->g_finish) (void)hipGraphExecDestroy(e->g_finish);
   e->g_density = e->g_finish = nullptr;
 }
+
+// Device copies of the pass state (h and the (io) fields).
+int copy_state(swsph_engine *e, std::vector<float *> &buf, bool save) {
+  std::vector<float *> f = io_fields(e->d);
+  const size_t n = (size_t)(e->n ? e->n : 1);
+  if (buf.empty()) {
+    buf.assign(f.size(), nullptr);
+    for (auto &p : buf) HIPCHK(dalloc(&p, std::max(n, e->cap_n)));
+  }
+  for (size_t k = 0; k < f.size(); k++)
+    HIPCHK(hipMemcpyAsync(save ? buf[k] : f[k], save ? f[k] : buf[k],
+                          sizeof(float) * n, hipMemcpyDeviceToDevice, e->st));
+  return 0;
+}
+
+// The eager batched pass (the engine mutex held by the caller).
+int pass_batched_locked(swsph_engine *e, swsph_pass_stats *stats) {
+  int rc = 0;
+  if (e->need_particles)
+    return fail(1, "the grid changed: call swsph_set_particles first");
+  const int64_t l0 = e->launches;
+  e->list_rebuilt = 0;
+  e->list_fallback = 0;
+  e->density_work = 0;
+  if (e->use_lists && (rc = ensure_lists(e))) return rc;
+  if ((rc = phase_start(e))) return rc;
+  int redo = 0, it = 0;
+  for (; it < e->props.max_iter; it++) {
+    if ((rc = phase_ghost(e))) return rc;
+    int relist = 0, nolist = 0;
+    if ((rc = read_redo(e, &redo, &relist, &nolist))) return rc;
+    e->list_fallback += nolist;
+    if (redo == 0) break;
+    if ((rc = phase_rerun(e, redo, relist))) return rc;
+  }
+  if (redo > 0)
+    return fail(3, "Smoothing length failed to converge on %d particles", redo);
+  e->last_rounds = it + 1;
+  if ((rc = phase_finish(e))) return rc;
+  return finish_stats(e, stats, it + 1, l0);
+}
 }  // namespace
 
 extern "C" {
@@ -861,10 +964,22 @@ int swsph_set_props(swsph_engine *e, const swsph_props *props) {
   e->gen++;  // captured graphs no longer match
   int rc = check_props(props);
   if (rc) return rc;
+  bool grid = false;
   for (int k = 0; k < 3; k++)
-    if (props->ncell[k] != e->props.ncell[k] || props->dim[k] != e->props.dim[k])
-      return fail(1, "grid changes need swsph_set_particles on a new engine");
+    grid = grid || props->ncell[k] != e->props.ncell[k] ||
+           props->dim[k] != e->props.dim[k];
   e->props = *props;
+  if (grid) {
+    // the particles must be binned again (swsph_set_particles), which
+    // reuses the device buffers when they are large enough
+    for (int k = 0; k < 3; k++) {
+      e->grid.nc[k] = props->ncell[k];
+      e->grid.dim[k] = props->dim[k];
+    }
+    e->ncells = props->ncell[0] * props->ncell[1] * props->ncell[2];
+    e->need_particles = e->n > 0 || !e->cell_start.empty();
+    for (auto &t : e->tables) t.valid = false;
+  }
   return 0;
 }
 
@@ -889,10 +1004,14 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
                     (long long)i);
     }
   HIPCHK(hipStreamSynchronize(e->st));
-  free_dev(e);
+  int rc = 0;
+  if (!e->d.px || (size_t)n > e->cap_n || e->ncells > e->cap_cells) {
+    free_dev(e);
+    if ((rc = alloc_dev(e, (size_t)std::max<int64_t>(n, 1)))) return rc;
+  }
   e->n = n;
-  int rc = alloc_dev(e, (size_t)n);
-  if (rc) return rc;
+  e->need_particles = false;
+  if ((rc = reset_dev(e, (size_t)n))) return rc;
 
   // Counting sort by top-level cell (the binning rule of the oracle).
   const int *nc = e->grid.nc;
@@ -991,7 +1110,6 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
     HIPCHK(hipMemcpy(e->d.cell, cs.data(), sizeof(int) * n,
                      hipMemcpyHostToDevice));
   }
-  HIPCHK(dalloc(&e->d_cell_start, (size_t)e->ncells + 1));
   HIPCHK(hipMemcpy(e->d_cell_start, e->cell_start.data(),
                    sizeof(int) * (e->ncells + 1), hipMemcpyHostToDevice));
   // Neighbour records of every particle from the state just uploaded
@@ -1162,6 +1280,8 @@ int swsph_do_end_hydro_force(swsph_engine *e, int ci) {
 int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
   if (!e) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->need_particles)
+    return fail(1, "the grid changed: call swsph_set_particles first");
   int rc = 0;
   const int64_t l0 = e->launches;
   std::vector<int> up;
@@ -1212,27 +1332,7 @@ int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
 int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats) {
   if (!e) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
-  int rc = 0;
-  const int64_t l0 = e->launches;
-  e->list_rebuilt = 0;
-  e->list_fallback = 0;
-  e->density_work = 0;
-  if (e->use_lists && (rc = ensure_lists(e))) return rc;
-  if ((rc = phase_start(e))) return rc;
-  int redo = 0, it = 0;
-  for (; it < e->props.max_iter; it++) {
-    if ((rc = phase_ghost(e))) return rc;
-    int relist = 0, nolist = 0;
-    if ((rc = read_redo(e, &redo, &relist, &nolist))) return rc;
-    e->list_fallback += nolist;
-    if (redo == 0) break;
-    if ((rc = phase_rerun(e, redo, relist))) return rc;
-  }
-  if (redo > 0)
-    return fail(3, "Smoothing length failed to converge on %d particles", redo);
-  e->last_rounds = it + 1;
-  if ((rc = phase_finish(e))) return rc;
-  return finish_stats(e, stats, it + 1, l0);
+  return pass_batched_locked(e, stats);
 }
 
 int swsph_pass_graph(swsph_engine *e, swsph_pass_stats *stats) {
@@ -1241,6 +1341,8 @@ int swsph_pass_graph(swsph_engine *e, swsph_pass_stats *stats) {
   int rc = 0;
   if (e->use_lists)
     return fail(1, "graph replay needs the neighbour lists disabled");
+  if (e->need_particles)
+    return fail(1, "the grid changed: call swsph_set_particles first");
   if (e->n == 0) return fail(1, "no particles set");
   const int64_t l0 = e->launches;
   e->list_rebuilt = 0;
@@ -1292,18 +1394,20 @@ int swsph_save_state(swsph_engine *e) {
   const size_t n = (size_t)(e->n ? e->n : 1);
   if (e->saved.empty()) {
     e->saved.assign(src.size(), nullptr);
-    for (auto &p : e->saved) HIPCHK(dalloc(&p, n));
+    for (auto &p : e->saved) HIPCHK(dalloc(&p, std::max(n, e->cap_n)));
   }
   for (size_t k = 0; k < src.size(); k++)
     HIPCHK(hipMemcpyAsync(e->saved[k], src[k], sizeof(float) * n,
                           hipMemcpyDeviceToDevice, e->st));
+  e->saved_valid = true;
   return 0;
 }
 
 int swsph_restore_state(swsph_engine *e) {
   if (!e) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
-  if (e->saved.empty()) return fail(1, "no saved state");
+  if (e->saved.empty() || !e->saved_valid)
+    return fail(1, "no saved state for the current particles");
   std::vector<float *> dst = io_fields(e->d);
   const size_t n = (size_t)(e->n ? e->n : 1);
   for (size_t k = 0; k < dst.size(); k++)
@@ -1397,6 +1501,101 @@ int swsph_get_timing(swsph_engine *e, double ms[3], int64_t launches[3]) {
   for (int k = 0; k < 3; k++) {
     ms[k] = e->t_ms[k];
     launches[k] = e->t_launch[k];
+  }
+  return 0;
+}
+
+int swsph_autotune(swsph_engine *e, int repeats, swsph_tune_result *out) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->n == 0) return fail(1, "no particles set");
+  if (repeats < 1 || repeats > 100) return fail(1, "repeats %d outside [1, 100]", repeats);
+  int rc = 0;
+  if ((rc = copy_state(e, e->tune_saved, true))) return rc;
+  const int sched0 = e->schedule, waves0 = e->queue_waves;
+  const int c0[3] = {e->loop_c[0], e->loop_c[1], e->loop_c[2]};
+  const bool timing0 = e->timing;
+  // With lists the density loop builds them in tiles of kListChunk.
+  const int caps[3] = {8, 16, 32};
+  double best_total = 1e300;
+  int best_s = sched0, best_c[3] = {c0[0], c0[1], c0[2]};
+  double best_ms[3] = {0, 0, 0}, best_pass = 0;
+  double tab[3][3][3];  // [schedule][capacity][loop] ms per pass
+  double wall[3][3];
+  int trials = 0;
+  for (int sc = 0; sc < 3 && !rc; sc++) {
+    for (int ci = 0; ci < 3 && !rc; ci++) {
+      e->schedule = sc;
+      e->queue_waves = 0;
+      for (int k = 0; k < 3; k++) e->loop_c[k] = caps[ci];
+      if (e->use_lists) e->loop_c[0] = kListChunk;
+      e->gen++;
+      // warm-up pass (builds the chunk tables of this capacity)
+      if ((rc = copy_state(e, e->tune_saved, false)) ||
+          (rc = pass_batched_locked(e, nullptr)))
+        break;
+      e->timing = true;
+      for (int k = 0; k < 3; k++) e->t_ms[k] = 0.0, e->t_launch[k] = 0;
+      e->ev_pending.clear();
+      e->ev_used = 0;
+      HIPCHK(hipStreamSynchronize(e->st));
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int r = 0; r < repeats && !rc; r++)
+        if (!(rc = copy_state(e, e->tune_saved, false)))
+          rc = pass_batched_locked(e, nullptr);
+      if (rc) break;
+      const auto t1 = std::chrono::steady_clock::now();
+      wall[sc][ci] =
+          std::chrono::duration<double, std::milli>(t1 - t0).count() / repeats;
+      for (int k = 0; k < 3; k++) tab[sc][ci][k] = e->t_ms[k] / repeats;
+      e->timing = false;
+      trials++;
+    }
+  }
+  if (!rc) {
+    // best capacity per loop under each schedule, then the best schedule
+    for (int sc = 0; sc < 3; sc++) {
+      double tot = 0.0;
+      int cs[3];
+      double ms[3];
+      for (int k = 0; k < 3; k++) {
+        int bi = 0;
+        for (int ci = 1; ci < 3; ci++)
+          if (tab[sc][ci][k] < tab[sc][bi][k]) bi = ci;
+        cs[k] = caps[bi];
+        ms[k] = tab[sc][bi][k];
+        tot += ms[k];
+      }
+      if (e->use_lists) cs[0] = kListChunk;
+      if (tot < best_total) {
+        best_total = tot;
+        best_s = sc;
+        for (int k = 0; k < 3; k++) best_c[k] = cs[k], best_ms[k] = ms[k];
+        best_pass = wall[sc][1];
+      }
+    }
+  }
+  // apply the choice (or restore the settings after a failure) and the state
+  e->schedule = rc ? sched0 : best_s;
+  e->queue_waves = rc ? waves0 : 0;
+  for (int k = 0; k < 3; k++) e->loop_c[k] = rc ? c0[k] : best_c[k];
+  e->gen++;
+  e->timing = timing0;
+  for (int k = 0; k < 3; k++) e->t_ms[k] = 0.0, e->t_launch[k] = 0;
+  e->ev_pending.clear();
+  e->ev_used = 0;
+  const int rc2 = copy_state(e, e->tune_saved, false);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  HIPCHK(hipStreamSynchronize(e->st));
+  if (out) {
+    out->schedule = best_s;
+    for (int k = 0; k < 3; k++) {
+      out->chunk[k] = best_c[k];
+      out->loop_ms[k] = best_ms[k];
+    }
+    out->pass_ms = best_pass;
+    out->trials = trials;
   }
   return 0;
 }

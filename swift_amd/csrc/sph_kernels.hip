@@ -10,6 +10,7 @@
 // The neighbour loops are in sph_gather.h (engine) and sph_loops.h (the
 // three interaction bodies); this file holds the record builders, the
 // per-particle kernels of the ghosts and the launchers.
+#include <atomic>
 #include "sph_device.h"
 #include "sph_kernels.h"
 #include "sph_loops.h"
@@ -710,8 +711,8 @@ hipError_t launch_build_tiles(const DevParts &d, const LoopArgs &la, int n,
 // times the CU count (queried once per kernel instance).
 template <class L, int C, bool B>
 static int resident_blocks() {
-  static int cached = 0;
-  if (cached == 0) {
+  static std::atomic<int> cached{0};
+  if (cached.load() == 0) {
     int dev = 0, ncu = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
@@ -720,9 +721,9 @@ static int resident_blocks() {
             &per, k_loop_queue<L, C, B>, 64 * kWaves, 0) != hipSuccess ||
         per <= 0)
       return 0;
-    cached = ncu * per * kWaves;
+    cached.store(ncu * per * kWaves);
   }
-  return cached;
+  return cached.load();
 }
 
 template <class L, int C, bool B>

@@ -32,6 +32,12 @@ FIELDS = {
 }
 
 
+class _TuneResult(C.Structure):
+    _fields_ = [("schedule", C.c_int), ("chunk", C.c_int * 3),
+                ("loop_ms", C.c_double * 3), ("pass_ms", C.c_double),
+                ("trials", C.c_int)]
+
+
 class _Props(C.Structure):
     _fields_ = [("dim", C.c_double * 3), ("ncell", C.c_int * 3),
                 ("eta", C.c_float), ("h_tolerance", C.c_float),
@@ -96,6 +102,13 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+    def set_props(self, props: HydroProps) -> None:
+        """New run-time parameters; a new grid needs set_particles next
+        (the device buffers are reused when large enough)."""
+        check(lib().swsph_set_props(self._h, C.byref(_props(props))),
+              "swsph_set_props")
+        self.props = props
 
     # -- particles ---------------------------------------------------------
     def set_particles(self, st: dict) -> None:
@@ -194,6 +207,16 @@ class Engine:
         s = PassStats()
         check(lib().swsph_pass_batched(self._h, C.byref(s)), "swsph_pass_batched")
         return s.as_dict()
+
+    def autotune(self, repeats: int = 2) -> dict:
+        """Time the schedules and chunk capacities on the current particles
+        and apply the fastest (the pass state is restored afterwards)."""
+        r = _TuneResult()
+        check(lib().swsph_autotune(self._h, int(repeats), C.byref(r)),
+              "swsph_autotune")
+        return {"schedule": r.schedule, "chunk": list(r.chunk),
+                "loop_ms": list(r.loop_ms), "pass_ms": r.pass_ms,
+                "trials": r.trials}
 
     def pass_graph(self) -> dict:
         """The batched pass replayed from captured HIP graphs."""

@@ -368,3 +368,47 @@ def test_graph_replay_matches_eager(orc, mode):
     for k in eres:
         assert np.array_equal(eres[k], gres[k]), k
         assert np.array_equal(eres[k], g2res[k]), k
+
+
+def test_autotune_keeps_state_and_results(orc):
+    """The autotuner times every schedule and chunk capacity on the current
+    particles, applies a valid choice, and leaves the pass state as it found
+    it: the next pass gives the oracle's results."""
+    from swift_amd import Engine, HydroProps, ics
+    st = ics.modulated_box(16, seed=41, amp=0.5)
+    props = HydroProps(ncell=(4, 4, 4))
+    with Engine(props) as eng:
+        eng.set_particles(st)
+        h0 = eng.get("h")
+        t = eng.autotune(repeats=1)
+        assert t["trials"] == 9
+        assert t["schedule"] in (0, 1, 2)
+        assert all(c in (8, 16, 32) for c in t["chunk"])
+        assert np.array_equal(eng.get("h"), h0)
+        for loop, c in zip(("density", "gradient", "force"), t["chunk"]):
+            assert eng.get_chunk(loop) == c
+        eng.pass_batched()
+        res = eng.results()
+    _check(orc, props, st, res)
+
+
+def test_engine_reuse_matches_fresh():
+    """One engine re-gridded and re-filled with boxes of other sizes (its
+    device buffers reused when they fit, grown when not) gives what a fresh
+    engine gives for each box."""
+    from swift_amd import Engine, HydroProps, ics
+    cases = [(12, 3, 1), (16, 4, 2), (10, 3, 3), (16, 4, 4), (20, 5, 5)]
+    with Engine(HydroProps(ncell=(3, 3, 3))) as eng:
+        for n, nc, seed in cases:
+            props = HydroProps(ncell=(nc,) * 3)
+            st = ics.modulated_box(n, seed=seed, amp=0.3)
+            eng.set_props(props)
+            eng.set_particles(st)
+            eng.pass_batched()
+            a = eng.results()
+            with Engine(props) as fresh:
+                fresh.set_particles(st)
+                fresh.pass_batched()
+                b = fresh.results()
+            for k in a:
+                assert np.array_equal(a[k], b[k]), (n, k)

@@ -49,11 +49,13 @@ def test_structs_match_header(tmp_path):
     import shutil
     import subprocess
     from swift_amd._lib import HEADER_PATH
-    from swift_amd.engine import _Props, _HostParts, PassStats
+    from swift_amd.engine import _Props, _HostParts, _TuneResult, PassStats
+    from swift_amd.server import _Info, _Opts
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
     structs = {"swsph_props": _Props, "swsph_host_parts": _HostParts,
-               "swsph_pass_stats": PassStats}
+               "swsph_pass_stats": PassStats, "swsph_tune_result": _TuneResult,
+               "swsph_server_opts": _Opts, "swsph_request_info": _Info}
     lines = ['#include <stdio.h>', '#include <stddef.h>',
              f'#include "{HEADER_PATH}"', 'int main(void) {']
     for cname, py in structs.items():
