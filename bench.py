@@ -48,6 +48,19 @@ def _pct(a, q):
     return float(np.percentile(np.asarray(a), q))
 
 
+def _read_profile_value(path, kernel_key, name):
+    """A `kernel_key: name = value` line of a committed rocprofv3 summary."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        line = line.strip()
+        if line.startswith(kernel_key + ":"):
+            k, v = line[len(kernel_key) + 1:].split("=")
+            if k.strip() == name:
+                return float(v)
+    return None
+
+
 def _read_traffic(path, kernel_key):
     """HBM bytes per launch from a committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, copy-kernel scaled)."""
@@ -146,6 +159,70 @@ def serving_bench(duration_s: float, rate: float, seed: int = 0,
             "interactions_per_s": inter / wall, "wall_s": wall}
 
 
+def clustered_bench(n_side: int, ncell: int, steps: int, plain: bool,
+                    seed: int = 0) -> dict:
+    """The clustered, uneven workload: n_side^3 particles, 70 % in 1000
+    Plummer spheres (scale radii log-uniform in [0.002, 0.02]) and 30 %
+    uniform, u log-uniform in [0.1, 100], h from a converged all-active
+    pre-pass, then 10 % of the particles active (random time-bin mask)."""
+    from swift_amd import Engine, HydroProps, ics
+    N = n_side ** 3
+    t0 = time.perf_counter()
+    st = ics.clustered_box(N, seed=seed, n_spheres=1000, a_min=0.002,
+                           a_max=0.02)
+    props = HydroProps(ncell=(ncell,) * 3)
+    eng = Engine(props)
+    eng.set_particles(st)
+    pre = eng.pass_batched()
+    res = eng.results(["h", "rho", "pressure", "soundspeed", "f", "balsara",
+                       "alpha_visc", "alpha_diff"])
+    for k, v in res.items():
+        st[k] = v
+    ics.set_active_fraction(st, 0.1, seed=seed + 1)
+    eng.set_particles(st)
+    eng.save_state()
+    print(f"bench: clustered box ready ({time.perf_counter() - t0:.1f} s, "
+          f"pre-pass {pre['ghost_rounds']} rounds)", file=sys.stderr,
+          flush=True)
+    tuning = eng.autotune(repeats=2)
+    times = []
+    for k in range(steps + 1):
+        eng.restore_state()
+        eng.sync()
+        ts = time.perf_counter()
+        s = eng.pass_graph()
+        times.append(time.perf_counter() - ts)
+    times = times[1:]  # the first replay captures the graphs
+    inter = s["n_density"] + s["n_force"]
+    h = res["h"]
+    out = {"workload": f"clustered box {n_side}^3 ({N} particles), 1000 Plummer "
+                       "spheres + 30 % uniform, h from a converged all-active "
+                       f"pass, 10 % active, {ncell}^3 top-level cells",
+           "particles": N, "active": int((st["time_bin"] <= 56).sum()),
+           "h_max_over_min": float(h.max() / h.min()),
+           "value": inter / float(np.mean(times)), "unit": "interactions/s",
+           "ms_per_step": 1e3 * float(np.mean(times)),
+           "p50_step_ms": 1e3 * _pct(times, 50),
+           "p99_step_ms": 1e3 * _pct(times, 99),
+           "interactions_per_step": {"density": s["n_density"],
+                                     "force": s["n_force"]},
+           "ghost_rounds": s["ghost_rounds"], "tuning": tuning}
+    if plain:
+        eng.restore_state()
+        eng.sync()
+        ts = time.perf_counter()
+        ps = eng.pass_plain()
+        eng.sync()
+        pdt = time.perf_counter() - ts
+        pinter = ps["n_density"] + ps["n_force"]
+        out["plain"] = {"value": pinter / pdt, "unit": "interactions/s",
+                        "ms_per_step": 1e3 * pdt,
+                        "launches_per_step": ps["launches"]}
+        out["batched_over_plain"] = out["value"] / (pinter / pdt)
+    eng.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +244,9 @@ def main() -> int:
     ap.add_argument("--serving-s", type=float, default=5.0,
                     help="seconds of the serving stream (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=200.0)
+    ap.add_argument("--clustered-n", type=int, default=256,
+                    help="particles per axis of the clustered workload (0: skip)")
+    ap.add_argument("--clustered-ncell", type=int, default=48)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -256,9 +336,14 @@ def main() -> int:
     achieved = flops / 1e12
     traffic = _read_traffic(os.path.join(ROOT, "profiles", "pmc_traffic.txt"),
                             f"{names[dom]}_{args.n}")
+    valu_busy = _read_profile_value(
+        os.path.join(ROOT, "profiles", "pmc_valu_128cube.md"),
+        f"{names[dom]}_{args.n}", "valu_busy")
     roofline = {"bound": "valu", "kernel": names[dom], "achieved": achieved,
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                "hbm_gbs": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
+                "valu_busy": valu_busy,
                 "avg_launch_ms": avg_ms,
                 "flop_per_interaction": [FLOP_DENSITY, FLOP_GRADIENT,
                                          FLOP_FORCE][dom]}
@@ -317,6 +402,12 @@ def main() -> int:
         print(f"bench: plain pass {np.mean(ptimes):.2f} s", file=sys.stderr,
               flush=True)
     eng.close()
+
+    if args.clustered_n > 0:
+        out["clustered"] = clustered_bench(args.clustered_n, args.clustered_ncell,
+                                           args.steps, not args.no_plain)
+        print(f"bench: clustered {out['clustered']['ms_per_step']:.1f} ms/step",
+              file=sys.stderr, flush=True)
 
     if args.serving_s > 0:
         out["serving"] = serving_bench(args.serving_s, args.serving_rate)
