@@ -82,7 +82,14 @@ enum swsph_field {
   SWSPH_N_GRADIENT,         /* int32 */
   SWSPH_N_FORCE,            /* int32: neighbours j != i, r < max(H_i, H_j) */
   SWSPH_MIN_NGB_TIME_BIN,   /* int8 */
-  SWSPH_NUM_FIELDS
+  SWSPH_NUM_FIELDS,         /* count of the pass outputs above */
+  /* particle state advanced by the time integration below */
+  SWSPH_X = 32,             /* double [3n]: positions (part.x) */
+  SWSPH_V,                  /* [3n] predicted velocities (part.v) */
+  SWSPH_V_FULL,             /* [3n] xpart.v_full */
+  SWSPH_U,                  /* predicted internal energy (part.u) */
+  SWSPH_U_FULL,             /* xpart.u_full */
+  SWSPH_TIME_BIN            /* int8 */
 };
 
 typedef struct swsph_pass_stats {
@@ -214,6 +221,62 @@ int swsph_autotune(swsph_engine *e, int repeats, swsph_tune_result *out);
  * candidates; neighbour sets and results do not depend on the setting.
  * Defaults: disabled, skin 1.2, 48 slots. */
 int swsph_set_lists(swsph_engine *e, int enable, float skin, int slots);
+
+/* ---- time integration ---------------------------------------------------
+ * Gas-only, non-cosmological time integration on SWIFT's integer time line
+ * (src/timeline.h: a particle in bin b takes steps of 2^(b+1) time-line
+ * units of props.time_base; it is active when the current time is a
+ * multiple of its step).  Particles keep their state in HBM from step to
+ * step; the positions are re-binned on the device whenever a drifted
+ * particle could have left the 27-cell stencil of its cell (kernel support
+ * H plus twice the largest distance of a particle outside its cell above
+ * the cell width, the reference's rebuild criterion). */
+typedef struct swsph_integration {
+  float cfl;               /* SPH:CFL_condition */
+  float log_max_h_change;  /* log of the largest h change per step
+                              (hydro_props.log_max_h_change) */
+  float min_u;             /* hydro_props.minimal_internal_energy */
+  double dt_min, dt_max;   /* TimeIntegration:dt_min / dt_max */
+  int fixed_bins;          /* 1: keep the time bins the particles were given
+                              (benchmark hierarchies), 0: SWIFT's time-step
+                              rule */
+} swsph_integration;
+int swsph_set_integration(swsph_engine *e, const swsph_integration *p);
+
+/* runner_do_drift_part (src/runner_drift.c; drift_part, src/drift.h:138)
+ * for cell ci (-1: all cells) from ti_old to ti_current. */
+int swsph_do_drift_part(swsph_engine *e, int ci, int64_t ti_old,
+                        int64_t ti_current);
+/* runner_do_kick1 / runner_do_kick2 (src/runner_time_integration.c:87,
+ * 360) for cell ci (-1: all cells) at ti_current. */
+int swsph_do_kick1(swsph_engine *e, int ci, int64_t ti_current);
+int swsph_do_kick2(swsph_engine *e, int ci, int64_t ti_current);
+/* runner_do_timestep (src/runner_time_integration.c:639) for cell ci (-1:
+ * all cells): new bins of the active particles; *ti_end_min receives the
+ * next synchronisation point over the cell. */
+int swsph_do_timestep(swsph_engine *e, int ci, int64_t ti_current,
+                      int64_t *ti_end_min);
+/* Re-bins the particles on the device (the particle sort of
+ * space_rebuild, src/space.c, for this path). */
+int swsph_rebuild(swsph_engine *e);
+
+typedef struct swsph_step_info {
+  int64_t ti_current;      /* time of the step just taken */
+  int64_t ti_end_min;      /* next synchronisation point */
+  int max_active_bin;
+  int rebuilt;             /* particles were re-binned this step */
+  int64_t n_active;
+  float dx_max;            /* largest distance outside a cell after the drift */
+  int64_t launches;
+  swsph_pass_stats pass;
+} swsph_step_info;
+/* One step of the gas (engine_step, src/engine.c, with the hydro tasks
+ * only): the first call computes forces at t = 0 for all particles, their
+ * time bins and the first half-kick; every later call advances to the next
+ * synchronisation point, drifts all particles, re-bins if needed, runs the
+ * batched density -> ghost -> gradient -> force pass over the active
+ * particles, then kick2 -> time step -> kick1. */
+int swsph_engine_step(swsph_engine *e, swsph_step_info *info);
 
 /* ---- serving -------------------------------------------------------------
  * A request server for independent periodic boxes (no counterpart in the

@@ -199,3 +199,73 @@ def iact(kind: str, r2: float, dx, hi: float, hj: float, pi: np.ndarray,
     fn = getattr(lib(), f"orc_iact_{kind}32")
     fn(r2, dxa.ctypes.data, hi, hj, pi.ctypes.data, pj.ctypes.data,
        int(symmetric))
+
+
+# ---- time integration ------------------------------------------------------
+class OrcGas(C.Structure):
+    _fields_ = [("n", C.c_int)] + [(k, C.c_void_p) for k in (
+        "x", "v", "v_full", "a_hydro", "u", "u_full", "u_dt", "h", "h_dt",
+        "rho", "pressure", "soundspeed", "v_sig", "time_bin",
+        "min_ngb_time_bin")]
+
+
+class OrcInteg(C.Structure):
+    _fields_ = [("time_base", C.c_double), ("cfl", C.c_float),
+                ("log_max_h_change", C.c_float), ("min_u", C.c_float),
+                ("dt_min", C.c_double), ("dt_max", C.c_double),
+                ("fixed_bins", C.c_int)]
+
+
+GAS_TYPES = {"x": np.float64, "v": np.float32, "v_full": np.float32,
+             "a_hydro": np.float32, "u": np.float32, "u_full": np.float32,
+             "u_dt": np.float32, "h": np.float32, "h_dt": np.float32,
+             "rho": np.float32, "pressure": np.float32,
+             "soundspeed": np.float32, "v_sig": np.float32,
+             "time_bin": np.int8, "min_ngb_time_bin": np.int8}
+
+
+def _gas(g: dict) -> OrcGas:
+    s = OrcGas()
+    s.n = g["h"].shape[0]
+    for k, dt in GAS_TYPES.items():
+        a = g[k]
+        assert a.dtype == dt and a.flags["C_CONTIGUOUS"], k
+        setattr(s, k, a.ctypes.data)
+    return s
+
+
+def integ(time_base: float, p) -> OrcInteg:
+    """OrcInteg from a time base and an object with cfl, log_max_h_change,
+    min_u, dt_min, dt_max, fixed_bins attributes."""
+    return OrcInteg(time_base, p.cfl, p.log_max_h_change, p.min_u, p.dt_min,
+                    p.dt_max, int(p.fixed_bins))
+
+
+def max_active_bin(ti: int) -> int:
+    return lib().orc_max_active_bin(C.c_longlong(ti))
+
+
+def make_integer_timestep(new_dt: float, old_bin: int, min_ngb_bin: int,
+                          ti_current: int, time_base_inv: float) -> int:
+    f = lib().orc_make_integer_timestep
+    f.restype = C.c_longlong
+    f.argtypes = [C.c_float, C.c_int, C.c_int, C.c_longlong, C.c_double]
+    return f(new_dt, old_bin, min_ngb_bin, ti_current, time_base_inv)
+
+
+def drift(g: dict, c: OrcInteg, ti_old: int, ti_current: int) -> None:
+    lib().orc_drift(C.byref(_gas(g)), C.byref(c), C.c_longlong(ti_old),
+                    C.c_longlong(ti_current))
+
+
+def kick(g: dict, c: OrcInteg, ti_current: int, mode: int) -> None:
+    lib().orc_kick(C.byref(_gas(g)), C.byref(c), C.c_longlong(ti_current),
+                   C.c_int(mode))
+
+
+def timestep(g: dict, c: OrcInteg, ti_current: int):
+    """Returns (ti_end_min, n_active, n_bad)."""
+    te, na = C.c_longlong(), C.c_longlong()
+    bad = lib().orc_timestep(C.byref(_gas(g)), C.byref(c),
+                             C.c_longlong(ti_current), C.byref(te), C.byref(na))
+    return te.value, na.value, bad

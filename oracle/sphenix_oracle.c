@@ -1134,3 +1134,192 @@ int orc_count_bruteforce(const orc_config *cfg, const orc_parts *p, int mode,
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Time integration.                                                         */
+/* ------------------------------------------------------------------------ */
+#define ORC_NUM_TIME_BINS 56
+#define ORC_MAX_NR_TIMESTEPS (1LL << (ORC_NUM_TIME_BINS + 1))
+#define ORC_BIN_INHIBITED (ORC_NUM_TIME_BINS + 2)
+
+/* timeline.h: get_integer_timestep, get_time_bin, get_integer_time_begin,
+ * get_integer_time_end, get_max_active_bin. */
+static long long tl_step(int bin) { return bin <= 0 ? 0 : (1LL << (bin + 1)); }
+static int tl_bin(long long dti) {
+  return (8 * (int)sizeof(long long) - 2) - __builtin_clzll((unsigned long long)dti);
+}
+static long long tl_begin(long long ti, int bin) {
+  const long long dti = tl_step(bin);
+  return dti == 0 ? 0 : dti * ((ti - 1) / dti);
+}
+static long long tl_end(long long ti, int bin) {
+  const long long dti = tl_step(bin);
+  if (dti == 0) return 0;
+  const long long mod = ti % dti;
+  return mod == 0 ? ti : ti - mod + dti;
+}
+int orc_max_active_bin(long long ti) {
+  if (ti == 0) return ORC_NUM_TIME_BINS;
+  int bin = 1;
+  while (!((1LL << (bin + 1)) & ti)) ++bin;
+  return bin;
+}
+
+/* approx_math.h:35 */
+static float orc_approx_expf(float x) {
+  return 1.f + x * (1.f + x * (0.5f + x * (1.f / 6.f + 1.f / 24.f * x)));
+}
+
+/* timestep.h:49-89 (0 for a step that rounds to zero time-line units) */
+long long orc_make_integer_timestep(float new_dt, int old_bin, int min_ngb_bin,
+                                    long long ti_current, double time_base_inv) {
+  long long new_dti = (long long)(new_dt * time_base_inv);
+  if (new_dti <= 0) return 0;
+  int new_bin = tl_bin(new_dti);
+  if (min_ngb_bin + 2 < new_bin) new_bin = min_ngb_bin + 2;
+  new_dti = tl_step(new_bin);
+  const long long current_dti = tl_step(old_bin);
+  const long long ti_end = tl_end(ti_current, old_bin);
+  if (old_bin > 0 && 2 * current_dti < new_dti) new_dti = 2 * current_dti;
+  long long dti_timeline = ORC_MAX_NR_TIMESTEPS;
+  while (new_dti < dti_timeline) dti_timeline /= 2;
+  new_dti = dti_timeline;
+  if (new_dti > current_dti) {
+    if ((ORC_MAX_NR_TIMESTEPS - ti_end) % new_dti > 0) new_dti = current_dti;
+  }
+  return new_dti;
+}
+
+/* runner_do_drift_part -> drift_part (drift.h:138-200) with
+ * hydro_predict_extra (hydro.h:1029-1078); no gravity, no cosmology:
+ * dt_drift = dt_kick_hydro = dt_therm = (ti_current - ti_old) time_base. */
+void orc_drift(orc_gas *g, const orc_integ *c, long long ti_old,
+               long long ti_current) {
+  const double dt = (double)(ti_current - ti_old) * c->time_base;
+  for (int i = 0; i < g->n; i++) {
+    if (g->time_bin[i] == ORC_BIN_INHIBITED) continue;
+    for (int k = 0; k < 3; k++) {
+      g->x[3 * i + k] += g->v_full[3 * i + k] * dt;
+      g->v[3 * i + k] += g->a_hydro[3 * i + k] * dt;
+    }
+    const float dt_drift = (float)dt, dt_therm = (float)dt;
+    g->u[i] += g->u_dt[i] * dt_therm;
+    const float h_inv = 1.f / g->h[i];
+    const float w1 = g->h_dt[i] * h_inv * dt_drift;
+    if (fabsf(w1) < 0.2f)
+      g->h[i] *= orc_approx_expf(w1);
+    else
+      g->h[i] *= expf(w1);
+    const float w2 = -HDIM * w1;
+    if (fabsf(w2) < 0.2f)
+      g->rho[i] *= orc_approx_expf(w2);
+    else
+      g->rho[i] *= expf(w2);
+    /* no entropy floor (floor u = 0); absolute minimum */
+    g->u[i] = fmaxf(g->u[i], 0.f);
+    g->u[i] = fmaxf(g->u[i], c->min_u);
+    const float P = GAS_GM1 * g->u[i] * g->rho[i];
+    const float cs = sqrtf(GAS_GAMMA * P / g->rho[i]);
+    g->pressure[i] = P;
+    g->soundspeed[i] = cs;
+    g->v_sig[i] = fmaxf(g->v_sig[i], 2.f * cs);
+  }
+}
+
+/* runner_do_kick1 (runner_time_integration.c:143-183) / runner_do_kick2
+ * (:408-441) -> kick_part (kick.h) -> hydro_kick_extra (hydro.h:1120-1151);
+ * kick2 ends with hydro_reset_predicted_values (hydro.h:983-1006). */
+void orc_kick(orc_gas *g, const orc_integ *c, long long ti_current, int mode) {
+  const int max_active_bin = orc_max_active_bin(ti_current);
+  for (int i = 0; i < g->n; i++) {
+    const int tb = g->time_bin[i];
+    if (tb == ORC_BIN_INHIBITED) continue;
+    const long long dti = tl_step(tb);
+    long long ti_begin, ti_end;
+    if (mode == 1) {
+      ti_begin = tl_begin(ti_current + 1, tb);
+      if (ti_begin != ti_current) continue;
+      ti_end = ti_begin + dti / 2;
+    } else {
+      if (tb > max_active_bin) continue;
+      ti_begin = tl_begin(ti_current, tb) + dti / 2;
+      ti_end = ti_begin + dti / 2;
+    }
+    const double dt = (double)(ti_end - ti_begin) * c->time_base;
+    for (int k = 0; k < 3; k++) g->v_full[3 * i + k] += g->a_hydro[3 * i + k] * dt;
+    const float dt_therm = (float)dt;
+    const float delta_u = g->u_dt[i] * dt_therm;
+    g->u_full[i] = fmaxf(g->u_full[i] + delta_u, 0.5f * g->u_full[i]);
+    const float energy_min = fmaxf(c->min_u, 0.f);
+    if (g->u_full[i] < energy_min) {
+      g->u_full[i] = energy_min;
+      g->u_dt[i] = 0.f;
+    }
+    if (mode == 2) {
+      for (int k = 0; k < 3; k++) g->v[3 * i + k] = g->v_full[3 * i + k];
+      g->u[i] = g->u_full[i];
+      const float P = GAS_GM1 * g->u[i] * g->rho[i];
+      const float cs = sqrtf(GAS_GAMMA * P / g->rho[i]);
+      g->pressure[i] = P;
+      g->soundspeed[i] = cs;
+      g->v_sig[i] = fmaxf(g->v_sig[i], 2.f * cs);
+    }
+  }
+}
+
+/* runner_do_timestep (runner_time_integration.c:639-760) with
+ * get_part_timestep (timestep.h) and hydro_compute_timestep
+ * (hydro.h:475-487): a = 1, a_factor_sound_speed = 1, no other criteria. */
+int orc_timestep(orc_gas *g, const orc_integ *c, long long ti_current,
+                 long long *ti_end_min, long long *n_active) {
+  const int max_active_bin = orc_max_active_bin(ti_current);
+  long long tmin = ORC_MAX_NR_TIMESTEPS, nact = 0;
+  int bad = 0;
+  for (int i = 0; i < g->n; i++) {
+    const int tb = g->time_bin[i];
+    if (tb == ORC_BIN_INHIBITED) continue;
+    long long te;
+    if (tb <= max_active_bin) {
+      nact++;
+      long long new_dti;
+      if (c->fixed_bins) {
+        new_dti = tl_step(tb);
+        if (new_dti == 0) {
+          bad++;
+          new_dti = 1;
+        }
+      } else {
+        const double a = 1.0, a_fss = 1.0;
+        const float dt_cfl =
+            2.f * KG * c->cfl * a * g->h[i] / (a_fss * g->v_sig[i]);
+        float new_dt = dt_cfl;
+        const float dt_h_change =
+            g->h_dt[i] != 0.0f
+                ? fabsf(c->log_max_h_change * g->h[i] / g->h_dt[i])
+                : FLT_MAX;
+        if (dt_h_change < new_dt) new_dt = dt_h_change;
+        new_dt = (float)((double)new_dt < c->dt_max ? (double)new_dt : c->dt_max);
+        if ((double)new_dt < c->dt_min) {
+          bad++;
+          new_dti = tl_step(tb);
+        } else {
+          new_dti = orc_make_integer_timestep(new_dt, tb, g->min_ngb_time_bin[i],
+                                              ti_current, 1.0 / c->time_base);
+          if (new_dti == 0) {
+            bad++;
+            new_dti = tl_step(tb);
+          } else {
+            g->time_bin[i] = (int8_t)tl_bin(new_dti);
+          }
+        }
+      }
+      te = ti_current + new_dti;
+    } else {
+      te = tl_end(ti_current, tb);
+    }
+    if (te < tmin) tmin = te;
+  }
+  if (ti_end_min) *ti_end_min = tmin;
+  if (n_active) *n_active = nact;
+  return bad;
+}

@@ -133,6 +133,43 @@ int orc_pass_fp32(const orc_config *cfg, orc_parts *p, orc_out32 *out);
 int orc_count_bruteforce(const orc_config *cfg, const orc_parts *p, int mode,
                          int32_t *counts);
 
+/* ---- time integration (gas only, no cosmology) -------------------------
+ * Restates drift_part + hydro_predict_extra (src/drift.h:138,
+ * src/hydro/SPHENIX/hydro.h:1029), kick_part + hydro_kick_extra
+ * (src/kick.h, hydro.h:1120) with the half-kick intervals of
+ * runner_do_kick1 / runner_do_kick2 (src/runner_time_integration.c:87, 360)
+ * and hydro_reset_predicted_values (hydro.h:983) after kick2, and
+ * runner_do_timestep's get_part_timestep / make_integer_timestep
+ * (src/timestep.h) on the time line of src/timeline.h. */
+typedef struct orc_gas {
+  int n;
+  double *x;                       /* [3n] */
+  float *v, *v_full, *a_hydro;     /* [3n] */
+  float *u, *u_full, *u_dt, *h, *h_dt, *rho, *pressure, *soundspeed, *v_sig;
+  int8_t *time_bin;
+  const int8_t *min_ngb_time_bin;
+} orc_gas;
+
+typedef struct orc_integ {
+  double time_base;
+  float cfl, log_max_h_change, min_u;
+  double dt_min, dt_max;
+  int fixed_bins;
+} orc_integ;
+
+int orc_max_active_bin(long long ti);
+long long orc_make_integer_timestep(float new_dt, int old_bin, int min_ngb_bin,
+                                    long long ti_current, double time_base_inv);
+void orc_drift(orc_gas *g, const orc_integ *c, long long ti_old,
+               long long ti_current);
+/* mode 1: kick1 of the particles starting a step; 2: kick2 of the active
+ * particles, then hydro_reset_predicted_values. */
+void orc_kick(orc_gas *g, const orc_integ *c, long long ti_current, int mode);
+/* New bins of the active particles; returns the number of particles whose
+ * step falls below dt_min or to zero (their bin is kept). */
+int orc_timestep(orc_gas *g, const orc_integ *c, long long ti_current,
+                 long long *ti_end_min, long long *n_active);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,7 @@ def lib() -> C.CDLL:
                 "(there is no CPU fallback for the SPH pass)")
         _lib = C.CDLL(LIB_PATH)
         _lib.swsph_last_error.restype = C.c_char_p
+        _lib.swsph_server_last_error.restype = C.c_char_p
     return _lib
 
 

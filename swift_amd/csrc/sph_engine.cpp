@@ -18,6 +18,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/swsph.h"
@@ -44,8 +45,6 @@ int fail(int code, const char *fmt, ...) {
       return fail(2, "%s failed: %s (%s:%d)", #call,                        \
                   hipGetErrorString(err_), __FILE__, __LINE__);            \
   } while (0)
-
-constexpr float kGammaK = (float)(1.936492);
 
 template <class T>
 hipError_t dalloc(T **p, size_t n) {
@@ -130,6 +129,28 @@ struct swsph_engine {
   bool capturing = false;
   hipGraphExec_t g_density = nullptr, g_finish = nullptr;
   std::vector<float *> tune_saved;  // pass state kept by the autotuner
+  // Device binning (sph_step.hip): key / value buffers of the radix sort,
+  // its temporary storage, and a staging buffer of 3 doubles per particle
+  // for uploads in the caller's order and for permutations.
+  unsigned long long *d_key[2] = {nullptr, nullptr};
+  int *d_val[2] = {nullptr, nullptr};
+  void *d_sort_tmp = nullptr;
+  size_t sort_tmp_bytes = 0;
+  double *d_stage = nullptr;
+  bool perm_stale = false;  // host `perm` older than d.perm
+  // Time integration: parameters, the integer time line, and the largest
+  // distance by which a particle lies outside its cell since the last
+  // binning (float bits on the device; host copy).
+  swsph_integration integ{0.1f, 0.f, 0.f, 0.0, 1e30, 0};
+  int64_t ti_current = 0, ti_end_min = 0;
+  bool started = false;
+  unsigned int *d_dmax = nullptr;
+  unsigned long long *d_tiend = nullptr;  // [1] ti_end_min
+  unsigned int *d_err = nullptr;          // [2] time-step errors, active
+  unsigned int *h_small = nullptr;        // pinned readback: dmax, hg2
+  unsigned long long *h_tiend = nullptr;  // pinned [1]
+  unsigned int *h_err = nullptr;          // pinned [2]
+  float dmax = 0.f;
 };
 
 namespace {
@@ -146,13 +167,16 @@ void free_saved(swsph_engine *e) {
 
 namespace {
 
-// Component error bound of the FP32 pre-test: 2^-19 of the widest cell,
-// four times the worst case of the few float roundings of values bounded by
-// two cell widths that separate the pre-test distance from the contract's.
+// Component error bound of the FP32 pre-test: 2^-19 of 1.5 widths of the
+// widest cell, four times the worst case of the few float roundings that
+// separate the pre-test distance from the contract's, for values bounded by
+// two cell widths plus twice the distance d by which a drifted particle may
+// lie outside its cell (valid for d <= w / 2, which the stencil condition
+// H + 2 d <= w guarantees).
 float pos_eps(const swsph_engine *e) {
   double w = 0.0;
   for (int k = 0; k < 3; k++) w = std::max(w, e->grid.dim[k] / e->grid.nc[k]);
-  return (float)(w * 0x1p-19);
+  return (float)(1.5 * w * 0x1p-19);
 }
 
 LoopArgs loop_args(const swsph_engine *e) {
@@ -196,7 +220,9 @@ void free_dev(swsph_engine *e) {
                   e->nl.cnt, e->ts.flag, e->ts.scan, e->ts.blk, e->ts.cnt,
                   e->ts.cscan, e->ts.cblk, e->ts.alist, e->ts.tile_start,
                   e->ts.tile_cell, e->ts.ntiles, e->ts.heads,
-                  e->d_cell_start, e->d_tiles};
+                  e->d_cell_start, e->d_tiles, e->d.vfx, e->d.vfy, e->d.vfz,
+                  e->d.ufull, e->d.perm, e->d_key[0], e->d_key[1],
+                  e->d_val[0], e->d_val[1], e->d_sort_tmp, e->d_stage};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   e->d = DevParts{};
@@ -206,6 +232,11 @@ void free_dev(swsph_engine *e) {
   e->nl_elems = 0;
   e->d_cell_start = nullptr;
   e->d_tiles = nullptr;
+  e->d_key[0] = e->d_key[1] = nullptr;
+  e->d_val[0] = e->d_val[1] = nullptr;
+  e->d_sort_tmp = nullptr;
+  e->sort_tmp_bytes = 0;
+  e->d_stage = nullptr;
   e->cap_n = 0;
   e->cap_cells = 0;
   free_saved(e);
@@ -336,10 +367,12 @@ int alloc_dev(swsph_engine *e, size_t n) {
                    &d.wc, &d.wc_dh, &d.divv, &d.rotx, &d.roty, &d.rotz,
                    &d.left, &d.right, &d.P, &d.c, &d.f, &d.bal, &d.vsig,
                    &d.lapu, &d.avmax, &d.alpha, &d.adiff, &d.divv_prev,
-                   &d.divv_dt, &d.ax, &d.ay, &d.az, &d.udt, &d.hdt};
+                   &d.divv_dt, &d.ax, &d.ay, &d.az, &d.udt, &d.hdt,
+                   &d.vfx, &d.vfy, &d.vfz, &d.ufull};
   for (float **p : f32) HIPCHK(dalloc(p, n));
   HIPCHK(dalloc(&d.r2b, n));
-  int **i32[] = {&d.nd, &d.redo, &d.ng, &d.nf, &d.lfb, &d.sel, &d.cell};
+  int **i32[] = {&d.nd, &d.redo, &d.ng, &d.nf, &d.lfb, &d.sel, &d.cell,
+                 &d.perm};
   for (int **p : i32) HIPCHK(dalloc(p, n));
   HIPCHK(dalloc(&d.tb, n));
   HIPCHK(dalloc(&d.mintb, n));
@@ -362,6 +395,17 @@ int alloc_dev(swsph_engine *e, size_t n) {
     HIPCHK(dalloc(&t.ntiles, 1));
     HIPCHK(dalloc(&t.heads, 8 * 64));
   }
+  // binning buffers
+  HIPCHK(dalloc(&e->d_key[0], n));
+  HIPCHK(dalloc(&e->d_key[1], n));
+  HIPCHK(dalloc(&e->d_val[0], n));
+  HIPCHK(dalloc(&e->d_val[1], n));
+  HIPCHK(dalloc(&e->d_stage, 3 * n));
+  size_t bytes = 0;
+  HIPCHK(sphk::sort_pairs(nullptr, bytes, e->d_key[0], e->d_key[1],
+                          e->d_val[0], e->d_val[1], (int)n, 64, e->st));
+  HIPCHK(hipMalloc(&e->d_sort_tmp, std::max<size_t>(bytes, 16)));
+  e->sort_tmp_bytes = std::max<size_t>(bytes, 16);
   e->cap_n = n;
   e->cap_cells = e->ncells;
   return 0;
@@ -375,19 +419,19 @@ int reset_dev(swsph_engine *e, size_t n) {
                   d.wc, d.wc_dh, d.divv, d.rotx, d.roty, d.rotz,
                   d.left, d.right, d.P, d.c, d.f, d.bal, d.vsig,
                   d.lapu, d.avmax, d.alpha, d.adiff, d.divv_prev,
-                  d.divv_dt, d.ax, d.ay, d.az, d.udt, d.hdt};
-  int *i32[] = {d.nd, d.redo, d.ng, d.nf, d.lfb, d.sel, d.cell};
+                  d.divv_dt, d.ax, d.ay, d.az, d.udt, d.hdt,
+                  d.vfx, d.vfy, d.vfz, d.ufull};
+  int *i32[] = {d.nd, d.redo, d.ng, d.nf, d.lfb, d.sel, d.cell, d.perm};
   const size_t nn = n ? n : 1;
   for (float *p : f32) HIPCHK(hipMemsetAsync(p, 0, sizeof(float) * nn, e->st));
   for (int *p : i32) HIPCHK(hipMemsetAsync(p, 0, sizeof(int) * nn, e->st));
   HIPCHK(hipMemsetAsync(d.mintb, 0, nn, e->st));
   HIPCHK(sphk::launch_fill_f32(d.r2b, -1.f, (int)nn, e->st));  // no lists yet
+  HIPCHK(hipMemsetAsync(e->d_dmax, 0, sizeof(unsigned int), e->st));
+  e->dmax = 0.f;
   for (auto &t : e->tables) t.valid = false;
   e->saved_valid = false;
   e->tiles_c = 0;
-  // The uploads that follow use blocking copies on the null stream, which
-  // does not order against this non-blocking stream.
-  HIPCHK(hipStreamSynchronize(e->st));
   return 0;
 }
 
@@ -694,43 +738,250 @@ int count_into(swsph_engine *e, swsph_pass_stats *s) {
   return 0;
 }
 
-// After a pass: the largest kernel support met by the chunk bounds must fit
-// in a top-level cell, else the 27-cell stencil may have missed neighbours
-// (the reference would have to rebuild its cell tree).
-int check_support(swsph_engine *e) {
-  HIPCHK(hipMemcpy(e->h_hg2_all, e->d_hg2_all, sizeof(unsigned int),
-                   hipMemcpyDeviceToHost));
-  float hg2;
-  std::memcpy(&hg2, e->h_hg2_all, sizeof(float));
-  for (int k = 0; k < 3; k++) {
-    const double w = e->grid.dim[k] / e->grid.nc[k];
-    if ((double)hg2 > w * w)
-      return fail(4,
-                  "kernel support %g exceeds the top-level cell width %g after "
-                  "the ghost: regrid with fewer cells",
-                  std::sqrt((double)hg2), w);
-  }
+// Kernel support H (largest over the chunk bounds) and drift distance d
+// (largest distance of a particle outside its cell since the last binning)
+// of the current particles.  Every neighbour lies in the 27-cell stencil
+// while H + 2 d <= the cell width (the reference rebuilds its cells when
+// that fails).
+int read_fit(swsph_engine *e, double *H, double *dm) {
+  HIPCHK(hipMemcpyAsync(e->h_small, e->d_hg2_all, sizeof(unsigned int),
+                        hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipMemcpyAsync(e->h_small + 1, e->d_dmax, sizeof(unsigned int),
+                        hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  float hg2, d;
+  std::memcpy(&hg2, e->h_small, sizeof(float));
+  std::memcpy(&d, e->h_small + 1, sizeof(float));
+  e->dmax = d;
+  *H = std::sqrt((double)hg2);
+  *dm = (double)d;
   return 0;
 }
 
-int check_h_fits(swsph_engine *e) {
-  // Kernel support must fit in a top-level cell (27-cell stencil).
-  std::vector<float> h(e->n);
-  if (e->n) {
-    HIPCHK(hipMemcpy(h.data(), e->d.h, sizeof(float) * e->n,
+bool fits(const swsph_engine *e, double H, double dm) {
+  for (int k = 0; k < 3; k++)
+    if (H + 2.0 * dm > e->grid.dim[k] / e->grid.nc[k]) return false;
+  return true;
+}
+
+int check_fits(swsph_engine *e) {
+  double H, dm;
+  int rc = read_fit(e, &H, &dm);
+  if (rc) return rc;
+  if (!fits(e, H, dm))
+    return fail(4,
+                "kernel support %g plus twice the drift %g exceeds the "
+                "top-level cell width %g: regrid with fewer cells",
+                H, dm, e->grid.dim[0] / e->grid.nc[0]);
+  return 0;
+}
+
+// After a pass: the largest kernel support met by the chunk bounds (and the
+// drift since the last binning) must fit the stencil.
+int check_support(swsph_engine *e) { return check_fits(e); }
+
+// Keys, stable radix sort and cell bounds of the current n positions
+// (component k of particle i at p_k[stride * i]); the sort's values are the
+// positions' indices, written to vals_out in the new order.
+int bin_sort(swsph_engine *e, const double *px, const double *py,
+             const double *pz, int stride, int *vals_out) {
+  const int ni = (int)e->n;
+  double inv_w[3];
+  for (int k = 0; k < 3; k++) inv_w[k] = (double)e->grid.nc[k] / e->grid.dim[k];
+  HIPCHK(sphk::launch_bin_keys(px, py, pz, stride, e->grid, inv_w, ni,
+                               e->d_key[0], e->d_val[0], e->st));
+  int bits = 30;
+  while ((1LL << (bits - 30)) < (long long)e->ncells) bits++;
+  size_t bytes = e->sort_tmp_bytes;
+  HIPCHK(sphk::sort_pairs(e->d_sort_tmp, bytes, e->d_key[0], e->d_key[1],
+                          e->d_val[0], vals_out, ni, bits, e->st));
+  HIPCHK(sphk::launch_cell_bounds(e->d_key[1], ni, e->ncells,
+                                  e->d_cell_start, e->d.cell, e->st));
+  e->launches += 4;
+  return 0;
+}
+
+// After a binning: host copy of the cell bounds, neighbour records of every
+// particle (inactive particles keep theirs for the whole pass), candidate
+// positions and the chunk tables of the loops.
+int finish_binning(swsph_engine *e) {
+  int rc = 0;
+  e->cell_start.resize(e->ncells + 1);
+  HIPCHK(hipMemcpyAsync(e->cell_start.data(), e->d_cell_start,
+                        sizeof(int) * (e->ncells + 1), hipMemcpyDeviceToHost,
+                        e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  for (LoopKind k : {LoopKind::Density, LoopKind::Gradient, LoopKind::Force})
+    HIPCHK(sphk::launch_build_records(k, e->d, 0, (int)e->n, e->st));
+  HIPCHK(sphk::launch_build_pos(e->d, e->grid, pos_eps(e), 0, (int)e->n,
+                                e->st));
+  for (auto &t : e->tables) t.valid = false;
+  for (int k = 0; k < 3; k++)
+    if (!chunk_table(e, e->loop_c[k], &rc)) return rc;
+  HIPCHK(hipStreamSynchronize(e->st));
+  return 0;
+}
+
+// Host copy of the permutation sorted slot -> caller index.
+int ensure_perm(swsph_engine *e) {
+  if (!e->perm_stale) return 0;
+  std::vector<int> t((size_t)e->n);
+  if (e->n)
+    HIPCHK(hipMemcpy(t.data(), e->d.perm, sizeof(int) * e->n,
                      hipMemcpyDeviceToHost));
+  e->perm.assign(t.begin(), t.end());
+  e->perm_stale = false;
+  return 0;
+}
+
+// Re-binning of drifted particles (space_rebuild's particle sort for this
+// path): positions wrapped into the box, sorted by (cell, Morton code) on
+// the device, every per-particle field carried along.  Neighbour lists,
+// saved states and captured graphs refer to the old order and are dropped.
+int rebin(swsph_engine *e) {
+  const int ni = (int)e->n;
+  DevParts &d = e->d;
+  int rc = 0;
+  HIPCHK(sphk::launch_wrap(d, e->grid, ni, e->st));
+  int *idx = e->d_val[1];
+  if ((rc = bin_sort(e, d.px, d.py, d.pz, 1, idx))) return rc;
+  auto carry = [&](auto *field) -> int {
+    using T = typename std::remove_pointer<decltype(field)>::type;
+    T *tmp = (T *)e->d_stage;
+    HIPCHK(sphk::launch_gather(tmp, (const T *)field, 1, 0, idx, ni, e->st));
+    HIPCHK(hipMemcpyAsync(field, tmp, sizeof(T) * (size_t)ni,
+                          hipMemcpyDeviceToDevice, e->st));
+    e->launches += 2;
+    return 0;
+  };
+  for (double *f : {d.px, d.py, d.pz})
+    if ((rc = carry(f))) return rc;
+  for (float *f : {d.vx, d.vy, d.vz, d.vfx, d.vfy, d.vfz, d.m, d.u, d.ufull,
+                   d.h, d.rho, d.rho_dh, d.wc, d.wc_dh, d.divv, d.rotx,
+                   d.roty, d.rotz, d.P, d.c, d.f, d.bal, d.vsig, d.lapu,
+                   d.avmax, d.alpha, d.adiff, d.divv_prev, d.divv_dt, d.ax,
+                   d.ay, d.az, d.udt, d.hdt})
+    if ((rc = carry(f))) return rc;
+  for (int *f : {d.perm, d.nd, d.ng, d.nf})
+    if ((rc = carry(f))) return rc;
+  for (int8_t *f : {d.tb, d.mintb})
+    if ((rc = carry(f))) return rc;
+  HIPCHK(sphk::launch_fill_f32(d.r2b, -1.f, ni, e->st));
+  HIPCHK(hipMemsetAsync(e->d_dmax, 0, sizeof(unsigned int), e->st));
+  e->dmax = 0.f;
+  e->perm_stale = true;
+  e->saved_valid = false;
+  e->tiles_c = 0;
+  e->gen++;
+  return finish_binning(e);
+}
+
+// ---- time integration ------------------------------------------------------
+// get_max_active_bin (src/timeline.h)
+int max_active_bin_at(int64_t ti) {
+  if (ti == 0) return 56;  // num_time_bins
+  int bin = 1;
+  while (!((1LL << (bin + 1)) & ti)) ++bin;
+  return bin;
+}
+
+void set_active_bin(swsph_engine *e, int b) {
+  if (e->props.max_active_bin != b) {
+    e->props.max_active_bin = b;
+    e->gen++;  // captured graphs hold the old activity
   }
-  float hmax = 0.f;
-  for (float v : h) hmax = std::max(hmax, v);
-  const double H = (double)hmax * (double)kGammaK;
-  for (int k = 0; k < 3; k++) {
-    const double w = e->grid.dim[k] / e->grid.nc[k];
-    if (H > w)
-      return fail(4,
-                  "kernel support %g exceeds the top-level cell width %g: "
-                  "regrid with fewer cells",
-                  H, w);
+}
+
+// Particle range of cell ci, or of all particles for ci == -1.
+int part_range(swsph_engine *e, int ci, int *lo, int *hi, int *c_lo,
+               int *c_hi) {
+  if (ci == -1) {
+    *lo = 0;
+    *hi = (int)e->n;
+    *c_lo = 0;
+    *c_hi = e->ncells;
+    return 0;
   }
+  int rc = cell_ok(e, ci);
+  if (rc) return rc;
+  *lo = e->cell_start[ci];
+  *hi = e->cell_start[ci + 1];
+  *c_lo = ci;
+  *c_hi = ci + 1;
+  return 0;
+}
+
+// runner_do_drift_part: drift, then the neighbour records, candidate
+// positions and chunk bounds of the drifted particles.
+int drift_cells(swsph_engine *e, int ci, int64_t ti_old, int64_t ti_current) {
+  int lo, hi, c_lo, c_hi, rc;
+  if ((rc = part_range(e, ci, &lo, &hi, &c_lo, &c_hi))) return rc;
+  if (ti_current < ti_old) return fail(1, "drift backwards in time");
+  DriftArgs a;
+  a.dt_drift = (double)(ti_current - ti_old) * e->props.time_base;
+  a.dt_kick_hydro = a.dt_drift;
+  a.dt_therm = a.dt_drift;
+  a.min_u = e->integ.min_u;
+  a.g = e->grid;
+  a.lo = lo;
+  a.hi = hi;
+  HIPCHK(sphk::launch_drift(e->d, a, e->d_dmax, e->st));
+  for (LoopKind k : {LoopKind::Density, LoopKind::Gradient, LoopKind::Force})
+    HIPCHK(sphk::launch_build_records(k, e->d, lo, hi, e->st));
+  e->launches += 4;
+  return refresh_bounds(e, c_lo, c_hi);
+}
+
+int kick_cells(swsph_engine *e, int ci, int64_t ti_current, int mode) {
+  int lo, hi, c_lo, c_hi, rc;
+  if ((rc = part_range(e, ci, &lo, &hi, &c_lo, &c_hi))) return rc;
+  KickArgs a;
+  a.ti_current = ti_current;
+  a.time_base = e->props.time_base;
+  a.min_u = e->integ.min_u;
+  a.mode = mode;
+  a.max_active_bin = max_active_bin_at(ti_current);
+  a.lo = lo;
+  a.hi = hi;
+  HIPCHK(sphk::launch_kick(e->d, a, e->st));
+  e->launches++;
+  return 0;
+}
+
+// runner_do_timestep: new bins of the active particles; returns the next
+// synchronisation point over the range and the number of active particles.
+int timestep_cells(swsph_engine *e, int ci, int64_t ti_current,
+                   int64_t *ti_end_min, int64_t *n_active) {
+  int lo, hi, c_lo, c_hi, rc;
+  if ((rc = part_range(e, ci, &lo, &hi, &c_lo, &c_hi))) return rc;
+  StepArgs a;
+  a.ti_current = ti_current;
+  a.time_base_inv = 1.0 / e->props.time_base;
+  a.dt_min = e->integ.dt_min;
+  a.dt_max = e->integ.dt_max;
+  a.cfl = e->integ.cfl;
+  a.log_max_h_change = e->integ.log_max_h_change;
+  a.max_active_bin = max_active_bin_at(ti_current);
+  a.fixed = e->integ.fixed_bins;
+  a.lo = lo;
+  a.hi = hi;
+  HIPCHK(sphk::launch_zero_u32(e->d_err, 2, e->st));
+  // the minimum starts from max_nr_timesteps
+  const unsigned long long top = 1ull << 57;
+  HIPCHK(hipMemcpyAsync(e->d_tiend, &top, sizeof(top), hipMemcpyHostToDevice,
+                        e->st));
+  HIPCHK(sphk::launch_timestep(e->d, a, e->d_tiend, e->d_err, e->st));
+  HIPCHK(hipMemcpyAsync(e->h_tiend, e->d_tiend, sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipMemcpyAsync(e->h_err, e->d_err, 2 * sizeof(unsigned int),
+                        hipMemcpyDeviceToHost, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  e->launches += 3;
+  if (e->h_err[0])
+    return fail(3, "%u particles want a time-step below dt_min or of size 0",
+                e->h_err[0]);
+  if (ti_end_min) *ti_end_min = (int64_t)*e->h_tiend;
+  if (n_active) *n_active = (int64_t)e->h_err[1];
   return 0;
 }
 
@@ -744,7 +995,6 @@ namespace {
 // hydro_init_part on the active particles and the first density round (with
 // lists: a fresh neighbour list for every active particle).
 int phase_start(swsph_engine *e) {
-  int rc = 0;
   HIPCHK(sphk::launch_init_density(e->d, 0, (int)e->n, e->props.max_active_bin,
                                    e->props.h_max, e->st));
   HIPCHK(sphk::launch_zero_u64(e->d_redo + 3, 1, e->st));
@@ -929,10 +1179,22 @@ int swsph_engine_create(const swsph_props *props, swsph_engine **out) {
           hipSuccess ||
       hipMalloc((void **)&e->d_hg2_all, sizeof(unsigned int)) != hipSuccess ||
       hipHostMalloc((void **)&e->h_hg2_all, sizeof(unsigned int), 0) !=
+          hipSuccess ||
+      hipMalloc((void **)&e->d_dmax, sizeof(unsigned int)) != hipSuccess ||
+      hipMalloc((void **)&e->d_tiend, sizeof(unsigned long long)) !=
+          hipSuccess ||
+      hipMalloc((void **)&e->d_err, 2 * sizeof(unsigned int)) != hipSuccess ||
+      hipHostMalloc((void **)&e->h_small, 2 * sizeof(unsigned int), 0) !=
+          hipSuccess ||
+      hipHostMalloc((void **)&e->h_tiend, sizeof(unsigned long long), 0) !=
+          hipSuccess ||
+      hipHostMalloc((void **)&e->h_err, 2 * sizeof(unsigned int), 0) !=
           hipSuccess) {
     delete e;
     return fail(2, "allocation of engine counters failed");
   }
+  // SPH:max_volume_change default 1.4 (src/hydro_properties.c:42, 122)
+  e->integ.log_max_h_change = logf(powf(1.4f, 0.3333333333f));
   // One table per admissible chunk capacity at most: never reallocated, so
   // table pointers stay valid.
   e->tables.reserve(4);
@@ -953,6 +1215,12 @@ int swsph_engine_destroy(swsph_engine *e) {
   if (e->d_hg2_all) (void)hipFree(e->d_hg2_all);
   if (e->h_hg2_all) (void)hipHostFree(e->h_hg2_all);
   if (e->h_cnt) (void)hipHostFree(e->h_cnt);
+  if (e->d_dmax) (void)hipFree(e->d_dmax);
+  if (e->d_tiend) (void)hipFree(e->d_tiend);
+  if (e->d_err) (void)hipFree(e->d_err);
+  if (e->h_small) (void)hipHostFree(e->h_small);
+  if (e->h_tiend) (void)hipHostFree(e->h_tiend);
+  if (e->h_err) (void)hipHostFree(e->h_err);
   if (e->st) (void)hipStreamDestroy(e->st);
   delete e;
   return 0;
@@ -1011,117 +1279,77 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
   }
   e->n = n;
   e->need_particles = false;
-  if ((rc = reset_dev(e, (size_t)n))) return rc;
-
-  // Counting sort by top-level cell (the binning rule of the oracle).
-  const int *nc = e->grid.nc;
-  double inv_w[3];
-  for (int k = 0; k < 3; k++) inv_w[k] = (double)nc[k] / e->grid.dim[k];
-  std::vector<int> cid(n);
-  e->cell_start.assign(e->ncells + 1, 0);
-  for (int64_t i = 0; i < n; i++) {
-    int c[3];
-    for (int k = 0; k < 3; k++) {
-      int v = (int)std::floor(p->x[3 * i + k] * inv_w[k]);
-      c[k] = v < 0 ? 0 : (v >= nc[k] ? nc[k] - 1 : v);
-    }
-    cid[i] = (c[0] * nc[1] + c[1]) * nc[2] + c[2];
-    e->cell_start[cid[i] + 1]++;
-  }
-  for (int c = 0; c < e->ncells; c++) e->cell_start[c + 1] += e->cell_start[c];
-  std::vector<int> fill(e->cell_start.begin(), e->cell_start.end() - 1);
-  e->perm.assign(n, 0);
-  for (int64_t i = 0; i < n; i++) e->perm[fill[cid[i]]++] = i;
-
-  // Inside each cell, order particles along a Morton curve (10 bits per
-  // axis of the cell-relative position) so that any run of consecutive
-  // particles is spatially compact.
-  {
-    auto spread = [](uint32_t v) {
-      uint64_t x = v & 0x3ff;
-      x = (x | (x << 16)) & 0x030000FFull;
-      x = (x | (x << 8)) & 0x0300F00Full;
-      x = (x | (x << 4)) & 0x030C30C3ull;
-      x = (x | (x << 2)) & 0x09249249ull;
-      return x;
-    };
-    std::vector<std::pair<uint64_t, int64_t>> keyed;
-    for (int c = 0; c < e->ncells; c++) {
-      const int lo = e->cell_start[c], hi = e->cell_start[c + 1];
-      if (hi - lo < 2) continue;
-      const int cc[3] = {c / (nc[1] * nc[2]), (c / nc[2]) % nc[1], c % nc[2]};
-      keyed.resize(hi - lo);
-      for (int s = lo; s < hi; s++) {
-        const int64_t i = e->perm[s];
-        uint64_t key = 0;
-        for (int k = 0; k < 3; k++) {
-          double f = (p->x[3 * i + k] * inv_w[k] - cc[k]) * 1024.0;
-          int q = (int)f;
-          q = q < 0 ? 0 : (q > 1023 ? 1023 : q);
-          key |= spread((uint32_t)q) << (2 - k);
-        }
-        keyed[s - lo] = {key, i};
-      }
-      std::sort(keyed.begin(), keyed.end());
-      for (int s = lo; s < hi; s++) e->perm[s] = keyed[s - lo].second;
-    }
-  }
+  e->started = false;  // a new particle set starts a new time line
   e->ntiles = 0;
-
-  // Gather to sorted SoA and upload.
-  std::vector<double> xd(n);
-  std::vector<float> fd(n);
-  std::vector<int8_t> bd(n);
-  auto up_d = [&](double *dst, int k) -> int {
-    for (int64_t s = 0; s < n; s++) xd[s] = p->x[3 * e->perm[s] + k];
-    HIPCHK(hipMemcpy(dst, xd.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    return 0;
-  };
-  auto up_f = [&](float *dst, const float *src, int stride, int k,
-                  float dflt) -> int {
-    for (int64_t s = 0; s < n; s++)
-      fd[s] = src ? src[stride * e->perm[s] + k] : dflt;
-    HIPCHK(hipMemcpy(dst, fd.data(), sizeof(float) * n, hipMemcpyHostToDevice));
-    return 0;
-  };
+  if ((rc = reset_dev(e, (size_t)n))) return rc;
+  const int ni = (int)n;
+  DevParts &d = e->d;
   if (n > 0) {
-    if ((rc = up_d(e->d.px, 0)) || (rc = up_d(e->d.py, 1)) ||
-        (rc = up_d(e->d.pz, 2)))
+    // Positions in the caller's order to the staging buffer; keys, radix
+    // sort and cell bounds on the device; the sort's values are the
+    // caller's indices, i.e. the permutation sorted slot -> caller index.
+    HIPCHK(hipMemcpyAsync(e->d_stage, p->x, sizeof(double) * 3 * n,
+                          hipMemcpyHostToDevice, e->st));
+    if ((rc = bin_sort(e, e->d_stage, e->d_stage + 1, e->d_stage + 2, 3,
+                       d.perm)))
       return rc;
-    if ((rc = up_f(e->d.vx, p->v, 3, 0, 0.f)) ||
-        (rc = up_f(e->d.vy, p->v, 3, 1, 0.f)) ||
-        (rc = up_f(e->d.vz, p->v, 3, 2, 0.f)) ||
-        (rc = up_f(e->d.m, p->m, 1, 0, 0.f)) ||
-        (rc = up_f(e->d.u, p->u, 1, 0, 0.f)) ||
-        (rc = up_f(e->d.h, p->h, 1, 0, 0.f)) ||
-        (rc = up_f(e->d.rho, p->rho, 1, 0, 0.f)) ||
-        (rc = up_f(e->d.P, p->pressure, 1, 0, 0.f)) ||
-        (rc = up_f(e->d.c, p->soundspeed, 1, 0, 0.f)) ||
-        (rc = up_f(e->d.f, p->f, 1, 0, 0.f)) ||
-        (rc = up_f(e->d.bal, p->balsara, 1, 0, 0.f)) ||
-        (rc = up_f(e->d.alpha, p->alpha_visc, 1, 0, 0.1f)) ||
-        (rc = up_f(e->d.adiff, p->alpha_diff, 1, 0, 0.f)) ||
-        (rc = up_f(e->d.divv_prev, p->div_v_prev, 1, 0, 0.f)))
-      return rc;
-    for (int64_t s = 0; s < n; s++) bd[s] = p->time_bin[e->perm[s]];
-    HIPCHK(hipMemcpy(e->d.tb, bd.data(), n, hipMemcpyHostToDevice));
-    std::vector<int> cs(n);
-    for (int64_t s = 0; s < n; s++) cs[s] = cid[e->perm[s]];
-    HIPCHK(hipMemcpy(e->d.cell, cs.data(), sizeof(int) * n,
-                     hipMemcpyHostToDevice));
+    double *pd[3] = {d.px, d.py, d.pz};
+    for (int k = 0; k < 3; k++)
+      HIPCHK(sphk::launch_gather(pd[k], (const double *)e->d_stage, 3, k,
+                                 d.perm, ni, e->st));
+    // Velocities (predicted and v_full start equal, as in
+    // hydro_first_init_part), then the scalar fields, each through the
+    // staging buffer (stream order keeps uploads and gathers apart).
+    float *sf = (float *)e->d_stage;
+    HIPCHK(hipMemcpyAsync(sf, p->v, sizeof(float) * 3 * n,
+                          hipMemcpyHostToDevice, e->st));
+    float *pv[3] = {d.vx, d.vy, d.vz}, *pvf[3] = {d.vfx, d.vfy, d.vfz};
+    for (int k = 0; k < 3; k++) {
+      HIPCHK(sphk::launch_gather(pv[k], (const float *)sf, 3, k, d.perm, ni,
+                                 e->st));
+      HIPCHK(sphk::launch_gather(pvf[k], (const float *)sf, 3, k, d.perm, ni,
+                                 e->st));
+    }
+    struct Up {
+      const float *src;
+      float *dst, *dst2;
+      float dflt;
+    } ups[] = {{p->m, d.m, nullptr, 0.f},
+               {p->u, d.u, d.ufull, 0.f},
+               {p->h, d.h, nullptr, 0.f},
+               {p->rho, d.rho, nullptr, 0.f},
+               {p->pressure, d.P, nullptr, 0.f},
+               {p->soundspeed, d.c, nullptr, 0.f},
+               {p->f, d.f, nullptr, 0.f},
+               {p->balsara, d.bal, nullptr, 0.f},
+               {p->alpha_visc, d.alpha, nullptr, 0.1f},
+               {p->alpha_diff, d.adiff, nullptr, 0.f},
+               {p->div_v_prev, d.divv_prev, nullptr, 0.f}};
+    for (const Up &u : ups) {
+      if (!u.src) {
+        HIPCHK(sphk::launch_fill_f32(u.dst, u.dflt, ni, e->st));
+        if (u.dst2) HIPCHK(sphk::launch_fill_f32(u.dst2, u.dflt, ni, e->st));
+        continue;
+      }
+      HIPCHK(hipMemcpyAsync(sf, u.src, sizeof(float) * n,
+                            hipMemcpyHostToDevice, e->st));
+      HIPCHK(sphk::launch_gather(u.dst, (const float *)sf, 1, 0, d.perm, ni,
+                                 e->st));
+      if (u.dst2)
+        HIPCHK(sphk::launch_gather(u.dst2, (const float *)sf, 1, 0, d.perm,
+                                   ni, e->st));
+    }
+    int8_t *sb = (int8_t *)e->d_stage;
+    HIPCHK(hipMemcpyAsync(sb, p->time_bin, n, hipMemcpyHostToDevice, e->st));
+    HIPCHK(sphk::launch_gather(d.tb, (const int8_t *)sb, 1, 0, d.perm, ni,
+                               e->st));
+  } else {
+    HIPCHK(sphk::launch_cell_bounds(e->d_key[1], 0, e->ncells,
+                                    e->d_cell_start, d.cell, e->st));
   }
-  HIPCHK(hipMemcpy(e->d_cell_start, e->cell_start.data(),
-                   sizeof(int) * (e->ncells + 1), hipMemcpyHostToDevice));
-  // Neighbour records of every particle from the state just uploaded
-  // (inactive particles keep these for the whole pass), candidate positions
-  // and the chunk tables of the loops.
-  for (LoopKind k : {LoopKind::Density, LoopKind::Gradient, LoopKind::Force})
-    HIPCHK(sphk::launch_build_records(k, e->d, 0, (int)n, e->st));
-  HIPCHK(sphk::launch_build_pos(e->d, e->grid, pos_eps(e), 0, (int)n, e->st));
-  for (int k = 0; k < 3; k++)
-    if (!chunk_table(e, e->loop_c[k], &rc)) return rc;
-  HIPCHK(hipStreamSynchronize(e->st));
-  return check_h_fits(e);
+  e->perm_stale = true;
+  if ((rc = finish_binning(e))) return rc;
+  return check_fits(e);
 }
 
 int swsph_get_field(swsph_engine *e, int field, void *out) {
@@ -1129,6 +1357,8 @@ int swsph_get_field(swsph_engine *e, int field, void *out) {
   std::lock_guard<std::mutex> lk(e->mtx);
   const int64_t n = e->n;
   HIPCHK(hipStreamSynchronize(e->st));
+  int rc0 = 0;
+  if ((rc0 = ensure_perm(e))) return rc0;
   const DevParts &d = e->d;
   auto get_f = [&](const float *src, int stride, int k) -> int {
     std::vector<float> t(n);
@@ -1174,13 +1404,35 @@ int swsph_get_field(swsph_engine *e, int field, void *out) {
     case SWSPH_N_DENSITY: return get_i(d.nd);
     case SWSPH_N_GRADIENT: return get_i(d.ng);
     case SWSPH_N_FORCE: return get_i(d.nf);
-    case SWSPH_MIN_NGB_TIME_BIN: {
+    case SWSPH_MIN_NGB_TIME_BIN:
+    case SWSPH_TIME_BIN: {
       std::vector<int8_t> t(n);
-      if (n) HIPCHK(hipMemcpy(t.data(), d.mintb, n, hipMemcpyDeviceToHost));
+      const int8_t *src = field == SWSPH_TIME_BIN ? d.tb : d.mintb;
+      if (n) HIPCHK(hipMemcpy(t.data(), src, n, hipMemcpyDeviceToHost));
       int8_t *o = (int8_t *)out;
       for (int64_t s = 0; s < n; s++) o[e->perm[s]] = t[s];
       return 0;
     }
+    case SWSPH_X: {
+      const double *src[3] = {d.px, d.py, d.pz};
+      std::vector<double> t(n);
+      double *o = (double *)out;
+      for (int k = 0; k < 3; k++) {
+        if (n)
+          HIPCHK(hipMemcpy(t.data(), src[k], sizeof(double) * n,
+                           hipMemcpyDeviceToHost));
+        for (int64_t s = 0; s < n; s++) o[3 * e->perm[s] + k] = t[s];
+      }
+      return 0;
+    }
+    case SWSPH_V:
+      if ((rc = get_f(d.vx, 3, 0)) || (rc = get_f(d.vy, 3, 1))) return rc;
+      return get_f(d.vz, 3, 2);
+    case SWSPH_V_FULL:
+      if ((rc = get_f(d.vfx, 3, 0)) || (rc = get_f(d.vfy, 3, 1))) return rc;
+      return get_f(d.vfz, 3, 2);
+    case SWSPH_U: return get_f(d.u, 1, 0);
+    case SWSPH_U_FULL: return get_f(d.ufull, 1, 0);
     default:
       return fail(1, "unknown field %d", field);
   }
@@ -1596,6 +1848,117 @@ int swsph_autotune(swsph_engine *e, int repeats, swsph_tune_result *out) {
     }
     out->pass_ms = best_pass;
     out->trials = trials;
+  }
+  return 0;
+}
+
+// ---- time integration -------------------------------------------------------
+
+int swsph_set_integration(swsph_engine *e, const swsph_integration *p) {
+  if (!e || !p) return fail(1, "null argument");
+  if (!(p->cfl > 0.f) || !(p->dt_max > 0.0) || p->dt_min < 0.0 ||
+      !(p->log_max_h_change > 0.f) || p->min_u < 0.f)
+    return fail(1, "invalid integration parameters");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  e->integ = *p;
+  return 0;
+}
+
+int swsph_do_drift_part(swsph_engine *e, int ci, int64_t ti_old,
+                        int64_t ti_current) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->need_particles) return fail(1, "no particles for the current grid");
+  e->gen++;  // positions moved: captured graphs hold stale bounds
+  return drift_cells(e, ci, ti_old, ti_current);
+}
+
+int swsph_do_kick1(swsph_engine *e, int ci, int64_t ti_current) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->need_particles) return fail(1, "no particles for the current grid");
+  return kick_cells(e, ci, ti_current, 1);
+}
+
+int swsph_do_kick2(swsph_engine *e, int ci, int64_t ti_current) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->need_particles) return fail(1, "no particles for the current grid");
+  return kick_cells(e, ci, ti_current, 2);
+}
+
+int swsph_do_timestep(swsph_engine *e, int ci, int64_t ti_current,
+                      int64_t *ti_end_min) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->need_particles) return fail(1, "no particles for the current grid");
+  if (!(e->props.time_base > 0.0)) return fail(1, "time_base must be positive");
+  return timestep_cells(e, ci, ti_current, ti_end_min, nullptr);
+}
+
+int swsph_rebuild(swsph_engine *e) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->need_particles) return fail(1, "no particles for the current grid");
+  int rc = rebin(e);
+  if (rc) return rc;
+  return check_fits(e);
+}
+
+int swsph_engine_step(swsph_engine *e, swsph_step_info *info) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->need_particles)
+    return fail(1, "the grid changed: call swsph_set_particles first");
+  if (!(e->props.time_base > 0.0))
+    return fail(1, "time_base must be positive for time integration");
+  int rc = 0, rebuilt = 0;
+  const int64_t l0 = e->launches;
+  const bool first = !e->started;
+  if (first) {
+    e->ti_current = 0;
+  } else {
+    const int64_t ti_old = e->ti_current;
+    e->ti_current = e->ti_end_min;
+    e->gen++;
+    if ((rc = drift_cells(e, -1, ti_old, e->ti_current))) return rc;
+    double H, dm;
+    if ((rc = read_fit(e, &H, &dm))) return rc;
+    if (!fits(e, H, dm)) {
+      if ((rc = rebin(e))) return rc;
+      rebuilt = 1;
+    }
+  }
+  const float dx_max = e->dmax;
+  set_active_bin(e, max_active_bin_at(e->ti_current));
+  swsph_pass_stats ps{};
+  rc = pass_batched_locked(e, &ps);
+  if (rc == 4 && e->dmax > 0.f) {
+    // the smoothing lengths outgrew the slack the drift left: re-bin and
+    // repeat the pass from the lengths found so far
+    if ((rc = rebin(e))) return rc;
+    rebuilt = 1;
+    rc = pass_batched_locked(e, &ps);
+  }
+  if (rc) return rc;
+  if (!first && (rc = kick_cells(e, -1, e->ti_current, 2))) return rc;
+  int64_t tiend = 0, nact = 0;
+  if ((rc = timestep_cells(e, -1, e->ti_current, &tiend, &nact))) return rc;
+  if ((rc = kick_cells(e, -1, e->ti_current, 1))) return rc;
+  HIPCHK(hipStreamSynchronize(e->st));
+  if (tiend <= e->ti_current)
+    return fail(3, "the next synchronisation point does not advance");
+  e->ti_end_min = tiend;
+  e->started = true;
+  if (info) {
+    info->ti_current = e->ti_current;
+    info->ti_end_min = tiend;
+    info->max_active_bin = e->props.max_active_bin;
+    info->rebuilt = rebuilt;
+    info->n_active = nact;
+    info->dx_max = dx_max;
+    info->launches = e->launches - l0;
+    info->pass = ps;
   }
   return 0;
 }

@@ -103,6 +103,10 @@ struct DevParts {
   int *lfb;
   int *sel;
   int *cell;  // top-level cell of each particle
+  // time integration (struct xpart: v_full, u_full) and the caller's index
+  // of every slot
+  float *vfx, *vfy, *vfz, *ufull;
+  int *perm;
 };
 
 // Per-particle neighbour lists (Verlet lists with a skin): the candidates
@@ -156,6 +160,32 @@ struct GhostArgs {
 };
 
 enum class LoopKind { Density, Gradient, Force };
+
+// Drift of particles [lo, hi) from ti_old to ti_current (intervals in
+// physical time; without cosmology all three are equal).
+struct DriftArgs {
+  double dt_drift, dt_kick_hydro, dt_therm;
+  float min_u;  // hydro_props minimal_internal_energy
+  GridDesc g;
+  int lo, hi;
+};
+
+// Half-kicks of particles [lo, hi): mode 1 kick1, mode 2 kick2.
+struct KickArgs {
+  long long ti_current;
+  double time_base;
+  float min_u;
+  int mode, max_active_bin, lo, hi;
+};
+
+// Time steps of the active particles of [lo, hi).
+struct StepArgs {
+  long long ti_current;
+  double time_base_inv;
+  double dt_min, dt_max;
+  float cfl, log_max_h_change;
+  int max_active_bin, fixed, lo, hi;
+};
 
 // Scratch of the tile builder, sized for n particles and ncells cells.
 struct TileScratch {
@@ -235,4 +265,33 @@ hipError_t launch_count(const DevParts &d, int n, int max_active_bin,
                         unsigned long long *out3, hipStream_t st);
 hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
                                 int hi, hipStream_t st);
+
+// ---- binning and time integration (sph_step.hip) ----
+// Keys (cell << 30 | in-cell Morton code) of n positions, component k of
+// particle i at p_k[stride * i]; val[i] = i.
+hipError_t launch_bin_keys(const double *px, const double *py, const double *pz,
+                           int stride, const GridDesc &g, const double inv_w[3],
+                           int n, unsigned long long *key, int *val,
+                           hipStream_t st);
+// Stable radix sort of (key, value) pairs on the low `bits` key bits; with
+// temp == nullptr only sets `bytes` (rocprim::radix_sort_pairs).
+hipError_t sort_pairs(void *temp, size_t &bytes, const unsigned long long *kin,
+                      unsigned long long *kout, const int *vin, int *vout,
+                      int n, int bits, hipStream_t st);
+hipError_t launch_wrap(const DevParts &d, const GridDesc &g, int n,
+                       hipStream_t st);
+// cell[k] and cell_start[0..ncells] from sorted keys.
+hipError_t launch_cell_bounds(const unsigned long long *key, int n, int ncells,
+                              int *cell_start, int *cell, hipStream_t st);
+// dst[k] = src[stride * idx[k] + comp]
+template <class T>
+hipError_t launch_gather(T *dst, const T *src, int stride, int comp,
+                         const int *idx, int n, hipStream_t st);
+hipError_t launch_fill_i8(int8_t *p, int8_t v, int n, hipStream_t st);
+hipError_t launch_drift(const DevParts &d, const DriftArgs &a,
+                        unsigned int *dmax, hipStream_t st);
+hipError_t launch_kick(const DevParts &d, const KickArgs &a, hipStream_t st);
+hipError_t launch_timestep(const DevParts &d, const StepArgs &a,
+                           unsigned long long *ti_end_min, unsigned int *err,
+                           hipStream_t st);
 }  // namespace sphk

@@ -32,10 +32,10 @@ double now_ms() {
 }
 
 // Device bytes of an engine for n particles on ncells cells: particle
-// arrays, records, state copy and tile scratch (~520 B per particle), chunk
-// tables and cell arrays, plus a fixed part.
+// arrays, records, state copy, tile scratch and binning buffers (~600 B per
+// particle), chunk tables and cell arrays, plus a fixed part.
 size_t engine_bytes(int64_t n, int64_t ncells) {
-  return (size_t)n * 520u + (size_t)ncells * 64u + ((size_t)4 << 20);
+  return (size_t)n * 600u + (size_t)ncells * 64u + ((size_t)4 << 20);
 }
 
 struct Box {

@@ -29,7 +29,12 @@ FIELDS = {
     "h_dt": (20, np.float32, 1), "n_density": (21, np.int32, 1),
     "n_gradient": (22, np.int32, 1), "n_force": (23, np.int32, 1),
     "min_ngb_time_bin": (24, np.int8, 1),
+    # particle state advanced by the time integration
+    "x": (32, np.float64, 3), "v": (33, np.float32, 3),
+    "v_full": (34, np.float32, 3), "u": (35, np.float32, 1),
+    "u_full": (36, np.float32, 1), "time_bin": (37, np.int8, 1),
 }
+PASS_FIELDS = [k for k, v in FIELDS.items() if v[0] < 25]
 
 
 class _TuneResult(C.Structure):
@@ -63,6 +68,37 @@ class PassStats(C.Structure):
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Integration(C.Structure):
+    """swsph_integration: time-step parameters (SPH:CFL_condition,
+    hydro_props.log_max_h_change, minimal_internal_energy, dt_min, dt_max)
+    and whether the particles keep the time bins they were given."""
+    _fields_ = [("cfl", C.c_float), ("log_max_h_change", C.c_float),
+                ("min_u", C.c_float), ("dt_min", C.c_double),
+                ("dt_max", C.c_double), ("fixed_bins", C.c_int)]
+
+
+class StepInfo(C.Structure):
+    _fields_ = [("ti_current", C.c_int64), ("ti_end_min", C.c_int64),
+                ("max_active_bin", C.c_int), ("rebuilt", C.c_int),
+                ("n_active", C.c_int64), ("dx_max", C.c_float),
+                ("launches", C.c_int64), ("pass_", PassStats)]
+
+    def as_dict(self) -> dict:
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "pass_"}
+        d["pass"] = self.pass_.as_dict()
+        return d
+
+
+def integration(cfl: float = 0.1, max_volume_change: float = 1.4,
+                min_u: float = 0.0, dt_min: float = 0.0, dt_max: float = 1e30,
+                fixed_bins: bool = False) -> Integration:
+    """Parameters with SWIFT's defaults (src/hydro_properties.c:42, 122)."""
+    third = np.float32(0.3333333333)
+    lmhc = np.log(np.power(np.float32(max_volume_change), third,
+                           dtype=np.float32), dtype=np.float32)
+    return Integration(cfl, float(lmhc), min_u, dt_min, dt_max, int(fixed_bins))
 
 
 def _props(p: HydroProps) -> _Props:
@@ -138,7 +174,7 @@ class Engine:
         return out
 
     def results(self, names=None) -> dict:
-        return {k: self.get(k) for k in (names or FIELDS)}
+        return {k: self.get(k) for k in (names or PASS_FIELDS)}
 
     def num_cells(self) -> int:
         n = C.c_int()
@@ -273,6 +309,40 @@ class Engine:
         check(lib().swsph_get_timing(self._h, ms, nl), "swsph_get_timing")
         return list(ms), list(nl)
 
+    # -- time integration (runner_do_drift_part, kick1, kick2, timestep) --
+    def set_integration(self, p: Integration) -> None:
+        check(lib().swsph_set_integration(self._h, C.byref(p)),
+              "swsph_set_integration")
+
+    def runner_do_drift_part(self, ci: int, ti_old: int, ti_current: int) -> None:
+        check(lib().swsph_do_drift_part(self._h, ci, C.c_int64(ti_old),
+                                        C.c_int64(ti_current)),
+              "swsph_do_drift_part")
+
+    def runner_do_kick1(self, ci: int, ti_current: int) -> None:
+        check(lib().swsph_do_kick1(self._h, ci, C.c_int64(ti_current)),
+              "swsph_do_kick1")
+
+    def runner_do_kick2(self, ci: int, ti_current: int) -> None:
+        check(lib().swsph_do_kick2(self._h, ci, C.c_int64(ti_current)),
+              "swsph_do_kick2")
+
+    def runner_do_timestep(self, ci: int, ti_current: int) -> int:
+        out = C.c_int64()
+        check(lib().swsph_do_timestep(self._h, ci, C.c_int64(ti_current),
+                                      C.byref(out)), "swsph_do_timestep")
+        return out.value
+
+    def rebuild(self) -> None:
+        check(lib().swsph_rebuild(self._h), "swsph_rebuild")
+
+    def step(self) -> dict:
+        """One engine_step of the gas (see swsph_engine_step)."""
+        info = StepInfo()
+        check(lib().swsph_engine_step(self._h, C.byref(info)),
+              "swsph_engine_step")
+        return info.as_dict()
+
     def cell_coords(self, c: int) -> tuple:
         nx, ny, nz = self.props.ncell
         return c // (ny * nz), (c // nz) % ny, c % nz
@@ -282,4 +352,5 @@ class Engine:
         return ((cx % nx) * ny + (cy % ny)) * nz + (cz % nz)
 
 
-__all__ = ["Engine", "SwsphError", "FIELDS", "PassStats"]
+__all__ = ["Engine", "SwsphError", "FIELDS", "PASS_FIELDS", "PassStats",
+           "Integration", "StepInfo", "integration"]

@@ -176,3 +176,14 @@ def set_active_fraction(st: dict, fraction: float, seed: int = 1,
 
 def copy_state(st: dict) -> dict:
     return {k: np.array(v, copy=True) for k, v in st.items()}
+
+
+def set_bin_hierarchy(st: dict, fractions: dict, seed: int = 2) -> None:
+    """Random time bins with the given fraction of particles per bin
+    (e.g. {1: 0.015, 2: 0.045, 3: 0.19, 4: 0.75}: at times that are odd
+    multiples of the smallest step 1.5 % of the particles are active, 6 %,
+    25 % and 100 % at multiples of 2, 4 and 8 of it)."""
+    rng = np.random.default_rng(seed)
+    bins = np.array(sorted(fractions), np.int8)
+    p = np.array([fractions[b] for b in sorted(fractions)], np.float64)
+    st["time_bin"] = rng.choice(bins, size=st["h"].shape[0], p=p / p.sum())

@@ -13,7 +13,7 @@ import ctypes as C
 import numpy as np
 
 from ._lib import SwsphError, lib
-from .engine import FIELDS, _HostParts, _props
+from .engine import FIELDS, PASS_FIELDS, _HostParts, _props
 from .ics import HydroProps
 
 
@@ -47,6 +47,8 @@ class Server:
         """`fields`: names of the result fields to return (None: all)."""
         mask = 0
         for k in fields or ():
+            if k not in PASS_FIELDS:
+                raise ValueError(f"{k} is not an output of a pass")
             mask |= 1 << FIELDS[k][0]
         o = _Opts(float(hbm_budget_bytes), int(max_inflight), int(use_graphs),
                   mask)
@@ -120,7 +122,7 @@ class Server:
         return out
 
     def results(self, rid: int, names=None) -> dict:
-        return {k: self.get(rid, k) for k in (names or self._fields or FIELDS)}
+        return {k: self.get(rid, k) for k in (names or self._fields or PASS_FIELDS)}
 
     def release(self, rid: int) -> None:
         _check(lib().swsph_server_release(self._s, C.c_int64(rid)),

@@ -80,3 +80,43 @@ def assert_parity(res: dict, ref: dict, active: np.ndarray, fields=None,
             bad.append(f"{k}: {errs[k]} mismatching particles")
     assert not bad, "; ".join(bad) + f"  (all: {errs})"
     return errs
+
+
+# The viscosity and diffusion switches (hydro_prepare_force,
+# src/hydro/SPHENIX/hydro.h:840-954) divide cancelling sums by the step
+# dt_alpha: alpha_loc = alpha_max S / (c^2 + S) with
+# S = (gamma_K h)^2 max(0, -(div_v - div_v_prev) / dt_alpha), and
+# alpha_diff += (beta gamma_K h laplace_u / sqrt(u) - ...) dt_alpha.  Their
+# error is the error of div_v and laplace_u (within the tolerances above,
+# relative to the sums' magnitudes) carried through those expressions; the
+# update itself has a sensitivity <= 1 to alpha_loc in both of its branches.
+KERNEL_GAMMA = 1.936492
+
+
+def alpha_bounds(ref: dict, pre: dict, props, time_base: float):
+    """Per-particle bounds on |alpha - alpha64| for alpha_visc and
+    alpha_diff, given the FP64 rerun `ref` and the pass inputs `pre`."""
+    tb = pre["time_bin"].astype(np.int64)
+    dt_a = np.where(tb > 0, np.ldexp(1.0, np.clip(tb, 0, 60) + 1) * time_base, 0.0)
+    ksp = pre["h"].astype(np.float64) * KERNEL_GAMMA
+    c2 = np.asarray(ref["soundspeed"], np.float64) ** 2
+    ddv = DENSITY_TOL * np.maximum(np.abs(ref["div_v"]), ref["s_div_v"])
+    with np.errstate(divide="ignore", invalid="ignore"):
+        b_visc = np.where(dt_a > 0,
+                          props.alpha_max * ksp ** 2 * ddv / (dt_a * c2), 0.0)
+    dl = FORCE_TOL * np.maximum(np.abs(ref["laplace_u"]), ref["s_laplace_u"])
+    b_diff = (props.diff_beta * ksp * dl / np.sqrt(pre["u"].astype(np.float64))
+              + FORCE_TOL * np.abs(pre["alpha_diff"] - props.diff_alpha_min)
+              * np.asarray(ref["v_sig"], np.float64) / ksp) * dt_a
+    b_visc = b_visc + FORCE_TOL * np.abs(ref["alpha_visc"]) + 1e-7
+    b_diff = b_diff + FORCE_TOL * np.abs(ref["alpha_diff"]) + 1e-7
+    return b_visc, b_diff
+
+
+def assert_alpha(res: dict, ref: dict, pre: dict, props, time_base: float,
+                 active: np.ndarray) -> None:
+    bv, bd = alpha_bounds(ref, pre, props, time_base)
+    for k, b in (("alpha_visc", bv), ("alpha_diff", bd)):
+        e = np.abs(np.asarray(res[k], np.float64) - ref[k])
+        bad = active & (e > b)
+        assert not bad.any(), (k, int(bad.sum()), float((e / b)[active].max()))
