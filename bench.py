@@ -223,6 +223,91 @@ def clustered_bench(n_side: int, ncell: int, steps: int, plain: bool,
     return out
 
 
+def _hbm_used_bytes() -> int:
+    """Device memory in use (hipMemGetInfo through the HIP runtime)."""
+    import ctypes as C
+    try:
+        hip = C.CDLL("libamdhip64.so")
+    except OSError:
+        return -1
+    free, total = C.c_size_t(), C.c_size_t()
+    if hip.hipMemGetInfo(C.byref(free), C.byref(total)) != 0:
+        return -1
+    return int(total.value - free.value)
+
+
+def evolve_bench(n_side: int, ncell: int, nsteps: int, seed: int = 0) -> dict:
+    """Multi-step run of the clustered construction: forces at t = 0 for all
+    particles (the initial h estimate converged by the ghost), then `nsteps`
+    drift-kick steps on a fixed 4-level time-bin hierarchy (bins 1..4 hold
+    1.5 / 4.5 / 19 / 75 % of the particles, so 1.5, 6, 25 or 100 % are
+    active), state resident in HBM, re-binned on the device when particles
+    leave the stencil."""
+    from swift_amd import Engine, HydroProps, ics
+    from swift_amd.engine import integration
+    N = n_side ** 3
+    t0 = time.perf_counter()
+    st = ics.clustered_box(N, seed=seed, n_spheres=1000, a_min=0.002,
+                           a_max=0.02)
+    ics.set_bin_hierarchy(st, {1: 0.015, 2: 0.045, 3: 0.19, 4: 0.75},
+                          seed=seed + 3)
+    t_ics = time.perf_counter() - t0
+    print(f"bench: evolve box built ({t_ics:.1f} s)", file=sys.stderr, flush=True)
+    # time line: the longest step (bin 4, 32 units) is 3.2e-8
+    props = HydroProps(ncell=(ncell,) * 3, time_base=1e-9)
+    eng = Engine(props)
+    ts = time.perf_counter()
+    eng.set_particles(st)
+    eng.sync()
+    t_up = time.perf_counter() - ts
+    del st
+    eng.set_integration(integration(fixed_bins=True))
+    ts = time.perf_counter()
+    first = eng.step()
+    t_first = time.perf_counter() - ts
+    print(f"bench: evolve step 0 (all active, {first['pass']['ghost_rounds']} "
+          f"ghost rounds) {t_first:.2f} s", file=sys.stderr, flush=True)
+    steps = []
+    for k in range(nsteps):
+        ts = time.perf_counter()
+        info = eng.step()          # returns after the step has completed
+        dt = time.perf_counter() - ts
+        p = info["pass"]
+        steps.append({"ti": info["ti_current"], "active": info["n_active"],
+                      "ms": 1e3 * dt, "rebuilt": info["rebuilt"],
+                      "ghost_rounds": p["ghost_rounds"],
+                      "interactions": p["n_density"] + p["n_force"]})
+        print(f"bench: evolve step {k + 1}: {info['n_active']} active, "
+              f"{1e3 * dt:.1f} ms", file=sys.stderr, flush=True)
+    hbm = _hbm_used_bytes()
+    eng.close()
+    ms = [s["ms"] for s in steps]
+    inter = sum(s["interactions"] for s in steps)
+    wall = sum(ms) * 1e-3
+    by_frac = {}
+    for s in steps:
+        f = round(100.0 * s["active"] / N, 1)
+        by_frac.setdefault(f, []).append(s["ms"])
+    return {
+        "workload": f"clustered box {n_side}^3 ({N} particles), 1000 Plummer "
+                    "spheres + 30 % uniform, u log-uniform [0.1, 100], "
+                    f"{ncell}^3 top-level cells, {nsteps} drift-kick steps on "
+                    "a fixed 4-level time-bin hierarchy (active 100/25/6/1.5 %),"
+                    " state resident in HBM",
+        "particles": N, "steps": nsteps,
+        "value": inter / wall, "unit": "interactions/s",
+        "ms_per_step": 1e3 * wall / nsteps,
+        "p50_step_ms": _pct([m * 1e-3 for m in ms], 50) * 1e3,
+        "p99_step_ms": _pct([m * 1e-3 for m in ms], 99) * 1e3,
+        "ms_by_active_percent": {str(k): float(np.mean(v))
+                                 for k, v in sorted(by_frac.items())},
+        "rebuilds": sum(s["rebuilt"] for s in steps),
+        "initial_step_s": t_first, "upload_s": t_up, "ics_s": t_ics,
+        "hbm_used_gb": hbm / 1e9 if hbm >= 0 else None,
+        "per_step": steps,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -247,6 +332,11 @@ def main() -> int:
     ap.add_argument("--clustered-n", type=int, default=256,
                     help="particles per axis of the clustered workload (0: skip)")
     ap.add_argument("--clustered-ncell", type=int, default=48)
+    ap.add_argument("--evolve-n", type=int, default=0,
+                    help="particles per axis of the multi-step clustered run "
+                         "(0: skip; 512 is the full-size case)")
+    ap.add_argument("--evolve-ncell", type=int, default=96)
+    ap.add_argument("--evolve-steps", type=int, default=20)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -407,6 +497,12 @@ def main() -> int:
         out["clustered"] = clustered_bench(args.clustered_n, args.clustered_ncell,
                                            args.steps, not args.no_plain)
         print(f"bench: clustered {out['clustered']['ms_per_step']:.1f} ms/step",
+              file=sys.stderr, flush=True)
+
+    if args.evolve_n > 0:
+        out["evolve"] = evolve_bench(args.evolve_n, args.evolve_ncell,
+                                     args.evolve_steps)
+        print(f"bench: evolve {out['evolve']['ms_per_step']:.1f} ms/step",
               file=sys.stderr, flush=True)
 
     if args.serving_s > 0:
