@@ -151,6 +151,17 @@ struct swsph_engine {
   unsigned long long *h_tiend = nullptr;  // pinned [1]
   unsigned int *h_err = nullptr;          // pinned [2]
   float dmax = 0.f;
+  // The neighbour records of every particle match its fields (the passes
+  // then rebuild only those of the particles they update), and the
+  // generation of the fields a saved state was taken from.
+  bool recs_ok = false;
+  uint64_t fields_epoch = 1, saved_epoch = 0;
+  // After a restore of the pass state, the records of the particles active
+  // in the last pass describe the overwritten values: valid only if the
+  // next pass updates the same particles (kNoMab: no such restore pending).
+  static constexpr int kNoMab = -1000;
+  int stale_mab = kNoMab;
+  int last_pass_mab = kNoMab;
 };
 
 namespace {
@@ -177,6 +188,28 @@ float pos_eps(const swsph_engine *e) {
   double w = 0.0;
   for (int k = 0; k < 3; k++) w = std::max(w, e->grid.dim[k] / e->grid.nc[k]);
   return (float)(1.5 * w * 0x1p-19);
+}
+
+// The records of some particle no longer match its fields: the next pass
+// rebuilds all of them (captured graphs are re-captured).
+void invalidate_records(swsph_engine *e) {
+  if (e->recs_ok) e->gen++;
+  e->recs_ok = false;
+}
+
+// Before a pass (never inside a graph capture): rebuild every particle's
+// records if some do not match its fields.
+int ensure_records(swsph_engine *e) {
+  if (e->stale_mab != swsph_engine::kNoMab &&
+      e->props.max_active_bin != e->stale_mab)
+    invalidate_records(e);
+  e->stale_mab = swsph_engine::kNoMab;
+  if (e->recs_ok) return 0;
+  for (LoopKind k : {LoopKind::Density, LoopKind::Gradient, LoopKind::Force})
+    HIPCHK(sphk::launch_build_records(k, e->d, 0, (int)e->n, e->st));
+  e->launches += 3;
+  e->recs_ok = true;
+  return 0;
 }
 
 LoopArgs loop_args(const swsph_engine *e) {
@@ -252,7 +285,7 @@ void free_dev(swsph_engine *e) {
 
 // Chunk table for capacity C: runs of <= C consecutive particles of a cell.
 // Built on first use; boxes and radii are filled by refresh_bounds.
-int refresh_bounds(swsph_engine *e, int c_lo, int c_hi);
+int refresh_bounds(swsph_engine *e, int c_lo, int c_hi, bool pos = true);
 
 swsph_engine::ChunkTable *chunk_table(swsph_engine *e, int C, int *rc) {
   *rc = 0;
@@ -338,14 +371,17 @@ ChunkArgs chunk_args(const swsph_engine::ChunkTable *t) {
   return ca;
 }
 
-// Candidate positions / radii and chunk bounds of cells [c_lo, c_hi), after
-// smoothing lengths changed.
-int refresh_bounds(swsph_engine *e, int c_lo, int c_hi) {
+// Candidate positions / radii (unless `pos` is false: already written) and
+// chunk bounds of cells [c_lo, c_hi), after positions or smoothing lengths
+// changed.
+int refresh_bounds(swsph_engine *e, int c_lo, int c_hi, bool pos) {
   const int lo = e->cell_start[c_lo], hi = e->cell_start[c_hi];
   if (c_lo == 0 && c_hi == e->ncells)
     HIPCHK(sphk::launch_zero_u32(e->d_hg2_all, 1, e->st));
-  HIPCHK(sphk::launch_build_pos(e->d, e->grid, pos_eps(e), lo, hi, e->st));
-  e->launches++;
+  if (pos) {
+    HIPCHK(sphk::launch_build_pos(e->d, e->grid, pos_eps(e), lo, hi, e->st));
+    e->launches++;
+  }
   for (auto &t : e->tables) {
     if (!t.valid) continue;
     HIPCHK(sphk::launch_chunk_bounds(e->d, t.d_start, t.cell_chunk_h[c_lo],
@@ -621,6 +657,7 @@ int collect_timing(swsph_engine *e) {
 int task(swsph_engine *e, LoopKind k, bool subset, int ci, int cj, bool both) {
   int rc;
   if ((rc = cell_ok(e, ci)) || (rc = cell_ok(e, cj))) return rc;
+  if ((rc = ensure_records(e))) return rc;
   TaskArgs ta{};
   double t[3];
   if (!adjacent(e, ci, cj, t))
@@ -815,6 +852,9 @@ int finish_binning(swsph_engine *e) {
     HIPCHK(sphk::launch_build_records(k, e->d, 0, (int)e->n, e->st));
   HIPCHK(sphk::launch_build_pos(e->d, e->grid, pos_eps(e), 0, (int)e->n,
                                 e->st));
+  e->recs_ok = true;
+  e->stale_mab = swsph_engine::kNoMab;
+  e->fields_epoch++;
   for (auto &t : e->tables) t.valid = false;
   for (int k = 0; k < 3; k++)
     if (!chunk_table(e, e->loop_c[k], &rc)) return rc;
@@ -922,14 +962,19 @@ int drift_cells(swsph_engine *e, int ci, int64_t ti_old, int64_t ti_current) {
   a.dt_kick_hydro = a.dt_drift;
   a.dt_therm = a.dt_drift;
   a.min_u = e->integ.min_u;
+  a.pos_eps = pos_eps(e);
   a.g = e->grid;
   a.lo = lo;
   a.hi = hi;
+  // the drift rewrites the records of the particles it moves
   HIPCHK(sphk::launch_drift(e->d, a, e->d_dmax, e->st));
-  for (LoopKind k : {LoopKind::Density, LoopKind::Gradient, LoopKind::Force})
-    HIPCHK(sphk::launch_build_records(k, e->d, lo, hi, e->st));
-  e->launches += 4;
-  return refresh_bounds(e, c_lo, c_hi);
+  e->launches++;
+  e->fields_epoch++;
+  if (ci == -1) {
+    e->recs_ok = true;
+    e->stale_mab = swsph_engine::kNoMab;
+  }
+  return refresh_bounds(e, c_lo, c_hi, false);
 }
 
 int kick_cells(swsph_engine *e, int ci, int64_t ti_current, int mode) {
@@ -945,6 +990,11 @@ int kick_cells(swsph_engine *e, int ci, int64_t ti_current, int mode) {
   a.hi = hi;
   HIPCHK(sphk::launch_kick(e->d, a, e->st));
   e->launches++;
+  if (mode == 2) {
+    // hydro_reset_predicted_values changed v, u, P and c
+    invalidate_records(e);
+    e->fields_epoch++;
+  }
   return 0;
 }
 
@@ -971,6 +1021,7 @@ int timestep_cells(swsph_engine *e, int ci, int64_t ti_current,
   HIPCHK(hipMemcpyAsync(e->d_tiend, &top, sizeof(top), hipMemcpyHostToDevice,
                         e->st));
   HIPCHK(sphk::launch_timestep(e->d, a, e->d_tiend, e->d_err, e->st));
+  if (!a.fixed) invalidate_records(e);  // force records hold the time bins
   HIPCHK(hipMemcpyAsync(e->h_tiend, e->d_tiend, sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, e->st));
   HIPCHK(hipMemcpyAsync(e->h_err, e->d_err, 2 * sizeof(unsigned int),
@@ -1040,7 +1091,9 @@ int phase_finish(swsph_engine *e) {
   int rc = 0;
   const int n = (int)e->n;
   const bool queue = e->schedule >= 1;
-  HIPCHK(sphk::launch_build_records(LoopKind::Gradient, e->d, 0, n, e->st));
+  // the records of the inactive particles are current (ensure_records)
+  const int mab = e->props.max_active_bin;
+  HIPCHK(sphk::launch_build_records(LoopKind::Gradient, e->d, 0, n, e->st, mab));
   e->launches++;
   if ((rc = refresh_bounds(e, 0, e->ncells))) return rc;
   if (e->use_lists) {
@@ -1054,7 +1107,7 @@ int phase_finish(swsph_engine *e) {
     return rc;
   }
   HIPCHK(sphk::launch_extra_ghost(e->d, ghost_args(e), 0, n, e->st));
-  HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, 0, n, e->st));
+  HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, 0, n, e->st, mab));
   e->launches += 2;
   if ((rc = queue ? queue_loop(e, LoopKind::Force, 0, nullptr, false)
                   : stencil(e, LoopKind::Force, false)))
@@ -1117,6 +1170,7 @@ int copy_state(swsph_engine *e, std::vector<float *> &buf, bool save) {
   for (size_t k = 0; k < f.size(); k++)
     HIPCHK(hipMemcpyAsync(save ? buf[k] : f[k], save ? f[k] : buf[k],
                           sizeof(float) * n, hipMemcpyDeviceToDevice, e->st));
+  if (!save) e->stale_mab = e->last_pass_mab;
   return 0;
 }
 
@@ -1130,6 +1184,8 @@ int pass_batched_locked(swsph_engine *e, swsph_pass_stats *stats) {
   e->list_fallback = 0;
   e->density_work = 0;
   if (e->use_lists && (rc = ensure_lists(e))) return rc;
+  if ((rc = ensure_records(e))) return rc;
+  e->last_pass_mab = e->props.max_active_bin;
   if ((rc = phase_start(e))) return rc;
   int redo = 0, it = 0;
   for (; it < e->props.max_iter; it++) {
@@ -1470,6 +1526,7 @@ int swsph_init_cell(swsph_engine *e, int ci) {
   std::lock_guard<std::mutex> lk(e->mtx);
   int rc = cell_ok(e, ci);
   if (rc) return rc;
+  if ((rc = ensure_records(e))) return rc;
   HIPCHK(sphk::launch_init_density(e->d, e->cell_start[ci],
                                    e->cell_start[ci + 1],
                                    e->props.max_active_bin, e->props.h_max,
@@ -1538,6 +1595,8 @@ int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
   const int64_t l0 = e->launches;
   std::vector<int> up;
   e->density_work = 0;
+  if ((rc = ensure_records(e))) return rc;
+  e->last_pass_mab = e->props.max_active_bin;
   HIPCHK(sphk::launch_zero_u64(e->d_redo + 3, 1, e->st));
   HIPCHK(sphk::launch_zero_u32(e->d_hg2_all, 1, e->st));
   for (int c = 0; c < e->ncells; c++) {
@@ -1600,6 +1659,8 @@ int swsph_pass_graph(swsph_engine *e, swsph_pass_stats *stats) {
   e->list_rebuilt = 0;
   e->list_fallback = 0;
   e->density_work = 0;
+  if ((rc = ensure_records(e))) return rc;
+  e->last_pass_mab = e->props.max_active_bin;
   // Re-runs captured: as many as the last pass needed.
   const int reruns = std::max(1, std::min(e->props.max_iter, e->last_rounds) - 1);
   if (!e->g_density || e->graph_gen != e->gen || e->graph_reruns != reruns) {
@@ -1652,6 +1713,7 @@ int swsph_save_state(swsph_engine *e) {
     HIPCHK(hipMemcpyAsync(e->saved[k], src[k], sizeof(float) * n,
                           hipMemcpyDeviceToDevice, e->st));
   e->saved_valid = true;
+  e->saved_epoch = e->fields_epoch;
   return 0;
 }
 
@@ -1665,6 +1727,12 @@ int swsph_restore_state(swsph_engine *e) {
   for (size_t k = 0; k < dst.size(); k++)
     HIPCHK(hipMemcpyAsync(dst[k], e->saved[k], sizeof(float) * n,
                           hipMemcpyDeviceToDevice, e->st));
+  if (e->saved_epoch != e->fields_epoch) {
+    invalidate_records(e);  // fields changed outside passes since the save
+    e->fields_epoch++;
+  } else {
+    e->stale_mab = e->last_pass_mab;
+  }
   return 0;
 }
 

@@ -165,7 +165,8 @@ enum class LoopKind { Density, Gradient, Force };
 // physical time; without cosmology all three are equal).
 struct DriftArgs {
   double dt_drift, dt_kick_hydro, dt_therm;
-  float min_u;  // hydro_props minimal_internal_energy
+  float min_u;    // hydro_props minimal_internal_energy
+  float pos_eps;  // of the candidate records (LoopArgs::pos_eps)
   GridDesc g;
   int lo, hi;
 };
@@ -263,8 +264,10 @@ hipError_t launch_end_force(const DevParts &d, int lo, int hi,
                             int max_active_bin, hipStream_t st);
 hipError_t launch_count(const DevParts &d, int n, int max_active_bin,
                         unsigned long long *out3, hipStream_t st);
+// Records of particles [lo, hi); with max_active_bin >= 0 only those of the
+// active particles.
 hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
-                                int hi, hipStream_t st);
+                                int hi, hipStream_t st, int max_active_bin = -1);
 
 // ---- binning and time integration (sph_step.hip) ----
 // Keys (cell << 30 | in-cell Morton code) of n positions, component k of
@@ -288,6 +291,8 @@ template <class T>
 hipError_t launch_gather(T *dst, const T *src, int stride, int comp,
                          const int *idx, int n, hipStream_t st);
 hipError_t launch_fill_i8(int8_t *p, int8_t v, int n, hipStream_t st);
+// Drift of [lo, hi), rewriting the neighbour and candidate records of the
+// drifted particles.
 hipError_t launch_drift(const DevParts &d, const DriftArgs &a,
                         unsigned int *dmax, hipStream_t st);
 hipError_t launch_kick(const DevParts &d, const KickArgs &a, hipStream_t st);

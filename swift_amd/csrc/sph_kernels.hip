@@ -25,17 +25,7 @@ namespace {
 __global__ void k_build_pos(DevParts d, GridDesc g, float pos_eps, int lo,
                             int hi) {
   const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi) return;
-  const bool inh = d.tb[i] == kBinInhibited;
-  const int c = d.cell[i];
-  const int cz = c % g.nc[2], cy = (c / g.nc[2]) % g.nc[1];
-  const int cx = c / (g.nc[2] * g.nc[1]);
-  PosRec p;
-  p.x = inh ? 1e30f : (float)(d.px[i] - (double)cx * (g.dim[0] / g.nc[0]));
-  p.y = inh ? 1e30f : (float)(d.py[i] - (double)cy * (g.dim[1] / g.nc[1]));
-  p.z = inh ? 1e30f : (float)(d.pz[i] - (double)cz * (g.dim[2] / g.nc[2]));
-  p.hg2m = inh ? -1.f : pretest_radius2(search_radius2(d.h[i]), pos_eps);
-  d.pos[i] = p;
+  if (i < hi) write_pos_rec(d, g, pos_eps, i);
 }
 
 // Bounding box (non-inhibited particles) and largest search radius of each
@@ -87,76 +77,24 @@ __global__ __launch_bounds__(kBoundsBlock) void k_chunk_bounds(
   }
 }
 
-__global__ void k_build_dens_rec(DevParts d, int lo, int hi) {
+// Records of particles [lo, hi); with max_active_bin >= 0 only of the
+// active ones (the others keep theirs).
+__global__ void k_build_dens_rec(DevParts d, int lo, int hi, int mab) {
   const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi) return;
-  const bool inh = d.tb[i] == kBinInhibited;
-  DensRec r;
-  r.x = inh ? kFar : d.px[i];
-  r.y = inh ? kFar : d.py[i];
-  r.z = inh ? kFar : d.pz[i];
-  r.vx = d.vx[i];
-  r.vy = d.vy[i];
-  r.vz = d.vz[i];
-  r.m = d.m[i];
-  r.pad[0] = r.pad[1] = 0.f;
-  d.rd[i] = r;
+  if (i >= hi || (mab >= 0 && !is_active(d.tb[i], mab))) return;
+  write_dens_rec(d, i);
 }
 
-__global__ void k_build_grad_rec(DevParts d, int lo, int hi) {
+__global__ void k_build_grad_rec(DevParts d, int lo, int hi, int mab) {
   const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi) return;
-  const bool inh = d.tb[i] == kBinInhibited;
-  GradRec r;
-  r.x = inh ? kFar : d.px[i];
-  r.y = inh ? kFar : d.py[i];
-  r.z = inh ? kFar : d.pz[i];
-  r.vx = d.vx[i];
-  r.vy = d.vy[i];
-  r.vz = d.vz[i];
-  const float rho = d.rho[i];
-  r.mrho = rho > 0.f ? d.m[i] / rho : 0.f;
-  r.u = d.u[i];
-  r.c = d.c[i];
-  r.alpha = d.alpha[i];
-  r.pad0 = 0.f;
-  for (int k = 0; k < 6; k++) r.pad[k] = 0.f;
-  d.rg[i] = r;
+  if (i >= hi || (mab >= 0 && !is_active(d.tb[i], mab))) return;
+  write_grad_rec(d, i);
 }
 
-__global__ void k_build_force_rec(DevParts d, int lo, int hi) {
+__global__ void k_build_force_rec(DevParts d, int lo, int hi, int mab) {
   const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi) return;
-  const int8_t tb = d.tb[i];
-  const bool inh = tb == kBinInhibited;
-  ForceRec r;
-  r.x = inh ? kFar : d.px[i];
-  r.y = inh ? kFar : d.py[i];
-  r.z = inh ? kFar : d.pz[i];
-  const float h = d.h[i];
-  r.hg2 = inh ? -1.f : search_radius2(h);
-  r.vx = d.vx[i];
-  r.vy = d.vy[i];
-  r.vz = d.vz[i];
-  r.m = d.m[i];
-  r.inv_m = 1.f / r.m;
-  r.h_inv = 1.f / h;
-  const float hi2 = r.h_inv * r.h_inv;
-  r.h_inv4 = hi2 * hi2;
-  r.rho = d.rho[i];
-  r.inv_rho = 1.f / r.rho;
-  r.P = d.P[i];
-  r.pr2 = r.P / (r.rho * r.rho);
-  r.f = d.f[i];
-  r.bal = d.bal[i];
-  r.c = d.c[i];
-  r.alpha = d.alpha[i];
-  r.Pad = r.P * d.adiff[i];
-  r.u = d.u[i];
-  r.tb = (float)tb;
-  r.pad0 = 0.f;
-  r.pad[0] = r.pad[1] = 0.f;
-  d.rf[i] = r;
+  if (i >= hi || (mab >= 0 && !is_active(d.tb[i], mab))) return;
+  write_force_rec(d, i);
 }
 
 // ------------------------------------------------------ per-particle work --
@@ -894,18 +832,19 @@ hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
 }
 
 hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
-                                int hi, hipStream_t st) {
+                                int hi, hipStream_t st, int max_active_bin) {
   if (hi <= lo) return hipSuccess;
   const dim3 g(nblk(hi - lo, 256)), b(256);
+  const int m = max_active_bin;
   switch (kind) {
     case LoopKind::Density:
-      hipLaunchKernelGGL(k_build_dens_rec, g, b, 0, st, d, lo, hi);
+      hipLaunchKernelGGL(k_build_dens_rec, g, b, 0, st, d, lo, hi, m);
       break;
     case LoopKind::Gradient:
-      hipLaunchKernelGGL(k_build_grad_rec, g, b, 0, st, d, lo, hi);
+      hipLaunchKernelGGL(k_build_grad_rec, g, b, 0, st, d, lo, hi, m);
       break;
     case LoopKind::Force:
-      hipLaunchKernelGGL(k_build_force_rec, g, b, 0, st, d, lo, hi);
+      hipLaunchKernelGGL(k_build_force_rec, g, b, 0, st, d, lo, hi, m);
       break;
   }
   return hipGetLastError();

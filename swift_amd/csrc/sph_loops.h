@@ -27,6 +27,92 @@ __device__ __forceinline__ bool is_active(int8_t tb, int max_active_bin) {
 // Position of an inhibited particle in its records: every test rejects it.
 constexpr double kFar = 1.0e30;
 
+// ------------------------------------------------------ neighbour records --
+// The fields a neighbour loop reads from particle j, written from the
+// particle arrays (rebuilt after the stage that changes them).
+__device__ __forceinline__ void write_dens_rec(const DevParts &d, int i) {
+  const bool inh = d.tb[i] == kBinInhibited;
+  DensRec r;
+  r.x = inh ? kFar : d.px[i];
+  r.y = inh ? kFar : d.py[i];
+  r.z = inh ? kFar : d.pz[i];
+  r.vx = d.vx[i];
+  r.vy = d.vy[i];
+  r.vz = d.vz[i];
+  r.m = d.m[i];
+  r.pad[0] = r.pad[1] = 0.f;
+  d.rd[i] = r;
+}
+
+__device__ __forceinline__ void write_grad_rec(const DevParts &d, int i) {
+  const bool inh = d.tb[i] == kBinInhibited;
+  GradRec r;
+  r.x = inh ? kFar : d.px[i];
+  r.y = inh ? kFar : d.py[i];
+  r.z = inh ? kFar : d.pz[i];
+  r.vx = d.vx[i];
+  r.vy = d.vy[i];
+  r.vz = d.vz[i];
+  const float rho = d.rho[i];
+  r.mrho = rho > 0.f ? d.m[i] / rho : 0.f;
+  r.u = d.u[i];
+  r.c = d.c[i];
+  r.alpha = d.alpha[i];
+  r.pad0 = 0.f;
+  for (int k = 0; k < 6; k++) r.pad[k] = 0.f;
+  d.rg[i] = r;
+}
+
+__device__ __forceinline__ void write_force_rec(const DevParts &d, int i) {
+  const int8_t tb = d.tb[i];
+  const bool inh = tb == kBinInhibited;
+  ForceRec r;
+  r.x = inh ? kFar : d.px[i];
+  r.y = inh ? kFar : d.py[i];
+  r.z = inh ? kFar : d.pz[i];
+  const float h = d.h[i];
+  r.hg2 = inh ? -1.f : search_radius2(h);
+  r.vx = d.vx[i];
+  r.vy = d.vy[i];
+  r.vz = d.vz[i];
+  r.m = d.m[i];
+  r.inv_m = 1.f / r.m;
+  r.h_inv = 1.f / h;
+  const float hi2 = r.h_inv * r.h_inv;
+  r.h_inv4 = hi2 * hi2;
+  r.rho = d.rho[i];
+  r.inv_rho = 1.f / r.rho;
+  r.P = d.P[i];
+  r.pr2 = r.P / (r.rho * r.rho);
+  r.f = d.f[i];
+  r.bal = d.bal[i];
+  r.c = d.c[i];
+  r.alpha = d.alpha[i];
+  r.Pad = r.P * d.adiff[i];
+  r.u = d.u[i];
+  r.tb = (float)tb;
+  r.pad0 = 0.f;
+  r.pad[0] = r.pad[1] = 0.f;
+  d.rf[i] = r;
+}
+
+// Candidate record of the FP32 pre-test: position relative to the corner of
+// the particle's cell and the widened search radius.
+__device__ __forceinline__ void write_pos_rec(const DevParts &d,
+                                              const GridDesc &g, float pos_eps,
+                                              int i) {
+  const bool inh = d.tb[i] == kBinInhibited;
+  const int c = d.cell[i];
+  const int cz = c % g.nc[2], cy = (c / g.nc[2]) % g.nc[1];
+  const int cx = c / (g.nc[2] * g.nc[1]);
+  PosRec p;
+  p.x = inh ? 1e30f : (float)(d.px[i] - (double)cx * (g.dim[0] / g.nc[0]));
+  p.y = inh ? 1e30f : (float)(d.py[i] - (double)cy * (g.dim[1] / g.nc[1]));
+  p.z = inh ? 1e30f : (float)(d.pz[i] - (double)cz * (g.dim[2] / g.nc[2]));
+  p.hg2m = inh ? -1.f : pretest_radius2(search_radius2(d.h[i]), pos_eps);
+  d.pos[i] = p;
+}
+
 // ---------------------------------------------------------------- density --
 struct DensAcc {
   float rho, rho_dh, wc, wc_dh, divv, rx, ry, rz;

@@ -24,20 +24,28 @@
 // Floating-point expressions keep the reference's operand order and types
 // (float fields, double time intervals); contraction is off so that each
 // operation rounds as written.
+#include <algorithm>
+
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include "sph_device.h"
 #include "sph_kernels.h"
+#include "sph_loops.h"
 
 using namespace sph;
+using namespace sphl;
 
 namespace {
 
 constexpr int kBlock = 256;
+// Grid cap of the kernels that reduce into one counter: a grid-stride loop
+// over the particles, one atomic per block.
+constexpr int kReduceBlocks = 4096;
 constexpr long long kMaxNrTimesteps = 1LL << (kNumTimeBins + 1);
 constexpr int kNeighbourMaxDeltaBin = 2;  // time_bin_neighbour_max_delta_bin
 
 int blocks(int n) { return (n + kBlock - 1) / kBlock; }
+int reduce_blocks(int n) { return std::max(1, std::min(blocks(n), kReduceBlocks)); }
 
 // ---------------------------------------------------------------- binning --
 __device__ __forceinline__ unsigned long long spread10(unsigned int v) {
@@ -184,65 +192,76 @@ __device__ __forceinline__ long long wave_min_ll(long long v) {
   return v;
 }
 
-// drift_part + hydro_predict_extra for the particles [lo, hi), and the
-// largest distance by which a drifted particle lies outside its cell's box
-// (max-reduced into *dmax as float bits; the value is non-negative).
+// drift_part + hydro_predict_extra for the particles [lo, hi), followed by
+// their neighbour records and candidate record (every field a record holds
+// may have changed), and the largest distance by which a drifted particle
+// lies outside its cell's box (max-reduced into *dmax as float bits; the
+// value is non-negative).
+__device__ __forceinline__ float drift_one(const DevParts &d, const DriftArgs &a,
+                                           int i) {
+#pragma clang fp contract(off)
+  const double dt = a.dt_drift;
+  const double x = d.px[i] + (double)d.vfx[i] * dt;
+  const double y = d.py[i] + (double)d.vfy[i] * dt;
+  const double z = d.pz[i] + (double)d.vfz[i] * dt;
+  d.px[i] = x;
+  d.py[i] = y;
+  d.pz[i] = z;
+  // predicted velocities (hydro terms)
+  d.vx[i] = (float)((double)d.vx[i] + (double)d.ax[i] * a.dt_kick_hydro);
+  d.vy[i] = (float)((double)d.vy[i] + (double)d.ay[i] * a.dt_kick_hydro);
+  d.vz[i] = (float)((double)d.vz[i] + (double)d.az[i] * a.dt_kick_hydro);
+  // hydro_predict_extra takes its intervals as float
+  const float dt_drift = (float)a.dt_drift;
+  const float dt_therm = (float)a.dt_therm;
+  float u = d.u[i] + d.udt[i] * dt_therm;
+  float h = d.h[i];
+  const float h_inv = 1.f / h;
+  const float w1 = d.hdt[i] * h_inv * dt_drift;
+  h *= fabsf(w1) < 0.2f ? approx_expf(w1) : expf(w1);
+  const float w2 = -kDim * w1;
+  float rho = d.rho[i];
+  rho *= fabsf(w2) < 0.2f ? approx_expf(w2) : expf(w2);
+  // entropy floor: none (floor u = 0); absolute minimum
+  u = fmaxf(u, 0.f);
+  u = fmaxf(u, a.min_u);
+  const float P = gas_pressure(rho, u);
+  const float c = gas_soundspeed(rho, P);
+  d.u[i] = u;
+  d.h[i] = h;
+  d.rho[i] = rho;
+  d.P[i] = P;
+  d.c[i] = c;
+  d.vsig[i] = fmaxf(d.vsig[i], 2.f * c);
+  write_dens_rec(d, i);
+  write_grad_rec(d, i);
+  write_force_rec(d, i);
+  write_pos_rec(d, a.g, a.pos_eps, i);
+  // distance outside the box of the particle's cell
+  const int cid = d.cell[i];
+  const int cz = cid % a.g.nc[2], cy = (cid / a.g.nc[2]) % a.g.nc[1];
+  const int cx = cid / (a.g.nc[2] * a.g.nc[1]);
+  const int cc[3] = {cx, cy, cz};
+  const double p[3] = {x, y, z};
+  double o = 0.0;
+  for (int k = 0; k < 3; k++) {
+    const double w = a.g.dim[k] / a.g.nc[k];
+    const double lo = (double)cc[k] * w;
+    o = fmax(o, fmax(lo - p[k], p[k] - (lo + w)));
+  }
+  // rounded up so that the bound stays a bound
+  float out = (float)o;
+  if ((double)out < o) out = nextafterf(out, 1e30f);
+  return out;
+}
+
 __global__ __launch_bounds__(kBlock) void k_drift(DevParts d, DriftArgs a,
                                                   unsigned int *dmax) {
-#pragma clang fp contract(off)
   __shared__ float red[kBlock / 64];
-  const int i = a.lo + blockIdx.x * blockDim.x + threadIdx.x;
   float out = 0.f;
-  if (i < a.hi && d.tb[i] != kBinInhibited) {
-    const double dt = a.dt_drift;
-    const double x = d.px[i] + (double)d.vfx[i] * dt;
-    const double y = d.py[i] + (double)d.vfy[i] * dt;
-    const double z = d.pz[i] + (double)d.vfz[i] * dt;
-    d.px[i] = x;
-    d.py[i] = y;
-    d.pz[i] = z;
-    // predicted velocities (hydro terms)
-    d.vx[i] = (float)((double)d.vx[i] + (double)d.ax[i] * a.dt_kick_hydro);
-    d.vy[i] = (float)((double)d.vy[i] + (double)d.ay[i] * a.dt_kick_hydro);
-    d.vz[i] = (float)((double)d.vz[i] + (double)d.az[i] * a.dt_kick_hydro);
-    // hydro_predict_extra takes its intervals as float
-    const float dt_drift = (float)a.dt_drift;
-    const float dt_therm = (float)a.dt_therm;
-    float u = d.u[i] + d.udt[i] * dt_therm;
-    float h = d.h[i];
-    const float h_inv = 1.f / h;
-    const float w1 = d.hdt[i] * h_inv * dt_drift;
-    h *= fabsf(w1) < 0.2f ? approx_expf(w1) : expf(w1);
-    const float w2 = -kDim * w1;
-    float rho = d.rho[i];
-    rho *= fabsf(w2) < 0.2f ? approx_expf(w2) : expf(w2);
-    // entropy floor: none (floor u = 0); absolute minimum
-    u = fmaxf(u, 0.f);
-    u = fmaxf(u, a.min_u);
-    const float P = gas_pressure(rho, u);
-    const float c = gas_soundspeed(rho, P);
-    d.u[i] = u;
-    d.h[i] = h;
-    d.rho[i] = rho;
-    d.P[i] = P;
-    d.c[i] = c;
-    d.vsig[i] = fmaxf(d.vsig[i], 2.f * c);
-    // distance outside the box of the particle's cell
-    const int cid = d.cell[i];
-    const int cz = cid % a.g.nc[2], cy = (cid / a.g.nc[2]) % a.g.nc[1];
-    const int cx = cid / (a.g.nc[2] * a.g.nc[1]);
-    const int cc[3] = {cx, cy, cz};
-    const double p[3] = {x, y, z};
-    double o = 0.0;
-    for (int k = 0; k < 3; k++) {
-      const double w = a.g.dim[k] / a.g.nc[k];
-      const double lo = (double)cc[k] * w;
-      o = fmax(o, fmax(lo - p[k], p[k] - (lo + w)));
-    }
-    // rounded up so that the bound stays a bound
-    out = (float)o;
-    if ((double)out < o) out = nextafterf(out, 1e30f);
-  }
+  for (int i = a.lo + blockIdx.x * blockDim.x + threadIdx.x; i < a.hi;
+       i += gridDim.x * blockDim.x)
+    if (d.tb[i] != kBinInhibited) out = fmaxf(out, drift_one(d, a, i));
   out = wave_max_u(out);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) red[w] = out;
@@ -333,15 +352,14 @@ __device__ long long make_integer_timestep(float new_dt, int old_bin,
 // smallest end of step over all particles, min-reduced into *ti_end_min.
 // err[0] counts particles whose step falls below dt_min or to zero, err[1]
 // the active particles.
-__global__ __launch_bounds__(kBlock) void k_timestep(DevParts d, StepArgs a,
-                                                     unsigned long long *ti_end_min,
-                                                     unsigned int *err) {
+__device__ __forceinline__ long long timestep_one(const DevParts &d,
+                                                  const StepArgs &a, int i,
+                                                  unsigned int *err,
+                                                  bool &active) {
 #pragma clang fp contract(off)
-  __shared__ long long red[kBlock / 64];
-  const int i = a.lo + blockIdx.x * blockDim.x + threadIdx.x;
   long long ti_end = kMaxNrTimesteps;
-  bool active = false;
-  if (i < a.hi && d.tb[i] != kBinInhibited) {
+  active = false;
+  if (d.tb[i] != kBinInhibited) {
     const int tb = d.tb[i];
     active = tb <= a.max_active_bin;
     if (active) {
@@ -383,10 +401,34 @@ __global__ __launch_bounds__(kBlock) void k_timestep(DevParts d, StepArgs a,
       ti_end = time_end(a.ti_current, tb);
     }
   }
+  return ti_end;
+}
+
+__global__ __launch_bounds__(kBlock) void k_timestep(DevParts d, StepArgs a,
+                                                     unsigned long long *ti_end_min,
+                                                     unsigned int *err) {
+  __shared__ long long red[kBlock / 64];
+  long long ti_end = kMaxNrTimesteps;
+  int nloc = 0;
+  for (int i = a.lo + blockIdx.x * blockDim.x + threadIdx.x; i < a.hi;
+       i += gridDim.x * blockDim.x) {
+    bool act;
+    const long long t = timestep_one(d, a, i, err, act);
+    ti_end = t < ti_end ? t : ti_end;
+    nloc += act ? 1 : 0;
+  }
   ti_end = wave_min_ll(ti_end);
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) nloc += __shfl_xor(nloc, off);
+  __shared__ int rn[kBlock / 64];
   const int w = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) red[w] = ti_end;
-  const int nact = __syncthreads_count(active);
+  if ((threadIdx.x & 63) == 0) {
+    red[w] = ti_end;
+    rn[w] = nloc;
+  }
+  __syncthreads();
+  int nact = 0;
+  for (int k = 0; k < kBlock / 64; k++) nact += rn[k];
   if (threadIdx.x == 0) {
     long long m = red[0];
     for (int k = 1; k < kBlock / 64; k++) m = red[k] < m ? red[k] : m;
@@ -456,7 +498,7 @@ hipError_t launch_fill_i8(int8_t *p, int8_t v, int n, hipStream_t st) {
 hipError_t launch_drift(const DevParts &d, const DriftArgs &a,
                         unsigned int *dmax, hipStream_t st) {
   if (a.hi <= a.lo) return hipSuccess;
-  k_drift<<<blocks(a.hi - a.lo), kBlock, 0, st>>>(d, a, dmax);
+  k_drift<<<reduce_blocks(a.hi - a.lo), kBlock, 0, st>>>(d, a, dmax);
   return hipGetLastError();
 }
 
@@ -470,7 +512,8 @@ hipError_t launch_timestep(const DevParts &d, const StepArgs &a,
                            unsigned long long *ti_end_min, unsigned int *err,
                            hipStream_t st) {
   if (a.hi <= a.lo) return hipSuccess;
-  k_timestep<<<blocks(a.hi - a.lo), kBlock, 0, st>>>(d, a, ti_end_min, err);
+  k_timestep<<<reduce_blocks(a.hi - a.lo), kBlock, 0, st>>>(d, a, ti_end_min,
+                                                          err);
   return hipGetLastError();
 }
 
