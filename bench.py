@@ -236,7 +236,8 @@ def _hbm_used_bytes() -> int:
     return int(total.value - free.value)
 
 
-def evolve_bench(n_side: int, ncell: int, nsteps: int, seed: int = 0) -> dict:
+def evolve_bench(n_side: int, ncell: int, nsteps: int, seed: int = 0,
+                 chunk: int = 0) -> dict:
     """Multi-step run of the clustered construction: forces at t = 0 for all
     particles (the initial h estimate converged by the ghost), then `nsteps`
     drift-kick steps on a fixed 4-level time-bin hierarchy (bins 1..4 hold
@@ -262,6 +263,9 @@ def evolve_bench(n_side: int, ncell: int, nsteps: int, seed: int = 0) -> dict:
     t_up = time.perf_counter() - ts
     del st
     eng.set_integration(integration(fixed_bins=True))
+    if chunk:
+        for loop in ("density", "gradient", "force"):
+            eng.set_chunk(loop, chunk)
     ts = time.perf_counter()
     first = eng.step()
     t_first = time.perf_counter() - ts
@@ -337,6 +341,9 @@ def main() -> int:
                          "(0: skip; 512 is the full-size case)")
     ap.add_argument("--evolve-ncell", type=int, default=96)
     ap.add_argument("--evolve-steps", type=int, default=20)
+    ap.add_argument("--evolve-chunk", type=int, default=0,
+                    help="chunk capacity of all loops in the multi-step run "
+                         "(0: the engine's choice)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -381,9 +388,9 @@ def main() -> int:
     value = args.steps * inter / total
 
     # Per-loop kernel times: HIP events recorded on the engine stream around
-    # every neighbour-loop launch (tile build included) of the same number of
-    # eager passes with the same kernels (a captured graph carries no
-    # per-launch events).
+    # every neighbour-loop kernel launch (the tile build excluded) of the
+    # same number of eager passes with the same kernels (a captured graph
+    # carries no per-launch events).
     eng.set_timing(True)
     t1 = time.perf_counter()
     for _ in range(args.steps):
@@ -501,7 +508,7 @@ def main() -> int:
 
     if args.evolve_n > 0:
         out["evolve"] = evolve_bench(args.evolve_n, args.evolve_ncell,
-                                     args.evolve_steps)
+                                     args.evolve_steps, chunk=args.evolve_chunk)
         print(f"bench: evolve {out['evolve']['ms_per_step']:.1f} ms/step",
               file=sys.stderr, flush=True)
 

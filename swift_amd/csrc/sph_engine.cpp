@@ -150,6 +150,7 @@ struct swsph_engine {
   unsigned int *h_small = nullptr;        // pinned readback: dmax, hg2
   unsigned long long *h_tiend = nullptr;  // pinned [1]
   unsigned int *h_err = nullptr;          // pinned [2]
+  int *h_ntiles = nullptr;                // pinned [1]
   float dmax = 0.f;
   // The neighbour records of every particle match its fields (the passes
   // then rebuild only those of the particles they update), and the
@@ -572,8 +573,6 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
   const int C = build ? kListChunk : e->loop_c[(int)k];
   const swsph_engine::ChunkTable *t = chunk_table(e, C, &rc);
   if (!t) return rc;
-  hipEvent_t a;
-  if ((rc = time_begin(e, &a))) return rc;
   const LoopArgs la = loop_args(e);
   // The all-active table is kept for the later loops of the pass.
   if (!(mode == 0 && e->tiles_c == C)) {
@@ -583,10 +582,22 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
   } else if (e->schedule == 2) {
     HIPCHK(sphk::launch_zero_u32((unsigned int *)e->ts.heads, 8 * 64, e->st));
   }
-  const int max_tiles = (int)(e->n / C) + e->ncells + 1;
-  HIPCHK(sphk::launch_queue(k, build, e->d, la, chunk_args(t), tile_args(e),
-                            e->nl, C, e->schedule == 2, e->queue_waves,
-                            max_tiles, e->st));
+  int max_tiles = (int)(e->n / C) + e->ncells + 1;
+  if (!e->capturing && e->schedule == 1) {
+    // Eager: one wave per tile of the device-side count (a grid for the
+    // largest possible count would mostly be waves that exit at once).
+    HIPCHK(hipMemcpyAsync(e->h_ntiles, e->ts.ntiles, sizeof(int),
+                          hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    max_tiles = *e->h_ntiles;
+  }
+  // the loop kernel alone is timed (the tile build is not)
+  hipEvent_t a;
+  if ((rc = time_begin(e, &a))) return rc;
+  if (max_tiles > 0)
+    HIPCHK(sphk::launch_queue(k, build, e->d, la, chunk_args(t), tile_args(e),
+                              e->nl, C, e->schedule == 2, e->queue_waves,
+                              max_tiles, e->st));
   e->launches += 13;
   return time_end(e, k, a);
 }
@@ -1245,7 +1256,8 @@ int swsph_engine_create(const swsph_props *props, swsph_engine **out) {
       hipHostMalloc((void **)&e->h_tiend, sizeof(unsigned long long), 0) !=
           hipSuccess ||
       hipHostMalloc((void **)&e->h_err, 2 * sizeof(unsigned int), 0) !=
-          hipSuccess) {
+          hipSuccess ||
+      hipHostMalloc((void **)&e->h_ntiles, sizeof(int), 0) != hipSuccess) {
     delete e;
     return fail(2, "allocation of engine counters failed");
   }
@@ -1277,6 +1289,7 @@ int swsph_engine_destroy(swsph_engine *e) {
   if (e->h_small) (void)hipHostFree(e->h_small);
   if (e->h_tiend) (void)hipHostFree(e->h_tiend);
   if (e->h_err) (void)hipHostFree(e->h_err);
+  if (e->h_ntiles) (void)hipHostFree(e->h_ntiles);
   if (e->st) (void)hipStreamDestroy(e->st);
   delete e;
   return 0;
