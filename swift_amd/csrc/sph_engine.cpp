@@ -973,19 +973,20 @@ int drift_cells(swsph_engine *e, int ci, int64_t ti_old, int64_t ti_current) {
   a.dt_kick_hydro = a.dt_drift;
   a.dt_therm = a.dt_drift;
   a.min_u = e->integ.min_u;
-  a.pos_eps = pos_eps(e);
   a.g = e->grid;
   a.lo = lo;
   a.hi = hi;
-  // the drift rewrites the records of the particles it moves
   HIPCHK(sphk::launch_drift(e->d, a, e->d_dmax, e->st));
-  e->launches++;
+  // every field a record holds may have changed
+  for (LoopKind k : {LoopKind::Density, LoopKind::Gradient, LoopKind::Force})
+    HIPCHK(sphk::launch_build_records(k, e->d, lo, hi, e->st));
+  e->launches += 4;
   e->fields_epoch++;
   if (ci == -1) {
     e->recs_ok = true;
     e->stale_mab = swsph_engine::kNoMab;
   }
-  return refresh_bounds(e, c_lo, c_hi, false);
+  return refresh_bounds(e, c_lo, c_hi);
 }
 
 int kick_cells(swsph_engine *e, int ci, int64_t ti_current, int mode) {

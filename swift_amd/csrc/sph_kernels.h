@@ -166,7 +166,6 @@ enum class LoopKind { Density, Gradient, Force };
 struct DriftArgs {
   double dt_drift, dt_kick_hydro, dt_therm;
   float min_u;    // hydro_props minimal_internal_energy
-  float pos_eps;  // of the candidate records (LoopArgs::pos_eps)
   GridDesc g;
   int lo, hi;
 };
@@ -291,8 +290,6 @@ template <class T>
 hipError_t launch_gather(T *dst, const T *src, int stride, int comp,
                          const int *idx, int n, hipStream_t st);
 hipError_t launch_fill_i8(int8_t *p, int8_t v, int n, hipStream_t st);
-// Drift of [lo, hi), rewriting the neighbour and candidate records of the
-// drifted particles.
 hipError_t launch_drift(const DevParts &d, const DriftArgs &a,
                         unsigned int *dmax, hipStream_t st);
 hipError_t launch_kick(const DevParts &d, const KickArgs &a, hipStream_t st);

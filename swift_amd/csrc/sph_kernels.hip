@@ -25,7 +25,7 @@ namespace {
 __global__ void k_build_pos(DevParts d, GridDesc g, float pos_eps, int lo,
                             int hi) {
   const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < hi) write_pos_rec(d, g, pos_eps, i);
+  if (i < hi) d.pos[i] = pos_rec(d, g, pos_eps, i);
 }
 
 // Bounding box (non-inhibited particles) and largest search radius of each
@@ -78,23 +78,61 @@ __global__ __launch_bounds__(kBoundsBlock) void k_chunk_bounds(
 }
 
 // Records of particles [lo, hi); with max_active_bin >= 0 only of the
-// active ones (the others keep theirs).
-__global__ void k_build_dens_rec(DevParts d, int lo, int hi, int mab) {
-  const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi || (mab >= 0 && !is_active(d.tb[i], mab))) return;
-  write_dens_rec(d, i);
+// active ones (the others keep theirs), stored one by one.  For all
+// particles of the range, each wave stages its 64 records in LDS and writes
+// them out as contiguous 16-byte words (a record per lane would scatter
+// every store instruction over the wave's whole span).
+constexpr int kRecBlock = 256;
+
+template <class R>
+__device__ __forceinline__ void store_wave_records(R *dst, int i0, int cnt,
+                                                   const R &r, uint4 *lds) {
+  constexpr int Q = (int)(sizeof(R) / 16);
+  static_assert(sizeof(R) % 16 == 0, "records are whole 16-byte words");
+  const int lane = threadIdx.x & 63;
+  const uint4 *src = reinterpret_cast<const uint4 *>(&r);
+#pragma unroll
+  for (int q = 0; q < Q; q++) lds[lane * Q + q] = src[q];
+  __syncthreads();
+  uint4 *out = reinterpret_cast<uint4 *>(dst + i0);
+  const int words = cnt * Q;
+  for (int k = lane; k < words; k += 64) out[k] = lds[k];
 }
 
-__global__ void k_build_grad_rec(DevParts d, int lo, int hi, int mab) {
-  const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi || (mab >= 0 && !is_active(d.tb[i], mab))) return;
-  write_grad_rec(d, i);
+template <class R, class Make>
+__device__ __forceinline__ void build_records(const DevParts &d, R *dst,
+                                              int lo, int hi, int mab,
+                                              Make make) {
+  const int i = lo + blockIdx.x * kRecBlock + threadIdx.x;
+  if (mab >= 0) {  // uniform
+    if (i < hi && is_active(d.tb[i], mab)) dst[i] = make(i);
+    return;
+  }
+  __shared__ uint4 buf[kRecBlock * sizeof(R) / 16];
+  R r{};
+  if (i < hi) r = make(i);
+  const int w = threadIdx.x >> 6;
+  const int i0 = lo + blockIdx.x * kRecBlock + (w << 6);
+  store_wave_records(dst, i0, max(0, min(64, hi - i0)), r,
+                     buf + w * 64 * (int)(sizeof(R) / 16));
 }
 
-__global__ void k_build_force_rec(DevParts d, int lo, int hi, int mab) {
-  const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi || (mab >= 0 && !is_active(d.tb[i], mab))) return;
-  write_force_rec(d, i);
+__global__ __launch_bounds__(kRecBlock) void k_build_dens_rec(DevParts d,
+                                                              int lo, int hi,
+                                                              int mab) {
+  build_records(d, d.rd, lo, hi, mab, [&](int i) { return dens_rec(d, i); });
+}
+
+__global__ __launch_bounds__(kRecBlock) void k_build_grad_rec(DevParts d,
+                                                              int lo, int hi,
+                                                              int mab) {
+  build_records(d, d.rg, lo, hi, mab, [&](int i) { return grad_rec(d, i); });
+}
+
+__global__ __launch_bounds__(kRecBlock) void k_build_force_rec(DevParts d,
+                                                               int lo, int hi,
+                                                               int mab) {
+  build_records(d, d.rf, lo, hi, mab, [&](int i) { return force_rec(d, i); });
 }
 
 // ------------------------------------------------------ per-particle work --
@@ -834,7 +872,7 @@ hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
 hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
                                 int hi, hipStream_t st, int max_active_bin) {
   if (hi <= lo) return hipSuccess;
-  const dim3 g(nblk(hi - lo, 256)), b(256);
+  const dim3 g(nblk(hi - lo, kRecBlock)), b(kRecBlock);
   const int m = max_active_bin;
   switch (kind) {
     case LoopKind::Density:

@@ -28,9 +28,9 @@ __device__ __forceinline__ bool is_active(int8_t tb, int max_active_bin) {
 constexpr double kFar = 1.0e30;
 
 // ------------------------------------------------------ neighbour records --
-// The fields a neighbour loop reads from particle j, written from the
-// particle arrays (rebuilt after the stage that changes them).
-__device__ __forceinline__ void write_dens_rec(const DevParts &d, int i) {
+// The fields a neighbour loop reads from particle j, from the particle
+// arrays (rebuilt after the stage that changes them).
+__device__ __forceinline__ DensRec dens_rec(const DevParts &d, int i) {
   const bool inh = d.tb[i] == kBinInhibited;
   DensRec r;
   r.x = inh ? kFar : d.px[i];
@@ -41,10 +41,10 @@ __device__ __forceinline__ void write_dens_rec(const DevParts &d, int i) {
   r.vz = d.vz[i];
   r.m = d.m[i];
   r.pad[0] = r.pad[1] = 0.f;
-  d.rd[i] = r;
+  return r;
 }
 
-__device__ __forceinline__ void write_grad_rec(const DevParts &d, int i) {
+__device__ __forceinline__ GradRec grad_rec(const DevParts &d, int i) {
   const bool inh = d.tb[i] == kBinInhibited;
   GradRec r;
   r.x = inh ? kFar : d.px[i];
@@ -60,10 +60,10 @@ __device__ __forceinline__ void write_grad_rec(const DevParts &d, int i) {
   r.alpha = d.alpha[i];
   r.pad0 = 0.f;
   for (int k = 0; k < 6; k++) r.pad[k] = 0.f;
-  d.rg[i] = r;
+  return r;
 }
 
-__device__ __forceinline__ void write_force_rec(const DevParts &d, int i) {
+__device__ __forceinline__ ForceRec force_rec(const DevParts &d, int i) {
   const int8_t tb = d.tb[i];
   const bool inh = tb == kBinInhibited;
   ForceRec r;
@@ -93,14 +93,14 @@ __device__ __forceinline__ void write_force_rec(const DevParts &d, int i) {
   r.tb = (float)tb;
   r.pad0 = 0.f;
   r.pad[0] = r.pad[1] = 0.f;
-  d.rf[i] = r;
+  return r;
 }
 
 // Candidate record of the FP32 pre-test: position relative to the corner of
 // the particle's cell and the widened search radius.
-__device__ __forceinline__ void write_pos_rec(const DevParts &d,
-                                              const GridDesc &g, float pos_eps,
-                                              int i) {
+__device__ __forceinline__ PosRec pos_rec(const DevParts &d,
+                                          const GridDesc &g, float pos_eps,
+                                          int i) {
   const bool inh = d.tb[i] == kBinInhibited;
   const int c = d.cell[i];
   const int cz = c % g.nc[2], cy = (c / g.nc[2]) % g.nc[1];
@@ -110,7 +110,7 @@ __device__ __forceinline__ void write_pos_rec(const DevParts &d,
   p.y = inh ? 1e30f : (float)(d.py[i] - (double)cy * (g.dim[1] / g.nc[1]));
   p.z = inh ? 1e30f : (float)(d.pz[i] - (double)cz * (g.dim[2] / g.nc[2]));
   p.hg2m = inh ? -1.f : pretest_radius2(search_radius2(d.h[i]), pos_eps);
-  d.pos[i] = p;
+  return p;
 }
 
 // ---------------------------------------------------------------- density --

@@ -192,11 +192,9 @@ __device__ __forceinline__ long long wave_min_ll(long long v) {
   return v;
 }
 
-// drift_part + hydro_predict_extra for the particles [lo, hi), followed by
-// their neighbour records and candidate record (every field a record holds
-// may have changed), and the largest distance by which a drifted particle
-// lies outside its cell's box (max-reduced into *dmax as float bits; the
-// value is non-negative).
+// drift_part + hydro_predict_extra for the particles [lo, hi), and the
+// largest distance by which a drifted particle lies outside its cell's box
+// (max-reduced into *dmax as float bits; the value is non-negative).
 __device__ __forceinline__ float drift_one(const DevParts &d, const DriftArgs &a,
                                            int i) {
 #pragma clang fp contract(off)
@@ -233,10 +231,6 @@ __device__ __forceinline__ float drift_one(const DevParts &d, const DriftArgs &a
   d.P[i] = P;
   d.c[i] = c;
   d.vsig[i] = fmaxf(d.vsig[i], 2.f * c);
-  write_dens_rec(d, i);
-  write_grad_rec(d, i);
-  write_force_rec(d, i);
-  write_pos_rec(d, a.g, a.pos_eps, i);
   // distance outside the box of the particle's cell
   const int cid = d.cell[i];
   const int cz = cid % a.g.nc[2], cy = (cid / a.g.nc[2]) % a.g.nc[1];
@@ -258,10 +252,9 @@ __device__ __forceinline__ float drift_one(const DevParts &d, const DriftArgs &a
 __global__ __launch_bounds__(kBlock) void k_drift(DevParts d, DriftArgs a,
                                                   unsigned int *dmax) {
   __shared__ float red[kBlock / 64];
+  const int i = a.lo + blockIdx.x * blockDim.x + threadIdx.x;
   float out = 0.f;
-  for (int i = a.lo + blockIdx.x * blockDim.x + threadIdx.x; i < a.hi;
-       i += gridDim.x * blockDim.x)
-    if (d.tb[i] != kBinInhibited) out = fmaxf(out, drift_one(d, a, i));
+  if (i < a.hi && d.tb[i] != kBinInhibited) out = drift_one(d, a, i);
   out = wave_max_u(out);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) red[w] = out;
@@ -498,7 +491,7 @@ hipError_t launch_fill_i8(int8_t *p, int8_t v, int n, hipStream_t st) {
 hipError_t launch_drift(const DevParts &d, const DriftArgs &a,
                         unsigned int *dmax, hipStream_t st) {
   if (a.hi <= a.lo) return hipSuccess;
-  k_drift<<<reduce_blocks(a.hi - a.lo), kBlock, 0, st>>>(d, a, dmax);
+  k_drift<<<blocks(a.hi - a.lo), kBlock, 0, st>>>(d, a, dmax);
   return hipGetLastError();
 }
 
