@@ -372,9 +372,8 @@ __device__ __forceinline__ void run_chunk(
         const int js = uniform(sh.qj0[q]) + (int)(e & kRowMask) * C;
         const int nj = min(C, uniform(sh.qj1[q]) - js);
         // chunk slot of the lane's own particle (a particle is not its own
-        // neighbour); the check runs only on chunks holding i-particles
+        // neighbour); out of the chunk's range unless the chunk holds it
         const int selfl = uniform(sh.qcell[q]) == cell_i ? i - js : -1;
-        const bool selfin = __any(act && selfl >= 0 && selfl < nj);
         // i relative to the neighbour image's corner
         const float xs = (float)(st.x - sh.qorg[3 * q]);
         const float ys = (float)(st.y - sh.qorg[3 * q + 1]);
@@ -394,7 +393,7 @@ __device__ __forceinline__ void run_chunk(
           const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
           const bool pass =
               L::kSymmetric ? (r2 < rt2m || r2 < p.hg2m) : r2 < rt2m;
-          nib |= (pass && (!selfin || jl != selfl)) ? ((Nib)1 << rr) : (Nib)0;
+          nib |= (pass && jl != selfl) ? ((Nib)1 << rr) : (Nib)0;
         }
         m |= (unsigned long long)nib << nrow;
         nrow += RPC;
