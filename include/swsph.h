@@ -182,6 +182,13 @@ int swsph_restore_state(swsph_engine *e);
  * summation order. */
 int swsph_set_chunk(swsph_engine *e, int which, int C);
 int swsph_get_chunk(swsph_engine *e, int which, int *C);
+/* The same with the candidate chunks of the work-queue schedules (modes 1
+ * and 2 of swsph_set_schedule) sized apart from the tiles: tiles of CI
+ * i-particles against candidate chunks of CJ particles, CJ = CI or
+ * CJ = CI / 2 for CI = 16, 32 (finer candidate culling under the same tile).
+ * swsph_set_chunk(e, which, C) is swsph_set_tile_shape(e, which, C, C). */
+int swsph_set_tile_shape(swsph_engine *e, int which, int CI, int CJ);
+int swsph_get_tile_shape(swsph_engine *e, int which, int *CI, int *CJ);
 
 /* Schedule of the batched pass's neighbour loops.  mode 0: one wave per
  * chunk of every cell, launched as one grid (inactive chunks exit at once).
@@ -198,15 +205,18 @@ int swsph_set_schedule(swsph_engine *e, int mode, int waves);
  * swsph_set_schedule) with chunk capacities 8, 16 and 32 on the engine's
  * current particles (`repeats` timed passes each, HIP events per loop),
  * keeps the schedule with the smallest summed loop time and, under it, the
- * fastest capacity of each loop, and applies them.  The pass state (h and
- * the (io) fields) is saved before and restored after, so results are those
- * of an untuned engine.  No counterpart in the reference. */
+ * fastest capacity of each loop; under a work-queue schedule it then tries
+ * candidate chunks of half the tile size per loop, and applies the result.
+ * The pass state (h and the (io) fields) is saved before and restored
+ * after, so results are those of an untuned engine.  No counterpart in the
+ * reference. */
 typedef struct swsph_tune_result {
   int schedule;       /* chosen schedule */
   int chunk[3];       /* chosen chunk capacity of each loop */
   double loop_ms[3];  /* loop times per pass of the choice (ms) */
   double pass_ms;     /* wall time per pass, chosen schedule, capacity 16 */
   int trials;         /* configurations timed */
+  int chunk_j[3];     /* chosen candidate chunk capacity of each loop */
 } swsph_tune_result;
 int swsph_autotune(swsph_engine *e, int repeats, swsph_tune_result *out);
 

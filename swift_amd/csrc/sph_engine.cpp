@@ -95,7 +95,8 @@ struct swsph_engine {
     float *d_hg2max = nullptr;
   };
   std::vector<ChunkTable> tables;
-  int loop_c[3] = {16, 16, 16};
+  int loop_c[3] = {16, 16, 16};   // tile (i-chunk) capacity CI per loop
+  int loop_cj[3] = {16, 16, 16};  // candidate chunk capacity CJ (work queue)
   unsigned int *d_hg2_all = nullptr;  // max (gamma h)^2 over all chunks
   unsigned int *h_hg2_all = nullptr;
   // Neighbour lists of the batched pass (allocated on first use).
@@ -372,6 +373,23 @@ ChunkArgs chunk_args(const swsph_engine::ChunkTable *t) {
   return ca;
 }
 
+// Chunk capacity of the table a loop reads: CJ in the work-queue schedules,
+// else the tile capacity (i-chunks are chunks of the table).
+int table_c(const swsph_engine *e, int k) {
+  return e->schedule >= 1 ? e->loop_cj[k] : e->loop_c[k];
+}
+
+// Tables no loop reads any more are dropped from the per-pass refresh
+// (rebuilt on their next use).
+void prune_tables(swsph_engine *e) {
+  for (auto &t : e->tables) {
+    bool used = false;
+    for (int k = 0; k < 3; k++)
+      used = used || t.C == table_c(e, k) || t.C == e->loop_c[k];
+    if (!used) t.valid = false;
+  }
+}
+
 // Candidate positions / radii (unless `pos` is false: already written) and
 // chunk bounds of cells [c_lo, c_hi), after positions or smoothing lengths
 // changed.
@@ -571,7 +589,8 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
                bool build) {
   int rc = 0;
   const int C = build ? kListChunk : e->loop_c[(int)k];
-  const swsph_engine::ChunkTable *t = chunk_table(e, C, &rc);
+  const int CJ = build ? kListChunk : e->loop_cj[(int)k];
+  const swsph_engine::ChunkTable *t = chunk_table(e, CJ, &rc);
   if (!t) return rc;
   const LoopArgs la = loop_args(e);
   // The all-active table is kept for the later loops of the pass.
@@ -596,7 +615,7 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
   if ((rc = time_begin(e, &a))) return rc;
   if (max_tiles > 0)
     HIPCHK(sphk::launch_queue(k, build, e->d, la, chunk_args(t), tile_args(e),
-                              e->nl, C, e->schedule == 2, e->queue_waves,
+                              e->nl, C, CJ, e->schedule == 2, e->queue_waves,
                               max_tiles, e->st));
   e->launches += 13;
   return time_end(e, k, a);
@@ -868,7 +887,7 @@ int finish_binning(swsph_engine *e) {
   e->fields_epoch++;
   for (auto &t : e->tables) t.valid = false;
   for (int k = 0; k < 3; k++)
-    if (!chunk_table(e, e->loop_c[k], &rc)) return rc;
+    if (!chunk_table(e, table_c(e, k), &rc)) return rc;
   HIPCHK(hipStreamSynchronize(e->st));
   return 0;
 }
@@ -1681,7 +1700,7 @@ int swsph_pass_graph(swsph_engine *e, swsph_pass_stats *stats) {
     drop_graphs(e);
     // chunk tables must exist before capture (they allocate)
     for (int k = 0; k < 3; k++)
-      if (!chunk_table(e, e->loop_c[k], &rc)) return rc;
+      if (!chunk_table(e, table_c(e, k), &rc)) return rc;
     if ((rc = capture(e, &e->g_density, [&]() -> int {
            int r = phase_start(e);
            for (int k = 0; k < reruns && !r; k++) {
@@ -1758,10 +1777,38 @@ int swsph_set_chunk(swsph_engine *e, int which, int C) {
   if (C != 8 && C != 16 && C != 32 && C != 64)
     return fail(1, "chunk capacity %d not in {8, 16, 32, 64}", C);
   e->loop_c[which] = C;
+  e->loop_cj[which] = C;
+  prune_tables(e);
   if (!e->cell_start.empty()) {
     int rc = 0;
     if (!chunk_table(e, C, &rc)) return rc;
   }
+  return 0;
+}
+
+int swsph_set_tile_shape(swsph_engine *e, int which, int CI, int CJ) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  e->gen++;  // captured graphs no longer match
+  if (which < 0 || which > 2) return fail(1, "loop index %d out of range", which);
+  const bool ok = (CJ == CI && (CI == 8 || CI == 16 || CI == 32 || CI == 64)) ||
+                  (CJ * 2 == CI && (CI == 16 || CI == 32));
+  if (!ok) return fail(1, "tile shape (%d, %d) not supported", CI, CJ);
+  e->loop_c[which] = CI;
+  e->loop_cj[which] = CJ;
+  prune_tables(e);
+  if (!e->cell_start.empty()) {
+    int rc = 0;
+    if (!chunk_table(e, table_c(e, which), &rc)) return rc;
+  }
+  return 0;
+}
+
+int swsph_get_tile_shape(swsph_engine *e, int which, int *CI, int *CJ) {
+  if (!e || !CI || !CJ) return fail(1, "null argument");
+  if (which < 0 || which > 2) return fail(1, "loop index %d out of range", which);
+  *CI = e->loop_c[which];
+  *CJ = e->loop_cj[which];
   return 0;
 }
 
@@ -1773,6 +1820,7 @@ int swsph_set_schedule(swsph_engine *e, int mode, int waves) {
   if (waves < 0) return fail(1, "negative wave count");
   e->schedule = mode;
   e->queue_waves = waves;
+  prune_tables(e);
   return 0;
 }
 
@@ -1848,42 +1896,51 @@ int swsph_autotune(swsph_engine *e, int repeats, swsph_tune_result *out) {
   if ((rc = copy_state(e, e->tune_saved, true))) return rc;
   const int sched0 = e->schedule, waves0 = e->queue_waves;
   const int c0[3] = {e->loop_c[0], e->loop_c[1], e->loop_c[2]};
+  const int cj0[3] = {e->loop_cj[0], e->loop_cj[1], e->loop_cj[2]};
   const bool timing0 = e->timing;
+  int trials = 0;
+  // Times `repeats` passes of the current settings: per-loop kernel time
+  // and wall time per pass (ms).
+  auto time_config = [&](double ms[3], double *wall) -> int {
+    int r = 0;
+    e->gen++;
+    // warm-up pass (builds the chunk tables of these capacities)
+    if ((r = copy_state(e, e->tune_saved, false)) ||
+        (r = pass_batched_locked(e, nullptr)))
+      return r;
+    e->timing = true;
+    for (int k = 0; k < 3; k++) e->t_ms[k] = 0.0, e->t_launch[k] = 0;
+    e->ev_pending.clear();
+    e->ev_used = 0;
+    HIPCHK(hipStreamSynchronize(e->st));
+    const auto t0 = std::chrono::steady_clock::now();
+    for (int q = 0; q < repeats && !r; q++)
+      if (!(r = copy_state(e, e->tune_saved, false)))
+        r = pass_batched_locked(e, nullptr);
+    if (r) return r;
+    if ((r = collect_timing(e))) return r;
+    const auto t1 = std::chrono::steady_clock::now();
+    *wall = std::chrono::duration<double, std::milli>(t1 - t0).count() / repeats;
+    for (int k = 0; k < 3; k++) ms[k] = e->t_ms[k] / repeats;
+    e->timing = false;
+    trials++;
+    return 0;
+  };
   // With lists the density loop builds them in tiles of kListChunk.
   const int caps[3] = {8, 16, 32};
   double best_total = 1e300;
   int best_s = sched0, best_c[3] = {c0[0], c0[1], c0[2]};
+  int best_cj[3] = {cj0[0], cj0[1], cj0[2]};
   double best_ms[3] = {0, 0, 0}, best_pass = 0;
   double tab[3][3][3];  // [schedule][capacity][loop] ms per pass
   double wall[3][3];
-  int trials = 0;
   for (int sc = 0; sc < 3 && !rc; sc++) {
     for (int ci = 0; ci < 3 && !rc; ci++) {
       e->schedule = sc;
       e->queue_waves = 0;
-      for (int k = 0; k < 3; k++) e->loop_c[k] = caps[ci];
-      if (e->use_lists) e->loop_c[0] = kListChunk;
-      e->gen++;
-      // warm-up pass (builds the chunk tables of this capacity)
-      if ((rc = copy_state(e, e->tune_saved, false)) ||
-          (rc = pass_batched_locked(e, nullptr)))
-        break;
-      e->timing = true;
-      for (int k = 0; k < 3; k++) e->t_ms[k] = 0.0, e->t_launch[k] = 0;
-      e->ev_pending.clear();
-      e->ev_used = 0;
-      HIPCHK(hipStreamSynchronize(e->st));
-      const auto t0 = std::chrono::steady_clock::now();
-      for (int r = 0; r < repeats && !rc; r++)
-        if (!(rc = copy_state(e, e->tune_saved, false)))
-          rc = pass_batched_locked(e, nullptr);
-      if (rc) break;
-      const auto t1 = std::chrono::steady_clock::now();
-      wall[sc][ci] =
-          std::chrono::duration<double, std::milli>(t1 - t0).count() / repeats;
-      for (int k = 0; k < 3; k++) tab[sc][ci][k] = e->t_ms[k] / repeats;
-      e->timing = false;
-      trials++;
+      for (int k = 0; k < 3; k++) e->loop_c[k] = e->loop_cj[k] = caps[ci];
+      if (e->use_lists) e->loop_c[0] = e->loop_cj[0] = kListChunk;
+      rc = time_config(tab[sc][ci], &wall[sc][ci]);
     }
   }
   if (!rc) {
@@ -1904,15 +1961,41 @@ int swsph_autotune(swsph_engine *e, int repeats, swsph_tune_result *out) {
       if (tot < best_total) {
         best_total = tot;
         best_s = sc;
-        for (int k = 0; k < 3; k++) best_c[k] = cs[k], best_ms[k] = ms[k];
+        for (int k = 0; k < 3; k++)
+          best_c[k] = best_cj[k] = cs[k], best_ms[k] = ms[k];
         best_pass = wall[sc][1];
       }
+    }
+  }
+  if (!rc && best_s >= 1) {
+    // Under the work queue: candidate chunks of half the tile size.
+    e->schedule = best_s;
+    e->queue_waves = 0;
+    bool any = false;
+    for (int k = 0; k < 3; k++) {
+      e->loop_c[k] = best_c[k];
+      const bool half = (best_c[k] == 16 || best_c[k] == 32) &&
+                        !(e->use_lists && k == 0);
+      e->loop_cj[k] = half ? best_c[k] / 2 : best_c[k];
+      any = any || half;
+    }
+    double ms[3], w = 0;
+    if (any && !(rc = time_config(ms, &w))) {
+      for (int k = 0; k < 3; k++)
+        if (e->loop_cj[k] != best_c[k] && ms[k] < best_ms[k]) {
+          best_cj[k] = e->loop_cj[k];
+          best_ms[k] = ms[k];
+        }
     }
   }
   // apply the choice (or restore the settings after a failure) and the state
   e->schedule = rc ? sched0 : best_s;
   e->queue_waves = rc ? waves0 : 0;
-  for (int k = 0; k < 3; k++) e->loop_c[k] = rc ? c0[k] : best_c[k];
+  for (int k = 0; k < 3; k++) {
+    e->loop_c[k] = rc ? c0[k] : best_c[k];
+    e->loop_cj[k] = rc ? cj0[k] : best_cj[k];
+  }
+  prune_tables(e);
   e->gen++;
   e->timing = timing0;
   for (int k = 0; k < 3; k++) e->t_ms[k] = 0.0, e->t_launch[k] = 0;
@@ -1926,6 +2009,7 @@ int swsph_autotune(swsph_engine *e, int repeats, swsph_tune_result *out) {
     out->schedule = best_s;
     for (int k = 0; k < 3; k++) {
       out->chunk[k] = best_c[k];
+      out->chunk_j[k] = best_cj[k];
       out->loop_ms[k] = best_ms[k];
     }
     out->pass_ms = best_pass;

@@ -145,11 +145,15 @@ __device__ __forceinline__ unsigned int image_code(double tx, double ty,
 
 // LDS of one wave: the neighbour table, the compacted surviving chunks, the
 // staged candidate positions and the chunk table of the current 64 rows.
-template <int C>
+// CI: i-particles per wave (S = 64 / CI candidate streams each); CJ:
+// particles per candidate chunk (CJ >= S: a row of S candidates lies in one
+// chunk).
+template <int CI, int CJ>
 struct GatherLds {
-  static constexpr int S = 64 / C;
-  static constexpr int RPC = C / S > 0 ? C / S : 1;  // rows per chunk
-  static constexpr int SLOTS = 64 / RPC;             // chunks per 64 rows
+  static constexpr int S = 64 / CI;
+  static_assert(CJ >= S && CJ <= 64, "a row must lie in one chunk");
+  static constexpr int RPC = CJ / S;        // rows per chunk
+  static constexpr int SLOTS = 64 / RPC;    // chunks per 64 rows
   PosRec lpos[64];
   double ttab[27 * 3];
   double qorg[27 * 3];  // corner of each neighbour's image
@@ -162,12 +166,12 @@ struct GatherLds {
 // Interaction phase over the rows recorded in the lanes' masks.  Called from
 // converged code; the per-lane loop below is divergent, so everything it
 // shares is in LDS.
-template <class L, int C, bool BUILD>
+template <class L, int CI, int CJ, bool BUILD>
 __device__ __forceinline__ void interact_rows(
-    const typename L::Rec *__restrict__ rec, const GatherLds<C> &sh,
+    const typename L::Rec *__restrict__ rec, const GatherLds<CI, CJ> &sh,
     const typename L::I &st, int s, unsigned long long m,
     typename L::Acc &acc, const NbrList &nl, int i, float rt2, int &ncnt) {
-  constexpr int S = GatherLds<C>::S, RPC = GatherLds<C>::RPC;
+  constexpr int S = GatherLds<CI, CJ>::S, RPC = GatherLds<CI, CJ>::RPC;
   while (m) {
     const int row = __builtin_ctzll(m);
     m &= m - 1ull;
@@ -197,17 +201,21 @@ __device__ __forceinline__ void interact_rows(
 // One i-chunk (TILE: one tile of the work queue) against a list of
 // neighbour cells.  `stencil` selects the 27 neighbours of the i-chunk's
 // cell; otherwise the single cell `cj0` with translation (tx0, ty0, tz0).
-template <class L, int C, bool SUBSET, bool BUILD, bool TILE = false,
+// i-chunks come from the chunk table itself (CI == CJ) unless TILE.
+template <class L, int CI, int CJ, bool SUBSET, bool BUILD, bool TILE = false,
           int W = kWaves>
 __device__ __forceinline__ void run_chunk(
     const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
     const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
     int ich, bool stencil, int cj0, double tx0, double ty0, double tz0,
     const NbrList &nl, const TileArgs *tq = nullptr, int tile = 0) {
-  using Sh = GatherLds<C>;
+  using Sh = GatherLds<CI, CJ>;
   constexpr int S = Sh::S, RPC = Sh::RPC;
-  constexpr int G = 64 / C;  // chunks per staging group
-  static_assert(!BUILD || (C == kListChunk && !L::kSymmetric),
+  constexpr int C = CI;
+  constexpr int G = 64 / CJ;  // chunks per staging group
+  static_assert(TILE || CI == CJ, "i-chunks are chunks of the table");
+  static_assert(!BUILD || (CI == kListChunk && CJ == kListChunk &&
+                           !L::kSymmetric),
                 "lists are built by the density loop in list tiles");
   __shared__ Sh sh_all[W];
   Sh &sh = sh_all[W == 1 ? 0 : threadIdx.x >> 6];
@@ -349,11 +357,11 @@ __device__ __forceinline__ void run_chunk(
     for (int pb = 0; pb < np; pb += G) {
       // Stage the candidates of up to G surviving chunks.
       {
-        const int c = lane / C;
+        const int c = lane / CJ;
         if (pb + c < np) {
           const unsigned int e = sh.plist[pb + c];
           const int q = (int)(e >> 27);
-          const int j = sh.qj0[q] + (int)(e & kRowMask) * C + lane % C;
+          const int j = sh.qj0[q] + (int)(e & kRowMask) * CJ + lane % CJ;
           PosRec p;
           if (j < sh.qj1[q]) {
             p = pos[j];
@@ -369,8 +377,8 @@ __device__ __forceinline__ void run_chunk(
         if (pb + c >= np) break;  // uniform
         const unsigned int e = (unsigned int)uniform((int)sh.plist[pb + c]);
         const int q = (int)(e >> 27);
-        const int js = uniform(sh.qj0[q]) + (int)(e & kRowMask) * C;
-        const int nj = min(C, uniform(sh.qj1[q]) - js);
+        const int js = uniform(sh.qj0[q]) + (int)(e & kRowMask) * CJ;
+        const int nj = min(CJ, uniform(sh.qj1[q]) - js);
         // chunk slot of the lane's own particle (a particle is not its own
         // neighbour); out of the chunk's range unless the chunk holds it
         const int selfl = uniform(sh.qcell[q]) == cell_i ? i - js : -1;
@@ -388,7 +396,7 @@ __device__ __forceinline__ void run_chunk(
         for (int rr = 0; rr < RPC; rr++) {
           if (rr * S >= nj) break;  // uniform
           const int jl = rr * S + s;
-          const PosRec p = sh.lpos[c * C + jl];
+          const PosRec p = sh.lpos[c * CJ + jl];
           const float dx = xs - p.x, dy = ys - p.y, dz = zs - p.z;
           const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
           const bool pass =
@@ -399,8 +407,8 @@ __device__ __forceinline__ void run_chunk(
         nrow += RPC;
         if (nrow == 64) {
           wave_sync();
-          interact_rows<L, C, BUILD>(rec, sh, st, s, m, acc, nl, i, rt2,
-                                     ncnt);
+          interact_rows<L, CI, CJ, BUILD>(rec, sh, st, s, m, acc, nl, i, rt2,
+                                          ncnt);
           m = 0ull;
           nrow = 0;
         }
@@ -410,7 +418,7 @@ __device__ __forceinline__ void run_chunk(
   }
   if (nrow > 0) {
     wave_sync();
-    interact_rows<L, C, BUILD>(rec, sh, st, s, m, acc, nl, i, rt2, ncnt);
+    interact_rows<L, CI, CJ, BUILD>(rec, sh, st, s, m, acc, nl, i, rt2, ncnt);
   }
   L::template combine<S>(acc);
   if (act && s == 0) L::store_acc(d, i, acc);
@@ -433,8 +441,8 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_stencil(
     NbrList nl) {
   const int ich = xcd_item(blockIdx.x, threadIdx.x >> 6, ca.nchunks);
   if (ich >= ca.nchunks) return;  // uniform per wave
-  run_chunk<L, C, SUBSET, BUILD>(d, la, ca, rec, pos, ich, true, 0, 0.0, 0.0,
-                                 0.0, nl);
+  run_chunk<L, C, C, SUBSET, BUILD>(d, la, ca, rec, pos, ich, true, 0, 0.0,
+                                    0.0, 0.0, nl);
 }
 
 // Per task: up to two directed (i-cell <- j-cell) interactions; blocks
@@ -452,10 +460,10 @@ __global__ __launch_bounds__(64) void k_loop_task(
   }
   const int ich = ca.cell_chunk[ta.icell[q]] + b;
   const NbrList none{};
-  run_chunk<L, C, SUBSET, false, false, 1>(d, la, ca, rec, pos, ich, false,
-                                           ta.jcell[q], ta.t[3 * q],
-                                           ta.t[3 * q + 1], ta.t[3 * q + 2],
-                                           none);
+  run_chunk<L, C, C, SUBSET, false, false, 1>(d, la, ca, rec, pos, ich, false,
+                                              ta.jcell[q], ta.t[3 * q],
+                                              ta.t[3 * q + 1], ta.t[3 * q + 2],
+                                              none);
 }
 
 // Persistent form: a fixed grid of waves drains the tile queue.  A wave
@@ -464,7 +472,7 @@ __global__ __launch_bounds__(64) void k_loop_task(
 // its own is exhausted; every queue is visited at most once after it runs
 // dry, so every wave exits.
 constexpr int kGrab = 2;
-template <class L, int C, bool BUILD>
+template <class L, int CI, int CJ, bool BUILD>
 __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_queue(
     DevParts d, LoopArgs la, ChunkArgs ca,
     const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
@@ -486,23 +494,23 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_queue(
     const int end = min(hi, lo + kGrab);
     for (int t = lo; t < end; t++) {
       wave_sync();
-      run_chunk<L, C, false, BUILD, true>(d, la, ca, rec, pos, 0, true, 0, 0.0,
-                                          0.0, 0.0, nl, &tq, t);
+      run_chunk<L, CI, CJ, false, BUILD, true>(d, la, ca, rec, pos, 0, true, 0,
+                                               0.0, 0.0, 0.0, nl, &tq, t);
     }
   }
 }
 
 // Grid form of the tile queue: one wave per tile over a grid sized for the
 // largest possible tile count; waves past the device-side count exit.
-template <class L, int C, bool BUILD>
+template <class L, int CI, int CJ, bool BUILD>
 __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_tiles(
     DevParts d, LoopArgs la, ChunkArgs ca,
     const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
     NbrList nl, TileArgs tq) {
   const int t = xcd_item(blockIdx.x, threadIdx.x >> 6, *tq.ntiles);
   if (t >= *tq.ntiles) return;  // uniform per wave
-  run_chunk<L, C, false, BUILD, true>(d, la, ca, rec, pos, 0, true, 0, 0.0, 0.0,
-                                      0.0, nl, &tq, t);
+  run_chunk<L, CI, CJ, false, BUILD, true>(d, la, ca, rec, pos, 0, true, 0, 0.0,
+                                           0.0, 0.0, nl, &tq, t);
 }
 
 // List loop: lane g + kListChunk * s walks stream s of the neighbour list of

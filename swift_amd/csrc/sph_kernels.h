@@ -225,12 +225,14 @@ hipError_t launch_fill_f32(float *p, float v, int n, hipStream_t st);
 hipError_t launch_build_tiles(const DevParts &d, const LoopArgs &la, int n,
                               int ncells, int C, int mode, const int *flag,
                               const TileScratch &ts, hipStream_t st);
-// A neighbour loop over the tiles: by a persistent grid of `waves` waves
-// (0: full occupancy on every CU) when `persistent`, else by one wave per
-// tile over a grid for `max_tiles` tiles.
+// A neighbour loop over the tiles (of <= CI particles) against candidate
+// chunks of CJ particles (the table `ca`): by a persistent grid of `waves`
+// waves (0: full occupancy on every CU) when `persistent`, else by one wave
+// per tile over a grid for `max_tiles` tiles.  Shapes: CJ = CI, or
+// CJ = CI / 2 for CI = 16, 32.
 hipError_t launch_queue(LoopKind kind, bool build_lists, const DevParts &d,
                         const LoopArgs &la, const ChunkArgs &ca,
-                        const TileArgs &ta, const NbrList &nl, int C,
+                        const TileArgs &ta, const NbrList &nl, int CI, int CJ,
                         bool persistent, int waves, int max_tiles,
                         hipStream_t st);
 hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,

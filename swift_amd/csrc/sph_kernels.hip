@@ -685,7 +685,7 @@ hipError_t launch_build_tiles(const DevParts &d, const LoopArgs &la, int n,
 
 // Resident grid of a persistent kernel: blocks per CU at full occupancy
 // times the CU count (queried once per kernel instance).
-template <class L, int C, bool B>
+template <class L, int CI, int CJ, bool B>
 static int resident_blocks() {
   static std::atomic<int> cached{0};
   if (cached.load() == 0) {
@@ -694,7 +694,7 @@ static int resident_blocks() {
         hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
                               dev) != hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per, k_loop_queue<L, C, B>, 64 * kWaves, 0) != hipSuccess ||
+            &per, k_loop_queue<L, CI, CJ, B>, 64 * kWaves, 0) != hipSuccess ||
         per <= 0)
       return 0;
     cached.store(ncu * per * kWaves);
@@ -702,19 +702,19 @@ static int resident_blocks() {
   return cached.load();
 }
 
-template <class L, int C, bool B>
+template <class L, int CI, int CJ, bool B>
 hipError_t queue_c(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
                    const TileArgs &ta, const NbrList &nl,
                    const typename L::Rec *rec, bool persistent, int waves,
                    int max_tiles, hipStream_t st) {
   if (!persistent) {
-    hipLaunchKernelGGL((k_loop_tiles<L, C, B>), dim3(xcd_blocks(max_tiles)),
-                       dim3(64 * kWaves), 0, st, d, la, ca, rec,
-                       (const PosRec *)d.pos, nl, ta);
+    hipLaunchKernelGGL((k_loop_tiles<L, CI, CJ, B>),
+                       dim3(xcd_blocks(max_tiles)), dim3(64 * kWaves), 0, st,
+                       d, la, ca, rec, (const PosRec *)d.pos, nl, ta);
     return hipGetLastError();
   }
-  auto kern = k_loop_queue<L, C, B>;
-  if (waves <= 0) waves = resident_blocks<L, C, B>();
+  auto kern = k_loop_queue<L, CI, CJ, B>;
+  if (waves <= 0) waves = resident_blocks<L, CI, CJ, B>();
   if (waves <= 0) return hipErrorInvalidValue;
   // `waves` counts waves; blocks hold kWaves of them
   hipLaunchKernelGGL(kern, dim3((waves + kWaves - 1) / kWaves),
@@ -723,40 +723,45 @@ hipError_t queue_c(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
   return hipGetLastError();
 }
 
+// Tile (CI) x chunk (CJ) shapes with kernels: CJ = CI, or CJ = CI / 2 for
+// CI = 16, 32 (finer candidate chunks under the same tile).
 template <class L>
 hipError_t queue_any(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
                      const TileArgs &ta, const NbrList &nl,
-                     const typename L::Rec *rec, int C, bool p, int waves,
-                     int mt, hipStream_t st) {
-  switch (C) {
-    case 8: return queue_c<L, 8, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
-    case 16: return queue_c<L, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
-    case 32: return queue_c<L, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
-    case 64: return queue_c<L, 64, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+                     const typename L::Rec *rec, int CI, int CJ, bool p,
+                     int waves, int mt, hipStream_t st) {
+  switch (CI * 100 + CJ) {
+    case 808: return queue_c<L, 8, 8, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 1616: return queue_c<L, 16, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 1608: return queue_c<L, 16, 8, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 3232: return queue_c<L, 32, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 3216: return queue_c<L, 32, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 6464: return queue_c<L, 64, 64, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
   }
   return hipErrorInvalidValue;
 }
 
 hipError_t launch_queue(LoopKind kind, bool build_lists, const DevParts &d,
                         const LoopArgs &la, const ChunkArgs &ca,
-                        const TileArgs &ta, const NbrList &nl, int C,
+                        const TileArgs &ta, const NbrList &nl, int CI, int CJ,
                         bool persistent, int waves, int max_tiles,
                         hipStream_t st) {
+  if (ca.C != CJ) return hipErrorInvalidValue;
   switch (kind) {
     case LoopKind::Density:
       if (build_lists) {
-        if (C != kListChunk) return hipErrorInvalidValue;
-        return queue_c<DensityLoop, kListChunk, true>(
+        if (CI != kListChunk || CJ != kListChunk) return hipErrorInvalidValue;
+        return queue_c<DensityLoop, kListChunk, kListChunk, true>(
             d, la, ca, ta, nl, d.rd, persistent, waves, max_tiles, st);
       }
-      return queue_any<DensityLoop>(d, la, ca, ta, nl, d.rd, C, persistent,
-                                    waves, max_tiles, st);
+      return queue_any<DensityLoop>(d, la, ca, ta, nl, d.rd, CI, CJ,
+                                    persistent, waves, max_tiles, st);
     case LoopKind::Gradient:
-      return queue_any<GradientLoop>(d, la, ca, ta, nl, d.rg, C, persistent,
-                                     waves, max_tiles, st);
+      return queue_any<GradientLoop>(d, la, ca, ta, nl, d.rg, CI, CJ,
+                                     persistent, waves, max_tiles, st);
     case LoopKind::Force:
-      return queue_any<ForceLoop>(d, la, ca, ta, nl, d.rf, C, persistent,
-                                  waves, max_tiles, st);
+      return queue_any<ForceLoop>(d, la, ca, ta, nl, d.rf, CI, CJ,
+                                  persistent, waves, max_tiles, st);
   }
   return hipErrorInvalidValue;
 }

@@ -40,7 +40,7 @@ PASS_FIELDS = [k for k, v in FIELDS.items() if v[0] < 25]
 class _TuneResult(C.Structure):
     _fields_ = [("schedule", C.c_int), ("chunk", C.c_int * 3),
                 ("loop_ms", C.c_double * 3), ("pass_ms", C.c_double),
-                ("trials", C.c_int)]
+                ("trials", C.c_int), ("chunk_j", C.c_int * 3)]
 
 
 class _Props(C.Structure):
@@ -251,8 +251,8 @@ class Engine:
         check(lib().swsph_autotune(self._h, int(repeats), C.byref(r)),
               "swsph_autotune")
         return {"schedule": r.schedule, "chunk": list(r.chunk),
-                "loop_ms": list(r.loop_ms), "pass_ms": r.pass_ms,
-                "trials": r.trials}
+                "chunk_j": list(r.chunk_j), "loop_ms": list(r.loop_ms),
+                "pass_ms": r.pass_ms, "trials": r.trials}
 
     def pass_graph(self) -> dict:
         """The batched pass replayed from captured HIP graphs."""
@@ -267,6 +267,19 @@ class Engine:
         candidate streams per particle (C in 8, 16, 32, 64)."""
         check(lib().swsph_set_chunk(self._h, self.LOOPS[loop], int(C)),
               "swsph_set_chunk")
+
+    def set_tile_shape(self, loop: str, CI: int, CJ: int) -> None:
+        """Tiles of CI i-particles against candidate chunks of CJ particles
+        in the work-queue schedules (CJ = CI, or CI / 2 for CI = 16, 32)."""
+        check(lib().swsph_set_tile_shape(self._h, self.LOOPS[loop], int(CI),
+                                         int(CJ)), "swsph_set_tile_shape")
+
+    def get_tile_shape(self, loop: str) -> tuple:
+        ci, cj = C.c_int(), C.c_int()
+        check(lib().swsph_get_tile_shape(self._h, self.LOOPS[loop],
+                                         C.byref(ci), C.byref(cj)),
+              "swsph_get_tile_shape")
+        return ci.value, cj.value
 
     def get_chunk(self, loop: str) -> int:
         c = C.c_int()

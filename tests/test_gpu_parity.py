@@ -148,6 +148,35 @@ def test_chunk_shapes(orc, C):
     _check(orc, props, st, res)
 
 
+@pytest.mark.parametrize("shape", [(16, 8), (32, 16)])
+@pytest.mark.parametrize("schedule", [1, 2])
+def test_tile_shapes(orc, shape, schedule):
+    """Work-queue tiles against finer candidate chunks give the oracle's
+    neighbour sets and values, on a uniform and on a clustered box (half of
+    the particles active)."""
+    from swift_amd import Engine, HydroProps, ics
+    CI, CJ = shape
+    for st, props in (
+            (ics.modulated_box(16, seed=13, amp=0.5), HydroProps(ncell=(4, 4, 4))),
+            (ics.clustered_box(12000, seed=3, n_spheres=6, a_min=0.01,
+                               a_max=0.04), HydroProps(ncell=(4, 4, 4)))):
+        # the inactive half brings the state of an all-active pass
+        base, _ = _run_sched(st, props, 0)
+        for k in ("h", "rho", "pressure", "soundspeed", "f", "balsara",
+                  "alpha_visc", "alpha_diff"):
+            st[k] = base[k].astype(st[k].dtype)
+        ics.set_active_fraction(st, 0.5, seed=3)
+        with Engine(props) as eng:
+            eng.set_schedule(schedule)
+            for loop in ("density", "gradient", "force"):
+                eng.set_tile_shape(loop, CI, CJ)
+            assert eng.get_tile_shape("force") == (CI, CJ)
+            eng.set_particles(st)
+            eng.pass_batched()
+            res = eng.results()
+        _check(orc, props, st, res)
+
+
 def test_clustered_small(orc):
     """Plummer spheres on a uniform background: h spread over two decades,
     cells of very uneven occupancy, chunk box culling doing real work."""
