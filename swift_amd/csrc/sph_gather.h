@@ -82,6 +82,13 @@ __device__ __forceinline__ float wave_max(float v) {
 
 constexpr unsigned int kRowMask = (1u << 27) - 1u;
 
+// 64-bit hit masks per lane between two interaction phases (1 or 2): more
+// rows per phase even out the lanes' hit counts, at two registers a mask.
+#ifndef SWSPH_MASKS
+#define SWSPH_MASKS 2
+#endif
+constexpr int kMasks = SWSPH_MASKS;
+
 // Waves per block of the batched kernels.  Each wave works on its own chunk
 // or tile with its own LDS.  Several waves per block lift the per-CU
 // workgroup limit off the occupancy, but measured slower on the 128^3 box
@@ -153,7 +160,7 @@ struct GatherLds {
   static constexpr int S = 64 / CI;
   static_assert(CJ >= S && CJ <= 64, "a row must lie in one chunk");
   static constexpr int RPC = CJ / S;        // rows per chunk
-  static constexpr int SLOTS = 64 / RPC;    // chunks per 64 rows
+  static constexpr int SLOTS = 64 * kMasks / RPC;  // chunks per phase
   PosRec lpos[64];
   double ttab[27 * 3];
   double qorg[27 * 3];  // corner of each neighbour's image
@@ -169,12 +176,23 @@ struct GatherLds {
 template <class L, int CI, int CJ, bool BUILD>
 __device__ __forceinline__ void interact_rows(
     const typename L::Rec *__restrict__ rec, const GatherLds<CI, CJ> &sh,
-    const typename L::I &st, int s, unsigned long long m,
-    typename L::Acc &acc, const NbrList &nl, int i, float rt2, int &ncnt) {
+    const typename L::I &st, int s, unsigned long long m0,
+    unsigned long long m1, typename L::Acc &acc, const NbrList &nl, int i,
+    float rt2, int &ncnt) {
   constexpr int S = GatherLds<CI, CJ>::S, RPC = GatherLds<CI, CJ>::RPC;
-  while (m) {
-    const int row = __builtin_ctzll(m);
-    m &= m - 1ull;
+  while (kMasks == 1 ? m0 != 0ull : (m0 | m1) != 0ull) {
+    int row;
+    if (kMasks == 1) {
+      row = __builtin_ctzll(m0);
+      m0 &= m0 - 1ull;
+    } else {
+      const bool lo = m0 != 0ull;
+      const unsigned long long mm = lo ? m0 : m1;
+      row = __builtin_ctzll(mm) + (lo ? 0 : 64);
+      const unsigned long long rest = mm & (mm - 1ull);
+      m0 = lo ? rest : m0;
+      m1 = lo ? m1 : rest;
+    }
     const unsigned int info = sh.cinfo[row / RPC];
     const int j = (int)(info & kRowMask) + (row % RPC) * S + s;
     const int q = (int)(info >> 27);
@@ -325,7 +343,7 @@ __device__ __forceinline__ void run_chunk(
   }
   wave_sync();
 
-  unsigned long long m = 0ull;  // row hits of this lane
+  unsigned long long m0 = 0ull, m1 = 0ull;  // row hits of this lane
   int nrow = 0;                 // rows recorded (uniform, multiple of RPC)
   for (int fb = 0; fb < total; fb += 64) {
     // Cull: one candidate chunk per lane.
@@ -403,13 +421,16 @@ __device__ __forceinline__ void run_chunk(
               L::kSymmetric ? (r2 < rt2m || r2 < p.hg2m) : r2 < rt2m;
           nib |= (pass && jl != selfl) ? ((Nib)1 << rr) : (Nib)0;
         }
-        m |= (unsigned long long)nib << nrow;
+        if (kMasks == 1 || nrow < 64)  // uniform
+          m0 |= (unsigned long long)nib << nrow;
+        else
+          m1 |= (unsigned long long)nib << (nrow - 64);
         nrow += RPC;
-        if (nrow == 64) {
+        if (nrow == 64 * kMasks) {
           wave_sync();
-          interact_rows<L, CI, CJ, BUILD>(rec, sh, st, s, m, acc, nl, i, rt2,
-                                          ncnt);
-          m = 0ull;
+          interact_rows<L, CI, CJ, BUILD>(rec, sh, st, s, m0, m1, acc, nl, i,
+                                          rt2, ncnt);
+          m0 = m1 = 0ull;
           nrow = 0;
         }
       }
@@ -418,7 +439,8 @@ __device__ __forceinline__ void run_chunk(
   }
   if (nrow > 0) {
     wave_sync();
-    interact_rows<L, CI, CJ, BUILD>(rec, sh, st, s, m, acc, nl, i, rt2, ncnt);
+    interact_rows<L, CI, CJ, BUILD>(rec, sh, st, s, m0, m1, acc, nl, i, rt2,
+                                    ncnt);
   }
   L::template combine<S>(acc);
   if (act && s == 0) L::store_acc(d, i, acc);
