@@ -1205,11 +1205,18 @@ int copy_state(swsph_engine *e, std::vector<float *> &buf, bool save) {
   return 0;
 }
 
+// A pass over no particles: nothing to do, zero counts.
+int empty_pass(swsph_pass_stats *stats) {
+  if (stats) *stats = swsph_pass_stats{};
+  return 0;
+}
+
 // The eager batched pass (the engine mutex held by the caller).
 int pass_batched_locked(swsph_engine *e, swsph_pass_stats *stats) {
   int rc = 0;
   if (e->need_particles)
     return fail(1, "the grid changed: call swsph_set_particles first");
+  if (e->n == 0) return empty_pass(stats);
   const int64_t l0 = e->launches;
   e->list_rebuilt = 0;
   e->list_fallback = 0;
@@ -1624,6 +1631,7 @@ int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
   std::lock_guard<std::mutex> lk(e->mtx);
   if (e->need_particles)
     return fail(1, "the grid changed: call swsph_set_particles first");
+  if (e->n == 0) return empty_pass(stats);
   int rc = 0;
   const int64_t l0 = e->launches;
   std::vector<int> up;
@@ -1687,7 +1695,7 @@ int swsph_pass_graph(swsph_engine *e, swsph_pass_stats *stats) {
     return fail(1, "graph replay needs the neighbour lists disabled");
   if (e->need_particles)
     return fail(1, "the grid changed: call swsph_set_particles first");
-  if (e->n == 0) return fail(1, "no particles set");
+  if (e->n == 0) return empty_pass(stats);
   const int64_t l0 = e->launches;
   e->list_rebuilt = 0;
   e->list_fallback = 0;
@@ -2078,6 +2086,7 @@ int swsph_engine_step(swsph_engine *e, swsph_step_info *info) {
     return fail(1, "the grid changed: call swsph_set_particles first");
   if (!(e->props.time_base > 0.0))
     return fail(1, "time_base must be positive for time integration");
+  if (e->n == 0) return fail(1, "no particles to integrate");
   int rc = 0, rebuilt = 0;
   const int64_t l0 = e->launches;
   const bool first = !e->started;
