@@ -59,7 +59,6 @@ struct swsph_engine {
   int ncells = 0;
   int64_t n = 0;
   std::vector<int> cell_start;    // host copy [ncells+1]
-  std::vector<int64_t> perm;      // sorted position -> caller index
   std::vector<int4> tiles_h;
   std::vector<int> cell_tiles;    // tiles per cell
   DevParts d{};
@@ -138,7 +137,6 @@ struct swsph_engine {
   void *d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
   double *d_stage = nullptr;
-  bool perm_stale = false;  // host `perm` older than d.perm
   // Time integration: parameters, the integer time line, and the largest
   // distance by which a particle lies outside its cell since the last
   // binning (float bits on the device; host copy).
@@ -892,18 +890,6 @@ int finish_binning(swsph_engine *e) {
   return 0;
 }
 
-// Host copy of the permutation sorted slot -> caller index.
-int ensure_perm(swsph_engine *e) {
-  if (!e->perm_stale) return 0;
-  std::vector<int> t((size_t)e->n);
-  if (e->n)
-    HIPCHK(hipMemcpy(t.data(), e->d.perm, sizeof(int) * e->n,
-                     hipMemcpyDeviceToHost));
-  e->perm.assign(t.begin(), t.end());
-  e->perm_stale = false;
-  return 0;
-}
-
 // Re-binning of drifted particles (space_rebuild's particle sort for this
 // path): positions wrapped into the box, sorted by (cell, Morton code) on
 // the device, every per-particle field carried along.  Neighbour lists,
@@ -939,7 +925,6 @@ int rebin(swsph_engine *e) {
   HIPCHK(sphk::launch_fill_f32(d.r2b, -1.f, ni, e->st));
   HIPCHK(hipMemsetAsync(e->d_dmax, 0, sizeof(unsigned int), e->st));
   e->dmax = 0.f;
-  e->perm_stale = true;
   e->saved_valid = false;
   e->tiles_c = 0;
   e->gen++;
@@ -1443,7 +1428,6 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
     HIPCHK(sphk::launch_cell_bounds(e->d_key[1], 0, e->ncells,
                                     e->d_cell_start, d.cell, e->st));
   }
-  e->perm_stale = true;
   if ((rc = finish_binning(e))) return rc;
   return check_fits(e);
 }
@@ -1451,84 +1435,75 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
 int swsph_get_field(swsph_engine *e, int field, void *out) {
   if (!e || !out) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
-  const int64_t n = e->n;
-  HIPCHK(hipStreamSynchronize(e->st));
-  int rc0 = 0;
-  if ((rc0 = ensure_perm(e))) return rc0;
+  const int n = (int)e->n;
   const DevParts &d = e->d;
-  auto get_f = [&](const float *src, int stride, int k) -> int {
-    std::vector<float> t(n);
-    if (n) HIPCHK(hipMemcpy(t.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost));
-    float *o = (float *)out;
-    for (int64_t s = 0; s < n; s++) o[stride * e->perm[s] + k] = t[s];
+  // Scattered into the caller's order on the device (through the staging
+  // buffer, 24 B per particle), then one contiguous copy to the host.
+  auto fetch = [&](auto *const *src, int ncomp) -> int {
+    using T = typename std::remove_const<
+        typename std::remove_pointer<typename std::remove_pointer<
+            decltype(src)>::type>::type>::type;
+    T *stage = (T *)e->d_stage;
+    for (int k = 0; k < ncomp; k++)
+      HIPCHK(sphk::launch_scatter(stage, (const T *)src[k], ncomp, k, d.perm,
+                                  n, e->st));
+    if (n)
+      HIPCHK(hipMemcpyAsync(out, stage, sizeof(T) * (size_t)ncomp * n,
+                            hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
     return 0;
   };
-  auto get_i = [&](const int *src) -> int {
-    std::vector<int> t(n);
-    if (n) HIPCHK(hipMemcpy(t.data(), src, sizeof(int) * n, hipMemcpyDeviceToHost));
-    int32_t *o = (int32_t *)out;
-    for (int64_t s = 0; s < n; s++) o[e->perm[s]] = t[s];
-    return 0;
+  auto f1 = [&](float *p) {
+    float *const v[1] = {p};
+    return fetch(v, 1);
   };
-  int rc = 0;
+  auto f3 = [&](float *a, float *b, float *c) {
+    float *const v[3] = {a, b, c};
+    return fetch(v, 3);
+  };
+  auto i1 = [&](int *p) {
+    int *const v[1] = {p};
+    return fetch(v, 1);
+  };
+  auto b1 = [&](int8_t *p) {
+    int8_t *const v[1] = {p};
+    return fetch(v, 1);
+  };
   switch (field) {
-    case SWSPH_H: return get_f(d.h, 1, 0);
-    case SWSPH_RHO: return get_f(d.rho, 1, 0);
-    case SWSPH_RHO_DH: return get_f(d.rho_dh, 1, 0);
-    case SWSPH_WCOUNT: return get_f(d.wc, 1, 0);
-    case SWSPH_WCOUNT_DH: return get_f(d.wc_dh, 1, 0);
-    case SWSPH_DIV_V: return get_f(d.divv, 1, 0);
-    case SWSPH_ROT_V:
-      if ((rc = get_f(d.rotx, 3, 0)) || (rc = get_f(d.roty, 3, 1))) return rc;
-      return get_f(d.rotz, 3, 2);
-    case SWSPH_PRESSURE: return get_f(d.P, 1, 0);
-    case SWSPH_SOUNDSPEED: return get_f(d.c, 1, 0);
-    case SWSPH_F: return get_f(d.f, 1, 0);
-    case SWSPH_BALSARA: return get_f(d.bal, 1, 0);
-    case SWSPH_V_SIG: return get_f(d.vsig, 1, 0);
-    case SWSPH_LAPLACE_U: return get_f(d.lapu, 1, 0);
-    case SWSPH_ALPHA_VISC_MAX_NGB: return get_f(d.avmax, 1, 0);
-    case SWSPH_ALPHA_VISC: return get_f(d.alpha, 1, 0);
-    case SWSPH_ALPHA_DIFF: return get_f(d.adiff, 1, 0);
-    case SWSPH_DIV_V_DT: return get_f(d.divv_dt, 1, 0);
-    case SWSPH_DIV_V_PREV: return get_f(d.divv_prev, 1, 0);
-    case SWSPH_A_HYDRO:
-      if ((rc = get_f(d.ax, 3, 0)) || (rc = get_f(d.ay, 3, 1))) return rc;
-      return get_f(d.az, 3, 2);
-    case SWSPH_U_DT: return get_f(d.udt, 1, 0);
-    case SWSPH_H_DT: return get_f(d.hdt, 1, 0);
-    case SWSPH_N_DENSITY: return get_i(d.nd);
-    case SWSPH_N_GRADIENT: return get_i(d.ng);
-    case SWSPH_N_FORCE: return get_i(d.nf);
-    case SWSPH_MIN_NGB_TIME_BIN:
-    case SWSPH_TIME_BIN: {
-      std::vector<int8_t> t(n);
-      const int8_t *src = field == SWSPH_TIME_BIN ? d.tb : d.mintb;
-      if (n) HIPCHK(hipMemcpy(t.data(), src, n, hipMemcpyDeviceToHost));
-      int8_t *o = (int8_t *)out;
-      for (int64_t s = 0; s < n; s++) o[e->perm[s]] = t[s];
-      return 0;
-    }
+    case SWSPH_H: return f1(d.h);
+    case SWSPH_RHO: return f1(d.rho);
+    case SWSPH_RHO_DH: return f1(d.rho_dh);
+    case SWSPH_WCOUNT: return f1(d.wc);
+    case SWSPH_WCOUNT_DH: return f1(d.wc_dh);
+    case SWSPH_DIV_V: return f1(d.divv);
+    case SWSPH_ROT_V: return f3(d.rotx, d.roty, d.rotz);
+    case SWSPH_PRESSURE: return f1(d.P);
+    case SWSPH_SOUNDSPEED: return f1(d.c);
+    case SWSPH_F: return f1(d.f);
+    case SWSPH_BALSARA: return f1(d.bal);
+    case SWSPH_V_SIG: return f1(d.vsig);
+    case SWSPH_LAPLACE_U: return f1(d.lapu);
+    case SWSPH_ALPHA_VISC_MAX_NGB: return f1(d.avmax);
+    case SWSPH_ALPHA_VISC: return f1(d.alpha);
+    case SWSPH_ALPHA_DIFF: return f1(d.adiff);
+    case SWSPH_DIV_V_DT: return f1(d.divv_dt);
+    case SWSPH_DIV_V_PREV: return f1(d.divv_prev);
+    case SWSPH_A_HYDRO: return f3(d.ax, d.ay, d.az);
+    case SWSPH_U_DT: return f1(d.udt);
+    case SWSPH_H_DT: return f1(d.hdt);
+    case SWSPH_N_DENSITY: return i1(d.nd);
+    case SWSPH_N_GRADIENT: return i1(d.ng);
+    case SWSPH_N_FORCE: return i1(d.nf);
+    case SWSPH_MIN_NGB_TIME_BIN: return b1(d.mintb);
+    case SWSPH_TIME_BIN: return b1(d.tb);
     case SWSPH_X: {
-      const double *src[3] = {d.px, d.py, d.pz};
-      std::vector<double> t(n);
-      double *o = (double *)out;
-      for (int k = 0; k < 3; k++) {
-        if (n)
-          HIPCHK(hipMemcpy(t.data(), src[k], sizeof(double) * n,
-                           hipMemcpyDeviceToHost));
-        for (int64_t s = 0; s < n; s++) o[3 * e->perm[s] + k] = t[s];
-      }
-      return 0;
+      double *const v[3] = {d.px, d.py, d.pz};
+      return fetch(v, 3);
     }
-    case SWSPH_V:
-      if ((rc = get_f(d.vx, 3, 0)) || (rc = get_f(d.vy, 3, 1))) return rc;
-      return get_f(d.vz, 3, 2);
-    case SWSPH_V_FULL:
-      if ((rc = get_f(d.vfx, 3, 0)) || (rc = get_f(d.vfy, 3, 1))) return rc;
-      return get_f(d.vfz, 3, 2);
-    case SWSPH_U: return get_f(d.u, 1, 0);
-    case SWSPH_U_FULL: return get_f(d.ufull, 1, 0);
+    case SWSPH_V: return f3(d.vx, d.vy, d.vz);
+    case SWSPH_V_FULL: return f3(d.vfx, d.vfy, d.vfz);
+    case SWSPH_U: return f1(d.u);
+    case SWSPH_U_FULL: return f1(d.ufull);
     default:
       return fail(1, "unknown field %d", field);
   }

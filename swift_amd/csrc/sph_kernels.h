@@ -291,6 +291,10 @@ hipError_t launch_cell_bounds(const unsigned long long *key, int n, int ncells,
 template <class T>
 hipError_t launch_gather(T *dst, const T *src, int stride, int comp,
                          const int *idx, int n, hipStream_t st);
+// dst[stride * idx[k] + comp] = src[k]
+template <class T>
+hipError_t launch_scatter(T *dst, const T *src, int stride, int comp,
+                          const int *idx, int n, hipStream_t st);
 hipError_t launch_fill_i8(int8_t *p, int8_t v, int n, hipStream_t st);
 hipError_t launch_drift(const DevParts &d, const DriftArgs &a,
                         unsigned int *dmax, hipStream_t st);

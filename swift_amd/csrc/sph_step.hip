@@ -134,6 +134,14 @@ __global__ void k_gather(T *dst, const T *src, int stride, int comp,
   if (k < n) dst[k] = src[(size_t)stride * idx[k] + comp];
 }
 
+// dst[stride * idx[k] + comp] = src[k]: back to the caller's order.
+template <class T>
+__global__ void k_scatter(T *dst, const T *src, int stride, int comp,
+                          const int *idx, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[(size_t)stride * idx[k] + comp] = src[k];
+}
+
 template <class T>
 __global__ void k_fill(T *dst, T v, int n) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -481,6 +489,22 @@ template hipError_t launch_gather<int>(int *, const int *, int, int,
                                        const int *, int, hipStream_t);
 template hipError_t launch_gather<int8_t>(int8_t *, const int8_t *, int, int,
                                           const int *, int, hipStream_t);
+
+template <class T>
+hipError_t launch_scatter(T *dst, const T *src, int stride, int comp,
+                          const int *idx, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  k_scatter<T><<<blocks(n), kBlock, 0, st>>>(dst, src, stride, comp, idx, n);
+  return hipGetLastError();
+}
+template hipError_t launch_scatter<double>(double *, const double *, int, int,
+                                           const int *, int, hipStream_t);
+template hipError_t launch_scatter<float>(float *, const float *, int, int,
+                                          const int *, int, hipStream_t);
+template hipError_t launch_scatter<int>(int *, const int *, int, int,
+                                        const int *, int, hipStream_t);
+template hipError_t launch_scatter<int8_t>(int8_t *, const int8_t *, int, int,
+                                           const int *, int, hipStream_t);
 
 hipError_t launch_fill_i8(int8_t *p, int8_t v, int n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
