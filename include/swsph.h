@@ -111,8 +111,10 @@ int swsph_engine_create(const swsph_props *props, swsph_engine **out);
 int swsph_engine_destroy(swsph_engine *e);
 int swsph_set_props(swsph_engine *e, const swsph_props *props);
 
-/* Bins the particles into the top-level grid (counting sort by cell) and
- * uploads them.  Replaces space_rebuild's particle sort for this path. */
+/* Uploads the particles (arrays in the caller's order, positions inside the
+ * box) and bins them into the top-level grid on the device: sorted by cell
+ * and by a Morton code inside each cell.  Replaces space_rebuild's particle
+ * sort for this path.  Starts a new time line for swsph_engine_step. */
 int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p);
 int swsph_get_field(swsph_engine *e, int field, void *host_out);
 int swsph_num_cells(swsph_engine *e, int *ncells);

@@ -2,12 +2,13 @@
 // include/swsph.h.
 //
 // The engine keeps the gas particles resident in HBM as a structure of
-// arrays sorted by top-level cell and issues the kernels of sph_kernels.hip
-// on one HIP stream.  The per-task entry points are the drop-in for SWIFT's
+// arrays sorted by top-level cell (binned on the device, sph_step.hip) and
+// issues the kernels of sph_kernels.hip and sph_step.hip on one HIP stream.  The per-task entry points are the drop-in for SWIFT's
 // runner task functions; swsph_pass_plain drives them in the reference's
 // dependency order (src/engine_maketasks.c:2546-2573: density -> ghost ->
 // gradient -> extra ghost -> force -> end force), swsph_pass_batched runs
-// the same pass as a few launches over all cells.
+// the same pass as a few launches over all cells, and swsph_engine_step
+// advances the gas in time around it (drift, kicks, time steps).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -3,12 +3,14 @@
 // src/runner_doiact_functions_hydro.h:1234, 2299, 1601, 2624).
 //
 // Particles are sorted by top-level cell and, inside a cell, along a Morton
-// curve, then cut into chunks of C consecutive particles (C = 8 ... 64), so a
-// chunk is spatially compact whatever the local density.  Every chunk has a
-// bounding box and the largest search radius of its particles.
+// curve, then cut into chunks of CJ consecutive particles (CJ = 8 ... 64), so
+// a chunk is spatially compact whatever the local density.  Every chunk has
+// a bounding box and the largest search radius of its particles.
 //
-// One wave64 owns one i-chunk: lane = g + C * s holds particle g of the
-// chunk and is "stream" s of S = 64 / C streams.  The wave first builds its
+// One wave64 owns one i-chunk or tile of CI particles: lane = g + CI * s
+// holds particle g and is "stream" s of S = 64 / CI streams (CJ = CI for
+// i-chunks taken from the chunk table; a work-queue tile may face candidate
+// chunks of half its size).  The wave first builds its
 // neighbour table (the 27 neighbour cells, their images and chunk ranges:
 // one lane per cell, all loads in flight together), then culls the flat list
 // of candidate chunks 64 at a time (one chunk per lane: box distance against
@@ -18,8 +20,9 @@
 // g.  The row test is an FP32 pre-test on cell-relative positions whose
 // radius is widened by a proven bound on its difference from the pair
 // contract (pretest_radius2), so it never rejects a neighbour.  A pass sets
-// one bit of the lane's 64-bit row mask; a per-chunk table in LDS remembers
-// where each chunk starts and which image it uses.  After 64 rows every lane
+// one bit of the lane's row masks (kMasks x 64 bits); a per-chunk table in
+// LDS remembers where each chunk starts and which image it uses.  After
+// 64 x kMasks rows every lane
 // walks its own set bits, gathers its j record, evaluates the exact FP64
 // contract and test, and runs the interaction body, so the expensive bodies
 // run dense although only a few per cent of the candidates are neighbours.

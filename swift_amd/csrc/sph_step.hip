@@ -1,5 +1,5 @@
 // sph_step.hip -- what changes between two neighbour passes, on the device:
-// binning of the particles into the top-level grid (the counting sort of
+// binning of the particles into the top-level grid (the particle sort of
 // space_rebuild plus the in-cell order of the chunk engine) and the time
 // integration of the gas (drift, kicks, time steps).
 //
