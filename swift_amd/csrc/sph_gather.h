@@ -92,6 +92,14 @@ constexpr unsigned int kRowMask = (1u << 27) - 1u;
 #endif
 constexpr int kMasks = SWSPH_MASKS;
 
+// Candidate pre-test of the density and gradient loops in packed FP16 (two
+// rows per instruction) when a chunk has an even number of rows; 0 keeps the
+// FP32 pre-test everywhere.
+#ifndef SWSPH_F16_PRETEST
+#define SWSPH_F16_PRETEST 1
+#endif
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+
 // Waves per block of the batched kernels.  Each wave works on its own chunk
 // or tile with its own LDS.  Several waves per block lift the per-CU
 // workgroup limit off the occupancy, but measured slower on the 128^3 box
@@ -165,6 +173,9 @@ struct GatherLds {
   static constexpr int RPC = CJ / S;        // rows per chunk
   static constexpr int SLOTS = 64 * kMasks / RPC;  // chunks per phase
   PosRec lpos[64];
+  // FP16 pre-test: staged candidates in the tile frame, rows paired
+  _Float16 hx[64], hy[64], hz[64];
+  float qoffc[27 * 3];  // neighbour image corners relative to the tile centre
   double ttab[27 * 3];
   double qorg[27 * 3];  // corner of each neighbour's image
   unsigned int tcode[27];
@@ -344,6 +355,43 @@ __device__ __forceinline__ void run_chunk(
     ihy = bb[4];
     ihz = bb[5];
   }
+  // FP16 pre-test frame: the centre of the i-box, lengths in units of the
+  // largest search radius R of the tile.  Every rounding between the
+  // positions and the half-precision squared distance is bounded by a few
+  // units of 2^-11 of the coordinates' magnitude (at most V + 1.1 for the
+  // candidates within reach, V the largest |i coordinate|), so widening
+  // each radius r to (r + 2e)^2 (1 + 8 2^-11), e = 2 (V + 1.1) 2^-11, never
+  // rejects a neighbour of the pair contract.
+  // (the force loop's symmetric test needs a widened radius per candidate
+  // as well, which costs more than the packed rows save: it stays FP32)
+  constexpr bool kF16 =
+      SWSPH_F16_PRETEST && (RPC % 2 == 0) && !L::kSymmetric;
+  constexpr float kU16 = 1.f / 2048.f;
+  float invR = 0.f, eps16 = 0.f;
+  _Float16 ixh = 0, iyh = 0, izh = 0, rt2h = -1;
+  if (kF16) {
+    const double cx = 0.5 * (ilx + ihx), cy = 0.5 * (ily + ihy),
+                 cz = 0.5 * (ilz + ihz);
+    invR = 1.f / sqrtf(hg2i);
+    if (lane < nq) {
+      sh.qoffc[3 * lane] = (float)(sh.qorg[3 * lane] - cx);
+      sh.qoffc[3 * lane + 1] = (float)(sh.qorg[3 * lane + 1] - cy);
+      sh.qoffc[3 * lane + 2] = (float)(sh.qorg[3 * lane + 2] - cz);
+    }
+    const float fx = (float)(st.x - cx) * invR;
+    const float fy = (float)(st.y - cy) * invR;
+    const float fz = (float)(st.z - cz) * invR;
+    ixh = (_Float16)fx;
+    iyh = (_Float16)fy;
+    izh = (_Float16)fz;
+    const float vbound =
+        1.001f * wave_max(act ? fmaxf(fabsf(fx), fmaxf(fabsf(fy), fabsf(fz)))
+                              : 0.f);
+    eps16 = 2.f * (vbound + 1.1f) * kU16 + 1e-6f;
+    const float r = sqrtf(rt2 * invR * invR) + 2.f * eps16;
+    const float b = r * r * (1.f + 8.f * kU16) * (1.f + 1.f / 1024.f);
+    rt2h = act ? (_Float16)fminf(b, 65504.f) : (_Float16)(-1.f);
+  }
   wave_sync();
 
   unsigned long long m0 = 0ull, m1 = 0ull;  // row hits of this lane
@@ -390,7 +438,16 @@ __device__ __forceinline__ void run_chunk(
             p.x = p.y = p.z = -1e30f;
             p.hg2m = -1.f;
           }
-          sh.lpos[lane] = p;
+          if (kF16) {
+            // slot of row pair (rr / 2) of stream sp, lane rr % 2 of the pair
+            const int jl = lane % CJ, rr = jl / S, sp = jl % S;
+            const int dst = ((c * (RPC / 2) + (rr >> 1)) * S + sp) * 2 + (rr & 1);
+            sh.hx[dst] = (_Float16)((p.x + sh.qoffc[3 * q]) * invR);
+            sh.hy[dst] = (_Float16)((p.y + sh.qoffc[3 * q + 1]) * invR);
+            sh.hz[dst] = (_Float16)((p.z + sh.qoffc[3 * q + 2]) * invR);
+          } else {
+            sh.lpos[lane] = p;
+          }
         }
       }
       wave_sync();
@@ -413,16 +470,34 @@ __device__ __forceinline__ void run_chunk(
         using Nib = typename std::conditional<(RPC > 32), unsigned long long,
                                               unsigned int>::type;
         Nib nib = 0;
+        if (kF16) {
+          const half2v ix2 = {ixh, ixh}, iy2 = {iyh, iyh}, iz2 = {izh, izh};
 #pragma unroll
-        for (int rr = 0; rr < RPC; rr++) {
-          if (rr * S >= nj) break;  // uniform
-          const int jl = rr * S + s;
-          const PosRec p = sh.lpos[c * CJ + jl];
-          const float dx = xs - p.x, dy = ys - p.y, dz = zs - p.z;
-          const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-          const bool pass =
-              L::kSymmetric ? (r2 < rt2m || r2 < p.hg2m) : r2 < rt2m;
-          nib |= (pass && jl != selfl) ? ((Nib)1 << rr) : (Nib)0;
+          for (int pr = 0; pr < RPC / 2; pr++) {
+            if (2 * pr * S >= nj) break;  // uniform
+            const int idx = 2 * ((c * (RPC / 2) + pr) * S + s);
+            const half2v x2 = *reinterpret_cast<const half2v *>(&sh.hx[idx]);
+            const half2v y2 = *reinterpret_cast<const half2v *>(&sh.hy[idx]);
+            const half2v z2 = *reinterpret_cast<const half2v *>(&sh.hz[idx]);
+            const half2v dx = ix2 - x2, dy = iy2 - y2, dz = iz2 - z2;
+            const half2v r2 = dz * dz + (dy * dy + dx * dx);
+            const bool p0 = r2.x < rt2h, p1 = r2.y < rt2h;
+            const int jl0 = 2 * pr * S + s, jl1 = jl0 + S;
+            nib |= (p0 && jl0 != selfl) ? ((Nib)1 << (2 * pr)) : (Nib)0;
+            nib |= (p1 && jl1 != selfl) ? ((Nib)1 << (2 * pr + 1)) : (Nib)0;
+          }
+        } else {
+#pragma unroll
+          for (int rr = 0; rr < RPC; rr++) {
+            if (rr * S >= nj) break;  // uniform
+            const int jl = rr * S + s;
+            const PosRec p = sh.lpos[c * CJ + jl];
+            const float dx = xs - p.x, dy = ys - p.y, dz = zs - p.z;
+            const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+            const bool pass =
+                L::kSymmetric ? (r2 < rt2m || r2 < p.hg2m) : r2 < rt2m;
+            nib |= (pass && jl != selfl) ? ((Nib)1 << rr) : (Nib)0;
+          }
         }
         if (kMasks == 1 || nrow < 64)  // uniform
           m0 |= (unsigned long long)nib << nrow;

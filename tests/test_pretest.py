@@ -79,3 +79,64 @@ def test_pretest_never_rejects_a_contract_neighbour():
                 assert np.all(passed[inside]), (dim, nc, hfrac)
                 missed_without_margin += int((inside & ~passed_nomargin).sum())
     assert missed_without_margin > 0
+
+
+def _f16_pretest_case(rng, n_tiles=150):
+    """The packed-FP16 pre-test of the density / gradient loops
+    (swift_amd/csrc/sph_gather.h): positions relative to the centre of the
+    tile's box in units of its largest search radius R, rounded to half; the
+    squared distance in half arithmetic (with and without fused
+    multiply-adds); each radius widened to (r + 2e)^2 (1 + 8u)(1 + 2^-10),
+    e = 2 (V + 1.1) u + 1e-6, u = 2^-11, V = 1.001 max |i coordinate|."""
+    f16, f32 = np.float16, np.float32
+    u = 2.0 ** -11
+    missed, missed_bare, total = 0, 0, 0
+    for _ in range(n_tiles):
+        w = 10 ** rng.uniform(-3, 3)
+        H = w * 10 ** rng.uniform(-2.5, 0)
+        a = H * 10 ** rng.uniform(-1, 1.5)          # tile extent
+        ct = rng.uniform(0, w, 3) + w
+        xi = ct + rng.uniform(-a / 2, a / 2, (16, 3))
+        Hi = H * rng.uniform(0.5, 1.0, 16)
+        Hi[0] = H
+        hg2 = (f32(Hi) * f32(Hi)).astype(f32)
+        invR = f32(1.0) / np.sqrt(hg2.max()).astype(f32)
+        fx = ((xi - ct).astype(f32) * invR).astype(f32)
+        irel = fx.astype(f16)
+        vb = f32(1.001) * np.abs(fx).max()
+        e = f32(2.0) * (vb + f32(1.1)) * f32(u) + f32(1e-6)
+        n = 3000
+        k = rng.integers(0, 16, n)
+        d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1)[:, None]
+        r = Hi[k] * (1 - rng.uniform(0, 2e-3, n))
+        xj = xi[k] + d * r[:, None]
+        corner = np.floor(xj / w) * w
+        pj = (xj - corner).astype(f32)
+        qoff = (corner - ct).astype(f32)
+        jrel = ((pj + qoff).astype(f32) * invR).astype(f16)
+        dd = (irel[k] - jrel).astype(f16).astype(np.float64)
+        # separate roundings, and fused multiply-adds
+        t = f16(dd[:, 0] ** 2).astype(np.float64)
+        t = f16(t + f16(dd[:, 1] ** 2)).astype(np.float64)
+        r2a = f16(t + f16(dd[:, 2] ** 2)).astype(np.float64)
+        t = f16(dd[:, 0] ** 2).astype(np.float64)
+        t = f16(dd[:, 1] ** 2 + t).astype(np.float64)
+        r2b = f16(dd[:, 2] ** 2 + t).astype(np.float64)
+        dc = (xi[k] - xj).astype(f32)
+        inside = _r2(dc[:, 0], dc[:, 1], dc[:, 2]) < hg2[k]
+        rn = np.sqrt(hg2[k] * invR * invR) + 2 * e
+        bnd = f16(np.minimum(rn * rn * (1 + 8 * u) * (1 + 2.0 ** -10),
+                             65504)).astype(np.float64)
+        bare = f16(hg2[k] * invR * invR).astype(np.float64)
+        missed += int((inside & ~((r2a < bnd) & (r2b < bnd))).sum())
+        missed_bare += int((inside & ~(r2a < bare)).sum())
+        total += int(inside.sum())
+    return missed, missed_bare, total
+
+
+def test_f16_pretest_never_rejects_a_contract_neighbour():
+    missed, missed_bare, total = _f16_pretest_case(np.random.default_rng(11))
+    assert total > 100000
+    assert missed == 0
+    assert missed_bare > 0  # the widening is needed
