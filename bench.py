@@ -326,6 +326,9 @@ def main() -> int:
                     help="host threads for the CPU baseline (0: min(16, cores))")
     ap.add_argument("--cpu-n", type=int, default=0,
                     help="particles per axis of the CPU sample (0: same as --n)")
+    ap.add_argument("--cpu-n1", type=int, default=0,
+                    help="particles per axis of the single-thread CPU sample "
+                         "(0: half of the threaded sample's)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default schedule and tile shapes")
     ap.add_argument("--graph", type=int, default=1,
@@ -525,19 +528,31 @@ def main() -> int:
         ncpu = os.cpu_count() or 1
         threads = args.cpu_threads or min(16, ncpu)
         cn = args.cpu_n or args.n
-        cst = ics.modulated_box(cn, seed=0, amp=0.5)
-        cprops = HydroProps(ncell=(max(3, args.ncell * cn // args.n),) * 3)
-        ts = time.perf_counter()
-        o = orc.pass_fp32(cprops, cst, nthreads=threads)
-        dt = time.perf_counter() - ts
-        cinter = int(o["n_density"].astype(np.int64).sum()) + int(
-            o["n_force"].astype(np.int64).sum())
+
+        def cpu_pass(n_side, nthreads):
+            cs = ics.modulated_box(n_side, seed=0, amp=0.5)
+            cp = HydroProps(ncell=(max(3, args.ncell * n_side // args.n),) * 3)
+            ts = time.perf_counter()
+            o = orc.pass_fp32(cp, cs, nthreads=nthreads, port=True)
+            dt = time.perf_counter() - ts
+            ci = int(o["n_density"].astype(np.int64).sum()) + int(
+                o["n_force"].astype(np.int64).sum())
+            return ci / dt, dt
+
+        cval, dt = cpu_pass(cn, threads)
+        cn1 = args.cpu_n1 or max(8, cn // 2)
+        cval1, dt1 = cpu_pass(cn1, 1)
         out["cpu_baseline"] = {
-            "value": cinter / dt, "unit": "interactions/s", "cores": threads,
+            "value": cval, "unit": "interactions/s", "cores": threads,
             "kind": "port",
             "sample": f"one full pass (density+ghost+gradient+force) over the "
                       f"{cn}^3 modulated box, {dt:.1f} s wall on {threads} "
-                      "host threads (oracle FP32 restatement, OpenMP)"}
+                      "host threads (the oracle's FP32 restatement with sorted "
+                      "per-axis candidate lists, -O3 -ffast-math, OpenMP over "
+                      "cells)",
+            "single_thread": {"value": cval1, "unit": "interactions/s",
+                              "sample": f"{cn1}^3 modulated box, {dt1:.1f} s "
+                                        "wall on 1 thread"}}
     print(json.dumps(out))
     return 0
 

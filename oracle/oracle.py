@@ -13,11 +13,24 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = os.path.join(_HERE, "liborc.so")
+# The threaded CPU baseline: the same FP32 restatement with sorted per-axis
+# candidate lists and SWIFT's production flags (timed, never the checker).
+_PORT = os.path.join(_HERE, "libcpuport.so")
 _lib = None
+_port = None
 
 
 def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def port_lib():
+    global _port
+    if _port is None:
+        if not os.path.exists(_PORT):
+            build()
+        _port = C.CDLL(_PORT)
+    return _port
 
 
 def lib():
@@ -135,9 +148,11 @@ def pass_fp64(props, st: dict, nthreads: int = 0) -> dict:
     return out
 
 
-def pass_fp32(props, st: dict, nthreads: int = 0) -> dict:
+def pass_fp32(props, st: dict, nthreads: int = 0, port: bool = False) -> dict:
     """FP32 restatement of the full pass including the ghost; updates the
-    (io) fields of ``st`` in place (h converged) and returns the outputs."""
+    (io) fields of ``st`` in place (h converged) and returns the outputs.
+    ``port`` runs the CPU-baseline build (sorted candidate lists, -O3
+    -ffast-math) instead of the oracle proper."""
     n = st["h"].shape[0]
     out = {}
     o = OrcOut32()
@@ -147,7 +162,8 @@ def pass_fp32(props, st: dict, nthreads: int = 0) -> dict:
         setattr(o, k, out[k].ctypes.data)
     cfg = make_config(props, nthreads)
     p, _keep = _parts(st)
-    rc = lib().orc_pass_fp32(C.byref(cfg), C.byref(p), C.byref(o))
+    rc = (port_lib() if port else lib()).orc_pass_fp32(C.byref(cfg), C.byref(p),
+                                                       C.byref(o))
     if rc != 0:
         raise RuntimeError(f"orc_pass_fp32 failed ({rc})")
     for k in ("rho", "pressure", "soundspeed", "f", "balsara", "alpha_visc",

@@ -8,6 +8,14 @@
  * the SWIFT source tree).  Build: oracle/Makefile (-O2 -ffp-contract=off, so
  * that every FP32 expression rounds exactly as written; the FMAs of the pair
  * contract are explicit fmaf calls).
+ *
+ * The same file, built with -DORC_SORTED and SWIFT's production flags
+ * (-O3 -ffast-math -funroll-loops), is the threaded CPU baseline
+ * (libcpuport.so): its candidate loops walk per-cell lists sorted along the
+ * 13 cell-pair axes and stop at the projected distance, as the sorted
+ * DOPAIR loops do (runner_doiact_functions_hydro.h:1297-1414, with the sort
+ * of runner_sort.c:203-473).  It is timed as the CPU baseline and checked
+ * against the oracle proper; it never serves as the checker.
  */
 #include "sphenix_oracle.h"
 
@@ -273,6 +281,13 @@ typedef struct grid {
   int *cell_of;  /* [n] */
   int *start;    /* [ncells+1] */
   int *index;    /* [n] particle ids sorted by cell */
+#ifdef ORC_SORTED
+  /* per axis s (13 cell-pair directions): each cell's particle ids sorted by
+   * the projection x . n_s, and the projections in that order */
+  int *sidx[13];
+  double *skey[13];
+  float *hgmax; /* [ncells] largest search radius gamma h of each cell */
+#endif
 } grid;
 
 static int cell_coord(double x, double inv_w, int n) {
@@ -292,6 +307,13 @@ static int grid_build(const orc_config *cfg, const orc_parts *p, grid *g) {
     g->w[k] = cfg->dim[k] / g->nc[k];
   }
   g->ncells = g->nc[0] * g->nc[1] * g->nc[2];
+#ifdef ORC_SORTED
+  for (int s = 0; s < 13; s++) {
+    g->sidx[s] = NULL;
+    g->skey[s] = NULL;
+  }
+  g->hgmax = NULL;
+#endif
   g->cell_of = (int *)malloc(sizeof(int) * (size_t)(p->n > 0 ? p->n : 1));
   g->start = (int *)calloc((size_t)g->ncells + 1, sizeof(int));
   g->index = (int *)malloc(sizeof(int) * (size_t)(p->n > 0 ? p->n : 1));
@@ -318,7 +340,91 @@ static void grid_free(grid *g) {
   free(g->cell_of);
   free(g->start);
   free(g->index);
+#ifdef ORC_SORTED
+  for (int s = 0; s < 13; s++) {
+    free(g->sidx[s]);
+    free(g->skey[s]);
+  }
+  free(g->hgmax);
+#endif
 }
+
+#ifdef ORC_SORTED
+/* Unit vector of stencil position q (0..26, ox-major; 13 is the cell itself):
+ * offset (q / 9 - 1, q / 3 % 3 - 1, q % 3 - 1) normalised.  The 13 positions
+ * q > 13 are the sort axes (SWIFT's sid with flip folds the other 13 onto
+ * them, sort_part.h:32-93). */
+static void axis_of(int q, double n[3]) {
+  const int o[3] = {q / 9 - 1, q / 3 % 3 - 1, q % 3 - 1};
+  const double len = sqrt((double)(o[0] * o[0] + o[1] * o[1] + o[2] * o[2]));
+  for (int k = 0; k < 3; k++) n[k] = o[k] / len;
+}
+
+typedef struct skey_entry {
+  double d;
+  int i;
+} skey_entry;
+
+static int skey_cmp(const void *a, const void *b) {
+  const double da = ((const skey_entry *)a)->d, db = ((const skey_entry *)b)->d;
+  return (da > db) - (da < db);
+}
+
+/* runner_do_hydro_sort (runner_sort.c:203-473): every cell sorted along
+ * every axis. */
+static int grid_sort(const orc_parts *p, grid *g) {
+  const size_t n = (size_t)(p->n > 0 ? p->n : 1);
+  g->hgmax = (float *)calloc((size_t)g->ncells, sizeof(float));
+  if (!g->hgmax) return 1;
+  for (int s = 0; s < 13; s++) {
+    g->sidx[s] = (int *)malloc(sizeof(int) * n);
+    g->skey[s] = (double *)malloc(sizeof(double) * n);
+    if (!g->sidx[s] || !g->skey[s]) return 1;
+  }
+#pragma omp parallel
+  {
+    int cap = 0;
+    skey_entry *buf = NULL;
+#pragma omp for schedule(dynamic, 16)
+    for (int c = 0; c < g->ncells; c++) {
+      const int a = g->start[c], e = g->start[c + 1];
+      if (e - a > cap) {
+        cap = e - a;
+        buf = (skey_entry *)realloc(buf, sizeof(skey_entry) * (size_t)cap);
+      }
+      for (int s = 0; s < 13; s++) {
+        double nv[3];
+        axis_of(14 + s, nv);
+        for (int k = a; k < e; k++) {
+          const int i = g->index[k];
+          buf[k - a].i = i;
+          buf[k - a].d = p->x[3 * i] * nv[0] + p->x[3 * i + 1] * nv[1] +
+                         p->x[3 * i + 2] * nv[2];
+        }
+        qsort(buf, (size_t)(e - a), sizeof(skey_entry), skey_cmp);
+        for (int k = a; k < e; k++) {
+          g->sidx[s][k] = buf[k - a].i;
+          g->skey[s][k] = buf[k - a].d;
+        }
+      }
+    }
+    free(buf);
+  }
+  return 0;
+}
+
+/* Largest search radius of every cell (the hj_max of DOPAIR2). */
+static void grid_hmax(const orc_parts *p, grid *g) {
+#pragma omp parallel for schedule(static)
+  for (int c = 0; c < g->ncells; c++) {
+    float m = 0.f;
+    for (int k = g->start[c]; k < g->start[c + 1]; k++)
+      m = fmaxf(m, KG * p->h[g->index[k]]);
+    g->hgmax[c] = m;
+  }
+}
+#endif
+
 
 /* The 27 neighbour cells of cell cid with the image translation t applied
  * to positions of j (x_j + t is the image nearest to cell cid). */
@@ -354,6 +460,56 @@ static void neighbours27(const grid *g, const double dim[3], int cid,
         out[m].cell = (cc[0] * g->nc[1] + cc[1]) * g->nc[2] + cc[2];
         m++;
       }
+}
+
+/* Candidates j of particle i (position xi) in stencil neighbour q with image
+ * translation b->t, given the search radius R: ids idx[a..e).  The oracle
+ * proper takes the whole cell.  The sorted build keeps the prefix (or
+ * suffix, for the flipped axes) of the cell's sorted list whose projected
+ * distance (x_j + t - x_i) . n is below R (DOPAIR1: sort_j[pjd].d < di,
+ * runner_doiact_functions_hydro.h:1322-1332); the projection never exceeds
+ * the distance, and the 1e-4 widening covers the FP32 rounding of r2. */
+static inline void cand_range(const grid *g, const nbr *b, int q,
+                              const double *xi, float R, const int **idx,
+                              int *a, int *e) {
+  const int c = b->cell;
+  *a = g->start[c];
+  *e = g->start[c + 1];
+  *idx = g->index;
+#ifdef ORC_SORTED
+  if (q == 13) return;
+  const int qa = q > 13 ? q : 26 - q;
+  const int s = qa - 14;
+  double nv[3];
+  axis_of(qa, nv);
+  const double pi = xi[0] * nv[0] + xi[1] * nv[1] + xi[2] * nv[2];
+  const double pt = b->t[0] * nv[0] + b->t[1] * nv[1] + b->t[2] * nv[2];
+  const double Rw = (double)R * 1.0001 + 1e-12;
+  const double *key = g->skey[s];
+  *idx = g->sidx[s];
+  int lo = *a, hi = *e;
+  if (q > 13) {
+    /* keep key + pt - pi < Rw: the first index with key >= cut ends it */
+    const double cut = pi - pt + Rw;
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if (key[m] < cut) lo = m + 1; else hi = m;
+    }
+    *e = lo;
+  } else {
+    /* keep pi - (key + pt) < Rw: the first index with key > cut starts it */
+    const double cut = pi - pt - Rw;
+    while (lo < hi) {
+      const int m = (lo + hi) >> 1;
+      if (key[m] <= cut) lo = m + 1; else hi = m;
+    }
+    *a = lo;
+  }
+#else
+  (void)q;
+  (void)xi;
+  (void)R;
+#endif
 }
 
 static int is_active(const orc_config *cfg, int8_t tb) {
@@ -748,8 +904,11 @@ static void density32(const orc_config *cfg, const grid *g, const orc_parts *p,
   const float h_inv = 1.f / hi;
   for (int q = 0; q < 27; q++) {
     const nbr *b = &nb[q];
-    for (int jj = g->start[b->cell]; jj < g->start[b->cell + 1]; jj++) {
-      const int j = g->index[jj];
+    const int *jl;
+    int ja, je;
+    cand_range(g, b, q, &p->x[3 * i], KG * hi, &jl, &ja, &je);
+    for (int jj = ja; jj < je; jj++) {
+      const int j = jl[jj];
       if (j == i || is_inhibited(p->time_bin[j])) continue;
       float dx[3], r2;
       orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], b->t, dx, &r2);
@@ -782,9 +941,12 @@ static void density32(const orc_config *cfg, const grid *g, const orc_parts *p,
 
 int orc_pass_fp32(const orc_config *cfg, orc_parts *p, orc_out32 *o) {
   grid g;
-  if (grid_build(cfg, p, &g)) return 1;
 #ifdef _OPENMP
   if (cfg->nthreads > 0) omp_set_num_threads(cfg->nthreads);
+#endif
+  if (grid_build(cfg, p, &g)) return 1;
+#ifdef ORC_SORTED
+  if (grid_sort(p, &g)) return 1;
 #endif
   const int n = p->n;
   const float eta_dim = cfg->eta * cfg->eta * cfg->eta;
@@ -960,8 +1122,11 @@ int orc_pass_fp32(const orc_config *cfg, orc_parts *p, orc_out32 *o) {
       int cnt = 0;
       for (int q = 0; q < 27; q++) {
         const nbr *b = &nb[q];
-        for (int jj = g.start[b->cell]; jj < g.start[b->cell + 1]; jj++) {
-          const int j = g.index[jj];
+        const int *jl;
+        int ja, je;
+        cand_range(&g, b, q, &p->x[3 * i], KG * hi, &jl, &ja, &je);
+        for (int jj = ja; jj < je; jj++) {
+          const int j = jl[jj];
           if (j == i || is_inhibited(p->time_bin[j])) continue;
           float dx[3], r2;
           orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], b->t, dx, &r2);
@@ -1034,6 +1199,9 @@ int orc_pass_fp32(const orc_config *cfg, orc_parts *p, orc_out32 *o) {
   }
 
   /* Force loop (runner_iact_nonsym_force + timebin) + hydro_end_force. */
+#ifdef ORC_SORTED
+  grid_hmax(p, &g);
+#endif
 #pragma omp parallel for schedule(dynamic, 1)
   for (int cid = 0; cid < g.ncells; cid++) {
     nbr nb[27];
@@ -1062,8 +1230,16 @@ int orc_pass_fp32(const orc_config *cfg, orc_parts *p, orc_out32 *o) {
       pi[P_MINTB] = NUM_TIME_BINS + 1;
       for (int q = 0; q < 27; q++) {
         const nbr *b = &nb[q];
-        for (int jj = g.start[b->cell]; jj < g.start[b->cell + 1]; jj++) {
-          const int j = g.index[jj];
+        const int *jl;
+        int ja, je;
+#ifdef ORC_SORTED
+        const float Rq = fmaxf(KG * hi, g.hgmax[b->cell]);
+#else
+        const float Rq = KG * hi;
+#endif
+        cand_range(&g, b, q, &p->x[3 * i], Rq, &jl, &ja, &je);
+        for (int jj = ja; jj < je; jj++) {
+          const int j = jl[jj];
           if (j == i || is_inhibited(p->time_bin[j])) continue;
           float dx[3], r2;
           orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], b->t, dx, &r2);
