@@ -110,17 +110,27 @@ typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 constexpr int kWaves = SWSPH_WAVES_PER_BLOCK;
 
 // Launch bounds of the batched neighbour kernels: block size and, unless
-// SWSPH_NO_OCC_TARGET is defined, the loop's occupancy target.
+// SWSPH_NO_OCC_TARGET is defined, the loop's occupancy target (its own one
+// for the one-tile-per-wave kernel).
 #ifdef SWSPH_NO_OCC_TARGET
 #define SWSPH_LOOP_BOUNDS(L) __launch_bounds__(64 * kWaves)
+#define SWSPH_TILE_BOUNDS(L) __launch_bounds__(64 * kWaves)
 #else
 #define SWSPH_LOOP_BOUNDS(L) __launch_bounds__(64 * kWaves, L::kWavesPerSimd)
+#define SWSPH_TILE_BOUNDS(L) \
+  __launch_bounds__(64 * kWaves, L::kTileWavesPerSimd)
 #endif
 
 // A value every lane of the wave holds (e.g. read from LDS at a common
 // address), moved to a scalar register so that branches on it stay uniform.
 __device__ __forceinline__ int uniform(int v) {
   return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ double uniform(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)b);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+  return __hiloint2double(hi, lo);
 }
 
 // Orders LDS accesses of one wave around a point where lanes exchange data
@@ -346,6 +356,13 @@ __device__ __forceinline__ void run_chunk(
       ihy = fmax(ihy, __shfl_xor(ihy, off));
       ihz = fmax(ihz, __shfl_xor(ihz, off));
     }
+    // every lane holds the tile's box: keep it in scalar registers
+    ilx = uniform(ilx);
+    ily = uniform(ily);
+    ilz = uniform(ilz);
+    ihx = uniform(ihx);
+    ihy = uniform(ihy);
+    ihz = uniform(ihz);
   } else {
     const double *bb = ca.bbox + 6 * ich;
     ilx = bb[0];
@@ -583,7 +600,7 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_queue(
   for (int tries = 0; tries < 8;) {
     int base = 0;
     if ((threadIdx.x & 63) == 0) base = atomicAdd(&tq.heads[64 * q], kGrab);
-    base = __shfl(base, 0);
+    base = uniform(__shfl(base, 0));
     const int lo = q * per + base;
     const int hi = min(nt, (q + 1) * per);
     if (lo >= hi) {  // uniform: this queue is exhausted
@@ -603,7 +620,7 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_queue(
 // Grid form of the tile queue: one wave per tile over a grid sized for the
 // largest possible tile count; waves past the device-side count exit.
 template <class L, int CI, int CJ, bool BUILD>
-__global__ SWSPH_LOOP_BOUNDS(L) void k_loop_tiles(
+__global__ SWSPH_TILE_BOUNDS(L) void k_loop_tiles(
     DevParts d, LoopArgs la, ChunkArgs ca,
     const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
     NbrList nl, TileArgs tq) {

@@ -16,6 +16,12 @@
 #ifndef SWSPH_OCC_FORCE
 #define SWSPH_OCC_FORCE 5
 #endif
+// The one-tile-per-wave kernel needs fewer registers than the grid and
+// persistent forms: its force loop fits 6 waves per SIMD without spilling
+// (the others spill 16-31 VGPRs there).
+#ifndef SWSPH_OCC_FORCE_TILES
+#define SWSPH_OCC_FORCE_TILES 6
+#endif
 
 namespace sphl {
 using namespace sph;
@@ -126,6 +132,7 @@ struct DensI {
 struct DensityLoop {
   // occupancy target of the batched kernels (waves per SIMD)
   static constexpr int kWavesPerSimd = SWSPH_OCC_DENSITY;
+  static constexpr int kTileWavesPerSimd = SWSPH_OCC_DENSITY;
   using Rec = DensRec;
   using Acc = DensAcc;
   using I = DensI;
@@ -226,6 +233,7 @@ struct GradI {
 struct GradientLoop {
   // occupancy target of the batched kernels (waves per SIMD)
   static constexpr int kWavesPerSimd = SWSPH_OCC_GRADIENT;
+  static constexpr int kTileWavesPerSimd = SWSPH_OCC_GRADIENT;
   using Rec = GradRec;
   using Acc = GradAcc;
   using I = GradI;
@@ -307,6 +315,7 @@ struct ForceI {
 struct ForceLoop {
   // occupancy target of the batched kernels (waves per SIMD)
   static constexpr int kWavesPerSimd = SWSPH_OCC_FORCE;
+  static constexpr int kTileWavesPerSimd = SWSPH_OCC_FORCE_TILES;
   using Rec = ForceRec;
   using Acc = ForceAcc;
   using I = ForceI;
