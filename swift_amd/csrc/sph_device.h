@@ -98,4 +98,31 @@ __device__ __forceinline__ float fast_sqrt(float x) {
   return __builtin_amdgcn_sqrtf(x);
 }
 
+// Per-particle kernels over [lo, hi): a block covers kRuns consecutive runs
+// of blockDim.x particles (launch run_blocks(hi - lo, blockDim.x) blocks).
+// The keys (time bin, redo flag) of all runs are loaded before any particle
+// is processed, so every thread has kRuns loads in flight: with one byte
+// load per thread a sweep over every particle slot is latency-bound, which
+// is most of the cost of a step with few active particles.
+constexpr int kRuns = 8;
+__host__ __device__ inline int run_blocks(int n, int block) {
+  return n > 0 ? (n + block * kRuns - 1) / (block * kRuns) : 0;
+}
+template <class K, class F>
+__device__ __forceinline__ void for_runs(int lo, int hi,
+                                         const K *__restrict__ key, F f) {
+  const int base = lo + blockIdx.x * blockDim.x * kRuns + threadIdx.x;
+  K k[kRuns];
+#pragma unroll
+  for (int u = 0; u < kRuns; u++) {
+    const int i = base + u * (int)blockDim.x;
+    k[u] = i < hi ? key[i] : K{};
+  }
+#pragma unroll
+  for (int u = 0; u < kRuns; u++) {
+    const int i = base + u * (int)blockDim.x;
+    if (i < hi) f(i, k[u]);
+  }
+}
+
 }  // namespace sph

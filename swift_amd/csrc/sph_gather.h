@@ -196,7 +196,13 @@ struct GatherLds {
 
 // Interaction phase over the rows recorded in the lanes' masks.  Called from
 // converged code; the per-lane loop below is divergent, so everything it
-// shares is in LDS.
+// shares is in LDS.  The density walk is software pipelined (the next
+// hit's record is loaded before the current hit is evaluated, so its latency
+// overlaps the interaction body); measured slower for the gradient loop's
+// larger records and register budget, so the others walk one hit at a time.
+#ifndef SWSPH_WALK_PREFETCH
+#define SWSPH_WALK_PREFETCH 1
+#endif
 template <class L, int CI, int CJ, bool BUILD>
 __device__ __forceinline__ void interact_rows(
     const typename L::Rec *__restrict__ rec, const GatherLds<CI, CJ> &sh,
@@ -204,39 +210,79 @@ __device__ __forceinline__ void interact_rows(
     unsigned long long m1, typename L::Acc &acc, const NbrList &nl, int i,
     float rt2, int &ncnt) {
   constexpr int S = GatherLds<CI, CJ>::S, RPC = GatherLds<CI, CJ>::RPC;
-  while (kMasks == 1 ? m0 != 0ull : (m0 | m1) != 0ull) {
+  // next row of this lane (ascending), taken out of the masks
+  auto pop = [&]() -> int {
     int row;
     if (kMasks == 1) {
-      row = __builtin_ctzll(m0);
+      row = m0 ? __builtin_ctzll(m0) : 0;
       m0 &= m0 - 1ull;
     } else {
       const bool lo = m0 != 0ull;
       const unsigned long long mm = lo ? m0 : m1;
-      row = __builtin_ctzll(mm) + (lo ? 0 : 64);
+      row = (mm ? __builtin_ctzll(mm) : 0) + (lo ? 0 : 64);
       const unsigned long long rest = mm & (mm - 1ull);
       m0 = lo ? rest : m0;
       m1 = lo ? m1 : rest;
     }
+    return row;
+  };
+  auto any = [&]() { return kMasks == 1 ? m0 != 0ull : (m0 | m1) != 0ull; };
+  // record index and image of a row
+  auto locate = [&](int row, int &q) -> int {
     const unsigned int info = sh.cinfo[row / RPC];
-    const int j = (int)(info & kRowMask) + (row % RPC) * S + s;
-    const int q = (int)(info >> 27);
+    q = (int)(info >> 27);
+    return (int)(info & kRowMask) + (row % RPC) * S + s;
+  };
+  using Rec = typename L::Rec;
+  // Evaluates hit (j, q) with record r.
+  auto eval = [&](int j, int q, const Rec &r) {
     const double xs = st.x - sh.ttab[3 * q];
     const double ys = st.y - sh.ttab[3 * q + 1];
     const double zs = st.z - sh.ttab[3 * q + 2];
-    const typename L::Rec &jr = rec[j];
     float dx, dy, dz;
-    const float r2 = pair_r2(xs, ys, zs, jr.x, jr.y, jr.z, dx, dy, dz);
+    const float r2 = pair_r2(xs, ys, zs, r.x, r.y, r.z, dx, dy, dz);
     if (BUILD) {
       if (r2 < rt2) {
         if (ncnt < nl.Ks)
           nl.idx[((size_t)ncnt * nl.npad + i) * kListStreams + s] =
               (sh.tcode[q] << 27) | (unsigned int)j;
         ncnt++;
-        if (L::test(r2, st, &jr)) L::interact(r2, dx, dy, dz, st, jr, acc);
+        if (L::test(r2, st, &r)) L::interact(r2, dx, dy, dz, st, r, acc);
       }
-    } else if (L::test(r2, st, &jr)) {
-      L::interact(r2, dx, dy, dz, st, jr, acc);
+    } else if (L::test(r2, st, &r)) {
+      L::interact(r2, dx, dy, dz, st, r, acc);
     }
+  };
+  if (!SWSPH_WALK_PREFETCH || !L::kPrefetchWalk) {
+    while (any()) {
+      int q;
+      const int j = locate(pop(), q);
+      eval(j, q, rec[j]);
+    }
+    return;
+  }
+  if (!any()) return;
+  // Pipelined: each step first takes the next row (if any) and loads its
+  // record into the other register set, then evaluates the current one.
+  // The loads are issued unconditionally (the current record again when no
+  // row is left), so the wait for the current record leaves the next one
+  // in flight; the loop is unrolled by two, so no copy waits for a load.
+  auto step = [&](int j, int q, const Rec &r, int &jn, int &qn,
+                  Rec &rn) -> bool {
+    const bool more = any();
+    const int row = pop();
+    int ql;
+    const int jl = locate(more ? row : 0, ql);
+    jn = more ? jl : j;
+    qn = more ? ql : q;
+    rn = rec[jn];
+    eval(j, q, r);
+    return more;
+  };
+  int ja, qa, jb = 0, qb = 0;
+  ja = locate(pop(), qa);
+  Rec ra = rec[ja], rb;
+  while (step(ja, qa, ra, jb, qb, rb) && step(jb, qb, rb, ja, qa, ra)) {
   }
 }
 

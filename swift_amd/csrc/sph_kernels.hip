@@ -103,11 +103,13 @@ template <class R, class Make>
 __device__ __forceinline__ void build_records(const DevParts &d, R *dst,
                                               int lo, int hi, int mab,
                                               Make make) {
-  const int i = lo + blockIdx.x * kRecBlock + threadIdx.x;
-  if (mab >= 0) {  // uniform
-    if (i < hi && is_active(d.tb[i], mab)) dst[i] = make(i);
+  if (mab >= 0) {  // uniform; launched with run_blocks(hi - lo, kRecBlock)
+    for_runs(lo, hi, d.tb, [&](int i, int8_t tb) {
+      if (is_active(tb, mab)) dst[i] = make(i);
+    });
     return;
   }
+  const int i = lo + blockIdx.x * kRecBlock + threadIdx.x;
   __shared__ uint4 buf[kRecBlock * sizeof(R) / 16];
   R r{};
   if (i < hi) r = make(i);
@@ -140,24 +142,24 @@ __global__ __launch_bounds__(kRecBlock) void k_build_force_rec(DevParts d,
 // resets the ghost bracket.
 __global__ void k_init_density(DevParts d, int lo, int hi, int max_active_bin,
                                float h_max) {
-  const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi) return;
-  const bool act = is_active(d.tb[i], max_active_bin);
-  d.redo[i] = act ? 1 : 0;
-  d.lfb[i] = 0;
-  if (!act) return;
-  d.rho[i] = 0.f;
-  d.rho_dh[i] = 0.f;
-  d.wc[i] = 0.f;
-  d.wc_dh[i] = 0.f;
-  d.divv[i] = 0.f;
-  d.rotx[i] = 0.f;
-  d.roty[i] = 0.f;
-  d.rotz[i] = 0.f;
-  d.nd[i] = 0;
-  d.lapu[i] = 0.f;
-  d.left[i] = 0.f;
-  d.right[i] = h_max;
+  for_runs(lo, hi, d.tb, [&](int i, int8_t tb) {
+    const bool act = is_active(tb, max_active_bin);
+    d.redo[i] = act ? 1 : 0;
+    d.lfb[i] = 0;
+    if (!act) return;
+    d.rho[i] = 0.f;
+    d.rho_dh[i] = 0.f;
+    d.wc[i] = 0.f;
+    d.wc_dh[i] = 0.f;
+    d.divv[i] = 0.f;
+    d.rotx[i] = 0.f;
+    d.roty[i] = 0.f;
+    d.rotz[i] = 0.f;
+    d.nd[i] = 0;
+    d.lapu[i] = 0.f;
+    d.left[i] = 0.f;
+    d.right[i] = h_max;
+  });
 }
 
 // hydro_prepare_gradient + hydro_reset_gradient (hydro.h:671-750) on a
@@ -303,32 +305,52 @@ __device__ bool ghost_one(const DevParts &d, const GhostArgs &ga, int i,
 }
 
 // Counts are reduced over the block first: one atomic per block and
-// counter keeps the same-address traffic small.
-constexpr int kGhostBlock = 1024;
+// counter keeps the same-address traffic small.  A block covers kRuns runs
+// of kGhostBlock particles (their redo flags loaded together); the
+// particles that go again are appended to d.sel in block, wave, run, lane
+// order.
+constexpr int kGhostBlock = 256;
 __global__ __launch_bounds__(kGhostBlock) void k_ghost(
     DevParts d, GhostArgs ga, int lo, int hi, unsigned long long *counters) {
   constexpr int W = kGhostBlock / 64;
   __shared__ int wcnt[3][W];
   __shared__ unsigned long long wwork[W];
   __shared__ int base;
-  const int i = lo + blockIdx.x * kGhostBlock + threadIdx.x;
-  bool again = false, relist = false;
-  // density interactions evaluated for this particle in the last round
+  const int i0 = lo + blockIdx.x * kGhostBlock * kRuns + threadIdx.x;
+  int redo[kRuns];
+#pragma unroll
+  for (int u = 0; u < kRuns; u++) {
+    const int i = i0 + u * kGhostBlock;
+    redo[u] = i < hi ? d.redo[i] : 0;
+  }
+  unsigned int again = 0u, relist = 0u;  // bit u: run u
+  // density interactions evaluated for these particles in the last round
   unsigned long long work = 0ull;
-  if (i < hi && d.redo[i]) {
-    work = (unsigned long long)d.nd[i];
-    again = ghost_one(d, ga, i, relist);
+#pragma unroll
+  for (int u = 0; u < kRuns; u++) {
+    if (redo[u]) {
+      const int i = i0 + u * kGhostBlock;
+      work += (unsigned long long)d.nd[i];
+      bool rl = false;
+      if (ghost_one(d, ga, i, rl)) again |= 1u << u;
+      if (rl) relist |= 1u << u;
+    }
   }
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) work += __shfl_xor(work, off);
-  const unsigned long long ba = __ballot(again);
-  const unsigned long long bb = __ballot(again && relist);
-  const unsigned long long bc = __ballot(!again && relist);
+  int na = 0, nb = 0, nc = 0;
+#pragma unroll
+  for (int u = 0; u < kRuns; u++) {
+    const bool ag = (again >> u) & 1u, rl = (relist >> u) & 1u;
+    na += __popcll(__ballot(ag));
+    nb += __popcll(__ballot(ag && rl));
+    nc += __popcll(__ballot(!ag && rl));
+  }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) {
-    wcnt[0][w] = __popcll(ba);
-    wcnt[1][w] = __popcll(bb);
-    wcnt[2][w] = __popcll(bc);
+    wcnt[0][w] = na;
+    wcnt[1][w] = nb;
+    wcnt[2][w] = nc;
     wwork[w] = work;
   }
   __syncthreads();
@@ -344,12 +366,18 @@ __global__ __launch_bounds__(kGhostBlock) void k_ghost(
     if (t) atomicAdd(&counters[3], t);
   }
   __syncthreads();
-  if (again) {
+  if (__any(again != 0u)) {
     // append to the compact list of particles for the next round
     int off = base;
     for (int k = 0; k < w; k++) off += wcnt[0][k];
-    off += __popcll(ba & ((1ull << lane) - 1ull));
-    d.sel[off] = i;
+    const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int u = 0; u < kRuns; u++) {
+      const bool ag = (again >> u) & 1u;
+      const unsigned long long ba = __ballot(ag);
+      if (ag) d.sel[off + __popcll(ba & below)] = i0 + u * kGhostBlock;
+      off += __popcll(ba);
+    }
   }
 }
 
@@ -362,65 +390,63 @@ __global__ void k_fill_f32(float *p, float v, int n) {
 // hydro_prepare_force (hydro.h:840-954, a = 1), hydro_reset_acceleration and
 // timestep_limiter_prepare_force for active particles in [lo, hi).
 __global__ void k_extra_ghost(DevParts d, GhostArgs ga, int lo, int hi) {
-  const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi) return;
-  const int8_t tb = d.tb[i];
-  if (!is_active(tb, ga.max_active_bin)) return;
-  const float h = d.h[i];
-  const float h_inv = 1.0f / h;
-  const float hid1 = h_inv * h_inv * h_inv * h_inv;
-  const float lapu = d.lapu[i] * (2.f * hid1);
-  d.lapu[i] = lapu;
-  const float dt_alpha =
-      tb <= 0 ? 0.f : (float)((double)(1LL << (tb + 1)) * ga.time_base);
-  const float ksp = h * kGamma;
-  const float ksp_inv = 1.f / ksp;
-  const float vsig = d.vsig[i];
-  const float rho = d.rho[i];
-  const float P = kGasGm1 * d.u[i] * rho;
-  const float c = sqrtf(kGasGamma * P / rho);
-  const float sct_inv = c * ksp_inv;
-  const float divv = d.divv[i];
-  const float div_v_dt =
-      dt_alpha == 0.f ? 0.f : (divv - d.divv_prev[i]) / dt_alpha;
-  const float S = divv < 0.f ? ksp * ksp * fmaxf(0.f, -div_v_dt) : 0.f;
-  const float alpha_loc = ga.alpha_max * S / (c * c + S);
-  float alpha = d.alpha[i];
-  if (alpha_loc > alpha) {
-    alpha = alpha_loc;
-  } else {
-    const float tr = dt_alpha * sct_inv * ga.visc_length;
-    alpha += alpha_loc * tr;
-    alpha /= (1.f + tr);
-  }
-  alpha = fmaxf(alpha, ga.alpha_min);
-  d.alpha[i] = alpha;
-  d.divv_prev[i] = divv;
-  d.divv_dt[i] = div_v_dt;
-  const float dtpi = vsig * ksp_inv;
-  const float sqrt_u_inv = 1.f / sqrtf(d.u[i]);
-  float adt = ga.diff_beta * ksp * lapu * sqrt_u_inv;
-  const float ad0 = d.adiff[i];
-  adt -= (ad0 - ga.diff_alpha_min) * dtpi;
-  float na = ad0 + adt * dt_alpha;
-  na = fmaxf(na, ga.diff_alpha_min);
-  const float lim = ga.diff_alpha_max * (1.f - d.avmax[i] / ga.alpha_max);
-  d.adiff[i] = fminf(na, lim);
-  d.ax[i] = 0.f;
-  d.ay[i] = 0.f;
-  d.az[i] = 0.f;
-  d.udt[i] = 0.f;
-  d.hdt[i] = 0.f;
-  d.nf[i] = 0;
-  d.mintb[i] = (int8_t)(kNumTimeBins + 1);
+  for_runs(lo, hi, d.tb, [&](int i, int8_t tb) {
+    if (!is_active(tb, ga.max_active_bin)) return;
+    const float h = d.h[i];
+    const float h_inv = 1.0f / h;
+    const float hid1 = h_inv * h_inv * h_inv * h_inv;
+    const float lapu = d.lapu[i] * (2.f * hid1);
+    d.lapu[i] = lapu;
+    const float dt_alpha =
+        tb <= 0 ? 0.f : (float)((double)(1LL << (tb + 1)) * ga.time_base);
+    const float ksp = h * kGamma;
+    const float ksp_inv = 1.f / ksp;
+    const float vsig = d.vsig[i];
+    const float rho = d.rho[i];
+    const float P = kGasGm1 * d.u[i] * rho;
+    const float c = sqrtf(kGasGamma * P / rho);
+    const float sct_inv = c * ksp_inv;
+    const float divv = d.divv[i];
+    const float div_v_dt =
+        dt_alpha == 0.f ? 0.f : (divv - d.divv_prev[i]) / dt_alpha;
+    const float S = divv < 0.f ? ksp * ksp * fmaxf(0.f, -div_v_dt) : 0.f;
+    const float alpha_loc = ga.alpha_max * S / (c * c + S);
+    float alpha = d.alpha[i];
+    if (alpha_loc > alpha) {
+      alpha = alpha_loc;
+    } else {
+      const float tr = dt_alpha * sct_inv * ga.visc_length;
+      alpha += alpha_loc * tr;
+      alpha /= (1.f + tr);
+    }
+    alpha = fmaxf(alpha, ga.alpha_min);
+    d.alpha[i] = alpha;
+    d.divv_prev[i] = divv;
+    d.divv_dt[i] = div_v_dt;
+    const float dtpi = vsig * ksp_inv;
+    const float sqrt_u_inv = 1.f / sqrtf(d.u[i]);
+    float adt = ga.diff_beta * ksp * lapu * sqrt_u_inv;
+    const float ad0 = d.adiff[i];
+    adt -= (ad0 - ga.diff_alpha_min) * dtpi;
+    float na = ad0 + adt * dt_alpha;
+    na = fmaxf(na, ga.diff_alpha_min);
+    const float lim = ga.diff_alpha_max * (1.f - d.avmax[i] / ga.alpha_max);
+    d.adiff[i] = fminf(na, lim);
+    d.ax[i] = 0.f;
+    d.ay[i] = 0.f;
+    d.az[i] = 0.f;
+    d.udt[i] = 0.f;
+    d.hdt[i] = 0.f;
+    d.nf[i] = 0;
+    d.mintb[i] = (int8_t)(kNumTimeBins + 1);
+  });
 }
 
 // runner_do_end_hydro_force (runner_others.c:815-870) -> hydro_end_force.
 __global__ void k_end_force(DevParts d, int lo, int hi, int max_active_bin) {
-  const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= hi) return;
-  if (!is_active(d.tb[i], max_active_bin)) return;
-  d.hdt[i] *= d.h[i] * kDimInv;
+  for_runs(lo, hi, d.tb, [&](int i, int8_t tb) {
+    if (is_active(tb, max_active_bin)) d.hdt[i] *= d.h[i] * kDimInv;
+  });
 }
 
 __global__ void k_zero_int(int *p, int n) {
@@ -439,12 +465,24 @@ __global__ void k_count(DevParts d, int n, int max_active_bin,
                         unsigned long long *out) {
   __shared__ unsigned long long part[3][4];
   unsigned long long a = 0, b = 0, c = 0;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x) {
-    if (is_active(d.tb[i], max_active_bin)) {
-      a += (unsigned)d.nd[i];
-      b += (unsigned)d.ng[i];
-      c += (unsigned)d.nf[i];
+  // kRuns time bins in flight per thread and sweep
+  const int stride = gridDim.x * blockDim.x * kRuns;
+  for (int i0 = blockIdx.x * blockDim.x * kRuns + threadIdx.x; i0 < n;
+       i0 += stride) {
+    int8_t t[kRuns];
+#pragma unroll
+    for (int u = 0; u < kRuns; u++) {
+      const int i = i0 + u * (int)blockDim.x;
+      t[u] = i < n ? d.tb[i] : kBinInhibited;
+    }
+#pragma unroll
+    for (int u = 0; u < kRuns; u++) {
+      const int i = i0 + u * (int)blockDim.x;
+      if (i < n && is_active(t[u], max_active_bin)) {
+        a += (unsigned)d.nd[i];
+        b += (unsigned)d.ng[i];
+        c += (unsigned)d.nf[i];
+      }
     }
   }
   for (int off = 32; off > 0; off >>= 1) {
@@ -500,8 +538,9 @@ __device__ __forceinline__ int block_excl_scan(int v, int *wsum, int *total) {
 }
 
 __device__ __forceinline__ bool tile_pred(const DevParts &d, const LoopArgs &la,
-                                          int mode, const int *flag, int i) {
-  if (!is_active(d.tb[i], la.max_active_bin)) return false;
+                                          int mode, const int *flag, int i,
+                                          int8_t tb) {
+  if (!is_active(tb, la.max_active_bin)) return false;
   if (mode >= 1 && !flag[i]) return false;
   // mode 2: only particles whose neighbour list does not cover their radius
   if (mode == 2 && search_radius2(d.h[i]) <= d.r2b[i]) return false;
@@ -510,8 +549,9 @@ __device__ __forceinline__ bool tile_pred(const DevParts &d, const LoopArgs &la,
 
 __global__ void k_sel_flag(DevParts d, LoopArgs la, int mode, const int *flag,
                            int n, int *f) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) f[i] = tile_pred(d, la, mode, flag, i) ? 1 : 0;
+  for_runs(0, n, d.tb, [&](int i, int8_t tb) {
+    f[i] = tile_pred(d, la, mode, flag, i, tb) ? 1 : 0;
+  });
 }
 
 __global__ __launch_bounds__(kScanBlock) void k_scan_up(const int *in, int n,
@@ -658,7 +698,7 @@ hipError_t launch_build_tiles(const DevParts &d, const LoopArgs &la, int n,
     return hipGetLastError();
   }
   const int nb = nblk(n, kScanBlock), ncb = nblk(ncells, kScanBlock);
-  hipLaunchKernelGGL(k_sel_flag, dim3(nblk(n, 256)), dim3(256), 0, st, d, la,
+  hipLaunchKernelGGL(k_sel_flag, dim3(run_blocks(n, 256)), dim3(256), 0, st, d, la,
                      mode, flag, n, ts.flag);
   hipLaunchKernelGGL(k_scan_up, dim3(nb), dim3(kScanBlock), 0, st,
                      (const int *)ts.flag, n, ts.blk);
@@ -877,8 +917,10 @@ hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
 hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
                                 int hi, hipStream_t st, int max_active_bin) {
   if (hi <= lo) return hipSuccess;
-  const dim3 g(nblk(hi - lo, kRecBlock)), b(kRecBlock);
   const int m = max_active_bin;
+  // active records: kRuns runs per block; all records: one run, staged
+  const dim3 g(m >= 0 ? run_blocks(hi - lo, kRecBlock) : nblk(hi - lo, kRecBlock)),
+      b(kRecBlock);
   switch (kind) {
     case LoopKind::Density:
       hipLaunchKernelGGL(k_build_dens_rec, g, b, 0, st, d, lo, hi, m);
@@ -896,7 +938,7 @@ hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
 hipError_t launch_init_density(const DevParts &d, int lo, int hi,
                                int max_active_bin, float h_max, hipStream_t st) {
   if (hi <= lo) return hipSuccess;
-  hipLaunchKernelGGL(k_init_density, dim3(nblk(hi - lo, 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_init_density, dim3(run_blocks(hi - lo, 256)), dim3(256), 0, st,
                      d, lo, hi, max_active_bin, h_max);
   return hipGetLastError();
 }
@@ -906,7 +948,7 @@ hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,
   // counters[3] accumulates over the rounds of a pass (zeroed by the pass)
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, counters, 3);
   if (hi > lo)
-    hipLaunchKernelGGL(k_ghost, dim3(nblk(hi - lo, kGhostBlock)),
+    hipLaunchKernelGGL(k_ghost, dim3(run_blocks(hi - lo, kGhostBlock)),
                        dim3(kGhostBlock), 0, st, d, ga, lo, hi, counters);
   return hipGetLastError();
 }
@@ -914,7 +956,7 @@ hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,
 hipError_t launch_extra_ghost(const DevParts &d, const GhostArgs &ga, int lo,
                               int hi, hipStream_t st) {
   if (hi <= lo) return hipSuccess;
-  hipLaunchKernelGGL(k_extra_ghost, dim3(nblk(hi - lo, 256)), dim3(256), 0, st,
+  hipLaunchKernelGGL(k_extra_ghost, dim3(run_blocks(hi - lo, 256)), dim3(256), 0, st,
                      d, ga, lo, hi);
   return hipGetLastError();
 }
@@ -922,7 +964,7 @@ hipError_t launch_extra_ghost(const DevParts &d, const GhostArgs &ga, int lo,
 hipError_t launch_end_force(const DevParts &d, int lo, int hi,
                             int max_active_bin, hipStream_t st) {
   if (hi <= lo) return hipSuccess;
-  hipLaunchKernelGGL(k_end_force, dim3(nblk(hi - lo, 256)), dim3(256), 0, st, d,
+  hipLaunchKernelGGL(k_end_force, dim3(run_blocks(hi - lo, 256)), dim3(256), 0, st, d,
                      lo, hi, max_active_bin);
   return hipGetLastError();
 }
@@ -930,7 +972,7 @@ hipError_t launch_end_force(const DevParts &d, int lo, int hi,
 hipError_t launch_count(const DevParts &d, int n, int max_active_bin,
                         unsigned long long *out3, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, out3, 3);
-  const int nb = n > 0 ? min(512, nblk(n, 256)) : 1;
+  const int nb = n > 0 ? min(1024, run_blocks(n, 256)) : 1;
   hipLaunchKernelGGL(k_count, dim3(nb), dim3(256), 0, st, d, n, max_active_bin,
                      out3);
   return hipGetLastError();

@@ -278,11 +278,9 @@ __global__ __launch_bounds__(kBlock) void k_drift(DevParts d, DriftArgs a,
 // the particles starting a step at ti_current (runner_do_kick1), mode 2 the
 // second half-kick of the active particles followed by
 // hydro_reset_predicted_values (runner_do_kick2).
-__global__ void k_kick(DevParts d, KickArgs a) {
+__device__ __forceinline__ void kick_one(const DevParts &d, const KickArgs &a,
+                                         int i, int tb) {
 #pragma clang fp contract(off)
-  const int i = a.lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.hi) return;
-  const int tb = d.tb[i];
   if (tb == kBinInhibited) return;
   const long long dti = integer_timestep(tb);
   long long ti_begin, ti_end;
@@ -324,6 +322,10 @@ __global__ void k_kick(DevParts d, KickArgs a) {
     d.c[i] = c;
     d.vsig[i] = fmaxf(d.vsig[i], 2.f * c);
   }
+}
+
+__global__ void k_kick(DevParts d, KickArgs a) {
+  for_runs(a.lo, a.hi, d.tb, [&](int i, int8_t tb) { kick_one(d, a, i, tb); });
 }
 
 // make_integer_timestep (src/timestep.h:49-89); 0 flags a step of size 0.
@@ -521,7 +523,7 @@ hipError_t launch_drift(const DevParts &d, const DriftArgs &a,
 
 hipError_t launch_kick(const DevParts &d, const KickArgs &a, hipStream_t st) {
   if (a.hi <= a.lo) return hipSuccess;
-  k_kick<<<blocks(a.hi - a.lo), kBlock, 0, st>>>(d, a);
+  k_kick<<<run_blocks(a.hi - a.lo, kBlock), kBlock, 0, st>>>(d, a);
   return hipGetLastError();
 }
 
