@@ -279,6 +279,7 @@ def evolve_bench(n_side: int, ncell: int, nsteps: int, seed: int = 0,
         p = info["pass"]
         steps.append({"ti": info["ti_current"], "active": info["n_active"],
                       "ms": 1e3 * dt, "rebuilt": info["rebuilt"],
+                      "sparse_tiles": info["sparse"],
                       "ghost_rounds": p["ghost_rounds"],
                       "interactions": p["n_density"] + p["n_force"]})
         print(f"bench: evolve step {k + 1}: {info['n_active']} active, "

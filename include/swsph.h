@@ -187,9 +187,17 @@ int swsph_get_chunk(swsph_engine *e, int which, int *C);
 /* The same with the candidate chunks of the work-queue schedules (modes 1
  * and 2 of swsph_set_schedule) sized apart from the tiles: tiles of CI
  * i-particles against candidate chunks of CJ particles, CJ = CI or
- * CJ = CI / 2 for CI = 16, 32 (finer candidate culling under the same tile).
+ * CJ = CI / 2 for CI = 16, 32 (finer candidate culling under the same tile),
+ * or the narrow tiles (4, 16), (4, 32), (8, 16) for steps with few active
+ * particles per cell (64 / CI candidate streams per particle).
  * swsph_set_chunk(e, which, C) is swsph_set_tile_shape(e, which, C, C). */
 int swsph_set_tile_shape(swsph_engine *e, int which, int CI, int CJ);
+/* Tile shape of the passes of swsph_engine_step when its active particles
+ * number fewer than per_cell per top-level cell on average (work-queue
+ * schedules only): all three loops then use tiles of CI against chunks of
+ * CJ (default (4, 32) below 16 per cell; per_cell = 0 turns it off).  No
+ * counterpart in the reference; speed only, as swsph_set_tile_shape. */
+int swsph_set_sparse_shape(swsph_engine *e, int CI, int CJ, int per_cell);
 int swsph_get_tile_shape(swsph_engine *e, int which, int *CI, int *CJ);
 
 /* Schedule of the batched pass's neighbour loops.  mode 0: one wave per
@@ -281,6 +289,7 @@ typedef struct swsph_step_info {
   float dx_max;            /* largest distance outside a cell after the drift */
   int64_t launches;
   swsph_pass_stats pass;
+  int sparse;              /* the pass ran with the sparse tile shape */
 } swsph_step_info;
 /* One step of the gas (engine_step, src/engine.c, with the hydro tasks
  * only): the first call computes forces at t = 0 for all particles, their

@@ -87,6 +87,7 @@ struct swsph_engine {
     int C = 0;
     int nchunks = 0;
     bool valid = false;    // built for the current particles
+    bool stale = false;    // bounds out of date (refreshed on next use)
     int cap_chunks = 0;    // device capacity (chunks, cells)
     int cap_cells = 0;
     std::vector<int> cell_chunk_h;  // [ncells + 1]
@@ -113,7 +114,12 @@ struct swsph_engine {
   int schedule = 1;
   int queue_waves = 0;  // 0: one full-occupancy wave set per CU
   TileScratch ts{};
+  size_t tile_cap = 0;  // entries of ts.tile_start / ts.tile_cell
   int tiles_c = 0;  // capacity of the all-active tile table in ts, 0: none
+  // Steps with few active particles per cell (fewer than sparse_per_cell on
+  // average) run their pass with tiles of sparse_ci against chunks of
+  // sparse_cj in all three loops; 0 disables.
+  int sparse_ci = 4, sparse_cj = 32, sparse_per_cell = 16;
   // Device capacity (particles, cells): buffers are reused by later
   // swsph_set_particles calls that fit, without reallocation.
   size_t cap_n = 0;
@@ -146,10 +152,11 @@ struct swsph_engine {
   bool started = false;
   unsigned int *d_dmax = nullptr;
   unsigned long long *d_tiend = nullptr;  // [1] ti_end_min
-  unsigned int *d_err = nullptr;          // [2] time-step errors, active
+  unsigned int *d_err = nullptr;          // [3] time-step errors, active,
+                                          // active count before a step
   unsigned int *h_small = nullptr;        // pinned readback: dmax, hg2
   unsigned long long *h_tiend = nullptr;  // pinned [1]
-  unsigned int *h_err = nullptr;          // pinned [2]
+  unsigned int *h_err = nullptr;          // pinned [3]
   int *h_ntiles = nullptr;                // pinned [1]
   float dmax = 0.f;
   // The neighbour records of every particle match its fields (the passes
@@ -286,14 +293,28 @@ void free_dev(swsph_engine *e) {
 
 // Chunk table for capacity C: runs of <= C consecutive particles of a cell.
 // Built on first use; boxes and radii are filled by refresh_bounds.
-int refresh_bounds(swsph_engine *e, int c_lo, int c_hi, bool pos = true);
+int refresh_bounds(swsph_engine *e, int c_lo, int c_hi, bool pos = true,
+                   const swsph_engine::ChunkTable *extra = nullptr);
 
 swsph_engine::ChunkTable *chunk_table(swsph_engine *e, int C, int *rc) {
   *rc = 0;
   swsph_engine::ChunkTable *tp = nullptr;
   for (auto &t : e->tables)
     if (t.C == C) tp = &t;
-  if (tp && tp->valid) return tp;
+  if (tp && tp->valid) {
+    if (tp->stale) {  // positions or radii changed while it was not in use
+      const hipError_t err = sphk::launch_chunk_bounds(
+          e->d, tp->d_start, 0, tp->nchunks, tp->C, tp->d_bbox, tp->d_hg2max,
+          e->d_hg2_all, e->st);
+      if (err != hipSuccess) {
+        *rc = fail(2, "chunk bounds: %s", hipGetErrorString(err));
+        return nullptr;
+      }
+      e->launches++;
+      tp->stale = false;
+    }
+    return tp;
+  }
   if (C != 8 && C != 16 && C != 32 && C != 64) {
     *rc = fail(1, "chunk capacity %d not in {8, 16, 32, 64}", C);
     return nullptr;
@@ -356,7 +377,8 @@ swsph_engine::ChunkTable *chunk_table(swsph_engine *e, int C, int *rc) {
     return nullptr;
   }
   t.valid = true;
-  if ((*rc = refresh_bounds(e, 0, e->ncells))) return nullptr;
+  t.stale = false;
+  if ((*rc = refresh_bounds(e, 0, e->ncells, true, &t))) return nullptr;
   return tp;
 }
 
@@ -391,21 +413,44 @@ void prune_tables(swsph_engine *e) {
 
 // Candidate positions / radii (unless `pos` is false: already written) and
 // chunk bounds of cells [c_lo, c_hi), after positions or smoothing lengths
-// changed.
-int refresh_bounds(swsph_engine *e, int c_lo, int c_hi, bool pos) {
+// changed.  Only the tables the loops read now (and `extra`) are refreshed;
+// the others are marked stale and refreshed whole on their next use.
+int refresh_bounds(swsph_engine *e, int c_lo, int c_hi, bool pos,
+                   const swsph_engine::ChunkTable *extra) {
   const int lo = e->cell_start[c_lo], hi = e->cell_start[c_hi];
-  if (c_lo == 0 && c_hi == e->ncells)
-    HIPCHK(sphk::launch_zero_u32(e->d_hg2_all, 1, e->st));
+  const bool all = c_lo == 0 && c_hi == e->ncells;
+  if (all) HIPCHK(sphk::launch_zero_u32(e->d_hg2_all, 1, e->st));
   if (pos) {
     HIPCHK(sphk::launch_build_pos(e->d, e->grid, pos_eps(e), lo, hi, e->st));
     e->launches++;
   }
-  for (auto &t : e->tables) {
-    if (!t.valid) continue;
+  auto used = [&](const swsph_engine::ChunkTable &t) {
+    bool u = &t == extra;
+    for (int k = 0; k < 3; k++) u = u || t.C == table_c(e, k);
+    return u;
+  };
+  auto refresh = [&](swsph_engine::ChunkTable &t) -> int {
     HIPCHK(sphk::launch_chunk_bounds(e->d, t.d_start, t.cell_chunk_h[c_lo],
                                      t.cell_chunk_h[c_hi], t.C, t.d_bbox,
                                      t.d_hg2max, e->d_hg2_all, e->st));
     e->launches++;
+    return 0;
+  };
+  bool any = false;
+  for (auto &t : e->tables) {
+    if (!t.valid || t.stale || !used(t)) continue;
+    if (int rc = refresh(t)) return rc;
+    any = true;
+  }
+  for (auto &t : e->tables) {
+    if (!t.valid || t.stale || used(t)) continue;
+    if (any) {
+      t.stale = true;
+    } else {
+      // no table in use is built yet: this one carries the largest radius
+      if (int rc = refresh(t)) return rc;
+      any = true;
+    }
   }
   return 0;
 }
@@ -444,8 +489,11 @@ int alloc_dev(swsph_engine *e, size_t n) {
     HIPCHK(dalloc(&t.cscan, nc + 1));
     HIPCHK(dalloc(&t.cblk, nc / 1024 + 2));
     HIPCHK(dalloc(&t.alist, n));
-    HIPCHK(dalloc(&t.tile_start, n / 8 + nc + 1));  // C >= 8
-    HIPCHK(dalloc(&t.tile_cell, n / 8 + nc + 1));
+    // a cell of k particles makes ceil(k / C) tiles: at most n / C + nc
+    // tiles for tiles of C >= kMinTile particles, plus the end marker
+    e->tile_cap = n / kMinTile + nc + 1;
+    HIPCHK(dalloc(&t.tile_start, e->tile_cap));
+    HIPCHK(dalloc(&t.tile_cell, e->tile_cap));
     HIPCHK(dalloc(&t.ntiles, 1));
     HIPCHK(dalloc(&t.heads, 8 * 64));
   }
@@ -589,6 +637,8 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
   int rc = 0;
   const int C = build ? kListChunk : e->loop_c[(int)k];
   const int CJ = build ? kListChunk : e->loop_cj[(int)k];
+  if ((size_t)e->n / C + (size_t)e->ncells + 1 > e->tile_cap)
+    return fail(1, "tiles of %d particles exceed the tile table", C);
   const swsph_engine::ChunkTable *t = chunk_table(e, CJ, &rc);
   if (!t) return rc;
   const LoopArgs la = loop_args(e);
@@ -1263,12 +1313,12 @@ int swsph_engine_create(const swsph_props *props, swsph_engine **out) {
       hipMalloc((void **)&e->d_dmax, sizeof(unsigned int)) != hipSuccess ||
       hipMalloc((void **)&e->d_tiend, sizeof(unsigned long long)) !=
           hipSuccess ||
-      hipMalloc((void **)&e->d_err, 2 * sizeof(unsigned int)) != hipSuccess ||
+      hipMalloc((void **)&e->d_err, 3 * sizeof(unsigned int)) != hipSuccess ||
       hipHostMalloc((void **)&e->h_small, 2 * sizeof(unsigned int), 0) !=
           hipSuccess ||
       hipHostMalloc((void **)&e->h_tiend, sizeof(unsigned long long), 0) !=
           hipSuccess ||
-      hipHostMalloc((void **)&e->h_err, 2 * sizeof(unsigned int), 0) !=
+      hipHostMalloc((void **)&e->h_err, 3 * sizeof(unsigned int), 0) !=
           hipSuccess ||
       hipHostMalloc((void **)&e->h_ntiles, sizeof(int), 0) != hipSuccess) {
     delete e;
@@ -1776,7 +1826,8 @@ int swsph_set_tile_shape(swsph_engine *e, int which, int CI, int CJ) {
   e->gen++;  // captured graphs no longer match
   if (which < 0 || which > 2) return fail(1, "loop index %d out of range", which);
   const bool ok = (CJ == CI && (CI == 8 || CI == 16 || CI == 32 || CI == 64)) ||
-                  (CJ * 2 == CI && (CI == 16 || CI == 32));
+                  (CJ * 2 == CI && (CI == 16 || CI == 32)) ||
+                  (CI == 4 && (CJ == 16 || CJ == 32)) || (CI == 8 && CJ == 16);
   if (!ok) return fail(1, "tile shape (%d, %d) not supported", CI, CJ);
   e->loop_c[which] = CI;
   e->loop_cj[which] = CJ;
@@ -1785,6 +1836,23 @@ int swsph_set_tile_shape(swsph_engine *e, int which, int CI, int CJ) {
     int rc = 0;
     if (!chunk_table(e, table_c(e, which), &rc)) return rc;
   }
+  return 0;
+}
+
+int swsph_set_sparse_shape(swsph_engine *e, int CI, int CJ, int per_cell) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (per_cell < 0) return fail(1, "per_cell %d is negative", per_cell);
+  if (per_cell > 0) {
+    const bool ok = (CJ == CI && (CI == 8 || CI == 16 || CI == 32 || CI == 64)) ||
+                    (CJ * 2 == CI && (CI == 16 || CI == 32)) ||
+                    (CI == 4 && (CJ == 16 || CJ == 32)) ||
+                    (CI == 8 && CJ == 16);
+    if (!ok) return fail(1, "tile shape (%d, %d) not supported", CI, CJ);
+    e->sparse_ci = CI;
+    e->sparse_cj = CJ;
+  }
+  e->sparse_per_cell = per_cell;
   return 0;
 }
 
@@ -2082,14 +2150,42 @@ int swsph_engine_step(swsph_engine *e, swsph_step_info *info) {
   }
   const float dx_max = e->dmax;
   set_active_bin(e, max_active_bin_at(e->ti_current));
+  // narrow tiles when the active particles are few per cell
+  bool sparse = false;
+  if (e->sparse_per_cell > 0 && e->schedule >= 1) {
+    HIPCHK(sphk::launch_count_active(e->d, (int)e->n, e->props.max_active_bin,
+                                     e->d_err + 2, e->st));
+    HIPCHK(hipMemcpyAsync(e->h_err + 2, e->d_err + 2, sizeof(unsigned int),
+                          hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    sparse = (double)e->h_err[2] <
+             (double)e->sparse_per_cell * (double)e->ncells;
+  }
+  const int c0[3] = {e->loop_c[0], e->loop_c[1], e->loop_c[2]};
+  const int cj0[3] = {e->loop_cj[0], e->loop_cj[1], e->loop_cj[2]};
+  if (sparse) {
+    for (int k = 0; k < 3; k++) {
+      e->loop_c[k] = e->sparse_ci;
+      e->loop_cj[k] = e->sparse_cj;
+    }
+    e->tiles_c = 0;
+  }
   swsph_pass_stats ps{};
   rc = pass_batched_locked(e, &ps);
   if (rc == 4 && e->dmax > 0.f) {
     // the smoothing lengths outgrew the slack the drift left: re-bin and
     // repeat the pass from the lengths found so far
-    if ((rc = rebin(e))) return rc;
-    rebuilt = 1;
-    rc = pass_batched_locked(e, &ps);
+    if (!(rc = rebin(e))) {
+      rebuilt = 1;
+      rc = pass_batched_locked(e, &ps);
+    }
+  }
+  if (sparse) {
+    for (int k = 0; k < 3; k++) {
+      e->loop_c[k] = c0[k];
+      e->loop_cj[k] = cj0[k];
+    }
+    e->tiles_c = 0;
   }
   if (rc) return rc;
   if (!first && (rc = kick_cells(e, -1, e->ti_current, 2))) return rc;
@@ -2102,6 +2198,7 @@ int swsph_engine_step(swsph_engine *e, swsph_step_info *info) {
   e->ti_end_min = tiend;
   e->started = true;
   if (info) {
+    info->sparse = sparse ? 1 : 0;
     info->ti_current = e->ti_current;
     info->ti_end_min = tiend;
     info->max_active_bin = e->props.max_active_bin;

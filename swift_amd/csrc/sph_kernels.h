@@ -9,18 +9,25 @@
 // loads.  The leading double position (and, for the force loop, the search
 // radius) is what the candidate test reads.  Inhibited particles get a far
 // position (and a negative radius) so that every test rejects them.
+#ifndef SWSPH_DENS_PAD
+#define SWSPH_DENS_PAD 2
+#endif
+#ifndef SWSPH_GRAD_PAD
+#define SWSPH_GRAD_PAD 6
+#endif
+#ifndef SWSPH_FORCE_PAD
+#define SWSPH_FORCE_PAD 2
+#endif
 struct __attribute__((aligned(16))) DensRec {
   double x, y, z;
   float vx, vy, vz, m;
-  float pad[2];
+  float pad[SWSPH_DENS_PAD];
 };
-// Record sizes are odd multiples of 16 bytes so that lanes gathering
-// different records from LDS spread over the banks.
 struct __attribute__((aligned(16))) GradRec {
   double x, y, z;
   float vx, vy, vz, mrho;  // m_j / rho_j
   float u, c, alpha, pad0;
-  float pad[6];
+  float pad[SWSPH_GRAD_PAD];
 };
 struct __attribute__((aligned(16))) ForceRec {
   double x, y, z;
@@ -30,7 +37,7 @@ struct __attribute__((aligned(16))) ForceRec {
   float rho, inv_rho, pr2, f;  // pr2 = P / rho^2
   float bal, c, alpha, P;
   float Pad, u, tb, pad0;      // Pad = P * alpha_diff
-  float pad[2];
+  float pad[SWSPH_FORCE_PAD];
 };
 
 // Candidate record of the FP32 pre-test: the position relative to the
@@ -119,6 +126,9 @@ struct DevParts {
 // (sx dim_x, sy dim_y, sz dim_z) of the j-cell.
 constexpr int kListStreams = 4;
 constexpr int kListChunk = 64 / kListStreams;
+// Smallest tile (i-particles per wave) of the work-queue schedules; the
+// tile tables are sized for it.
+constexpr int kMinTile = 4;
 struct NbrList {
   unsigned int *idx;
   int *cnt;
@@ -265,6 +275,9 @@ hipError_t launch_end_force(const DevParts &d, int lo, int hi,
                             int max_active_bin, hipStream_t st);
 hipError_t launch_count(const DevParts &d, int n, int max_active_bin,
                         unsigned long long *out3, hipStream_t st);
+// *out = number of active particles of [0, n) (device-side).
+hipError_t launch_count_active(const DevParts &d, int n, int max_active_bin,
+                               unsigned int *out, hipStream_t st);
 // Records of particles [lo, hi); with max_active_bin >= 0 only those of the
 // active particles.
 hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,

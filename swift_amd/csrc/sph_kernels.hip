@@ -504,6 +504,33 @@ __global__ void k_count(DevParts d, int n, int max_active_bin,
   }
 }
 
+// Active particles of [0, n) at max_active_bin: one atomic per block.
+__global__ void k_count_active(const int8_t *__restrict__ tb, int n,
+                               int max_active_bin, unsigned int *out) {
+  __shared__ unsigned int part[4];
+  unsigned int c = 0;
+  const int stride = gridDim.x * blockDim.x * kRuns;
+  for (int i0 = blockIdx.x * blockDim.x * kRuns + threadIdx.x; i0 < n;
+       i0 += stride) {
+    int8_t t[kRuns];
+#pragma unroll
+    for (int u = 0; u < kRuns; u++) {
+      const int i = i0 + u * (int)blockDim.x;
+      t[u] = i < n ? tb[i] : kBinInhibited;
+    }
+#pragma unroll
+    for (int u = 0; u < kRuns; u++) c += is_active(t[u], max_active_bin) ? 1u : 0u;
+  }
+  for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned int t = 0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); k++) t += part[k];
+    if (t) atomicAdd(out, t);
+  }
+}
+
 // ------------------------------------------------------------ tile builder --
 // Device-wide exclusive scans (block sums, one-block scan of the sums,
 // block-local scans) and the compaction / tiling of the particles a loop
@@ -763,8 +790,10 @@ hipError_t queue_c(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
   return hipGetLastError();
 }
 
-// Tile (CI) x chunk (CJ) shapes with kernels: CJ = CI, or CJ = CI / 2 for
-// CI = 16, 32 (finer candidate chunks under the same tile).
+// Tile (CI) x chunk (CJ) shapes with kernels: CJ = CI, CJ = CI / 2 for
+// CI = 16, 32 (finer candidate chunks under the same tile), and the narrow
+// tiles (4, 16), (4, 32), (8, 16) for sparse activity (a few active particles
+// per cell: more candidate streams per particle, fewer idle lanes).
 template <class L>
 hipError_t queue_any(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
                      const TileArgs &ta, const NbrList &nl,
@@ -777,6 +806,9 @@ hipError_t queue_any(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
     case 3232: return queue_c<L, 32, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
     case 3216: return queue_c<L, 32, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
     case 6464: return queue_c<L, 64, 64, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 416: return queue_c<L, 4, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 432: return queue_c<L, 4, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 816: return queue_c<L, 8, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
   }
   return hipErrorInvalidValue;
 }
@@ -966,6 +998,16 @@ hipError_t launch_end_force(const DevParts &d, int lo, int hi,
   if (hi <= lo) return hipSuccess;
   hipLaunchKernelGGL(k_end_force, dim3(run_blocks(hi - lo, 256)), dim3(256), 0, st, d,
                      lo, hi, max_active_bin);
+  return hipGetLastError();
+}
+
+hipError_t launch_count_active(const DevParts &d, int n, int max_active_bin,
+                               unsigned int *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, (int *)out, 1);
+  if (n > 0)
+    hipLaunchKernelGGL(k_count_active, dim3(min(1024, run_blocks(n, 256))),
+                       dim3(256), 0, st, (const int8_t *)d.tb, n, max_active_bin,
+                       out);
   return hipGetLastError();
 }
 

@@ -46,7 +46,7 @@ __device__ __forceinline__ DensRec dens_rec(const DevParts &d, int i) {
   r.vy = d.vy[i];
   r.vz = d.vz[i];
   r.m = d.m[i];
-  r.pad[0] = r.pad[1] = 0.f;
+  for (int k = 0; k < SWSPH_DENS_PAD; k++) r.pad[k] = 0.f;
   return r;
 }
 
@@ -65,7 +65,7 @@ __device__ __forceinline__ GradRec grad_rec(const DevParts &d, int i) {
   r.c = d.c[i];
   r.alpha = d.alpha[i];
   r.pad0 = 0.f;
-  for (int k = 0; k < 6; k++) r.pad[k] = 0.f;
+  for (int k = 0; k < SWSPH_GRAD_PAD; k++) r.pad[k] = 0.f;
   return r;
 }
 
@@ -98,7 +98,7 @@ __device__ __forceinline__ ForceRec force_rec(const DevParts &d, int i) {
   r.u = d.u[i];
   r.tb = (float)tb;
   r.pad0 = 0.f;
-  r.pad[0] = r.pad[1] = 0.f;
+  for (int k = 0; k < SWSPH_FORCE_PAD; k++) r.pad[k] = 0.f;
   return r;
 }
 

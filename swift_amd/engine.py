@@ -83,7 +83,8 @@ class StepInfo(C.Structure):
     _fields_ = [("ti_current", C.c_int64), ("ti_end_min", C.c_int64),
                 ("max_active_bin", C.c_int), ("rebuilt", C.c_int),
                 ("n_active", C.c_int64), ("dx_max", C.c_float),
-                ("launches", C.c_int64), ("pass_", PassStats)]
+                ("launches", C.c_int64), ("pass_", PassStats),
+                ("sparse", C.c_int)]
 
     def as_dict(self) -> dict:
         d = {k: getattr(self, k) for k, _ in self._fields_ if k != "pass_"}
@@ -270,9 +271,17 @@ class Engine:
 
     def set_tile_shape(self, loop: str, CI: int, CJ: int) -> None:
         """Tiles of CI i-particles against candidate chunks of CJ particles
-        in the work-queue schedules (CJ = CI, or CI / 2 for CI = 16, 32)."""
+        in the work-queue schedules (CJ = CI, or CI / 2 for CI = 16, 32, or
+        the narrow tiles (4, 16), (4, 32), (8, 16) for sparse activity)."""
         check(lib().swsph_set_tile_shape(self._h, self.LOOPS[loop], int(CI),
                                          int(CJ)), "swsph_set_tile_shape")
+
+    def set_sparse_shape(self, CI: int, CJ: int, per_cell: int) -> None:
+        """Tile shape of step() passes with fewer than per_cell active
+        particles per top-level cell (0 turns it off)."""
+        check(lib().swsph_set_sparse_shape(self._h, int(CI), int(CJ),
+                                           int(per_cell)),
+              "swsph_set_sparse_shape")
 
     def get_tile_shape(self, loop: str) -> tuple:
         ci, cj = C.c_int(), C.c_int()

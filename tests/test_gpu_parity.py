@@ -148,12 +148,14 @@ def test_chunk_shapes(orc, C):
     _check(orc, props, st, res)
 
 
-@pytest.mark.parametrize("shape", [(16, 8), (32, 16)])
+@pytest.mark.parametrize("shape", [(16, 8), (32, 16), (8, 16), (4, 16), (4, 32)])
 @pytest.mark.parametrize("schedule", [1, 2])
 def test_tile_shapes(orc, shape, schedule):
-    """Work-queue tiles against finer candidate chunks give the oracle's
-    neighbour sets and values, on a uniform and on a clustered box (half of
-    the particles active)."""
+    """Work-queue tiles against finer candidate chunks, and narrow tiles
+    against coarser ones (4 and 8 particles with 16 and 8 candidate streams
+    each, FP32 pre-test for (4, 16), packed FP16 for the others), give the
+    oracle's neighbour sets and values, on a uniform and on a clustered box
+    (half of the particles active)."""
     from swift_amd import Engine, HydroProps, ics
     CI, CJ = shape
     for st, props in (

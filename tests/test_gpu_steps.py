@@ -121,6 +121,8 @@ def test_steps_match_oracle_and_engine_step(orc):
         for e in (a, b):
             e.set_particles(st)
             e.set_integration(integ)
+        # the passes of the entry points use the engine's tile shapes
+        a.set_sparse_shape(4, 32, 0)
         ti, te = 0, 0
         seen = set()
         for k in range(6):
@@ -228,6 +230,8 @@ def test_engine_step_rebuilds_when_particles_leave_the_stencil(orc):
         for e in (a, b):
             e.set_particles(st)
             e.set_integration(integ)
+        # the passes of the entry points use the engine's tile shapes
+        a.set_sparse_shape(4, 32, 0)
         ti = te = 0
         rebuilt = 0
         for k in range(4):
@@ -313,3 +317,44 @@ def test_records_follow_fields_outside_passes(orc):
     for k in ra:
         np.testing.assert_array_equal(ra[k][act], rb[k][act],
                                       err_msg=f"kick2 {k}")
+
+
+def test_sparse_tile_shape_steps():
+    """Steps whose passes run with narrow tiles (4 particles against chunks
+    of 32) -- always, never, or by the default rule (fewer than 16 active
+    particles per cell: here the 10 % steps) -- find the same neighbours
+    (pass counts equal) and the same state to FP32 summation order."""
+    from swift_amd import Engine, HydroProps
+    st = _box()
+    props = HydroProps(ncell=(4, 4, 4), time_base=1e-5)
+    with Engine(props) as a, Engine(props) as b, Engine(props) as c:
+        for e in (a, b, c):
+            e.set_particles(st)
+            e.set_integration(_integ(fixed=True))
+        a.set_sparse_shape(4, 32, 1 << 30)  # every step
+        b.set_sparse_shape(4, 32, 0)        # none
+        kinds = set()
+        for k in range(8):
+            ia, ib, ic = a.step(), b.step(), c.step()
+            assert ia["sparse"] == 1 and ib["sparse"] == 0
+            kinds.add(ic["sparse"])
+            for i in (ia, ic):
+                for key in ("n_density", "n_gradient", "n_force", "ghost_rounds"):
+                    assert i["pass"][key] == ib["pass"][key], (k, key)
+                assert i["ti_end_min"] == ib["ti_end_min"]
+            sb = _state(b)
+            for e in (a, c):
+                se = _state(e)
+                np.testing.assert_array_equal(se["time_bin"], sb["time_bin"])
+                for key in ("x", "v", "v_full", "u", "u_full", "h", "rho"):
+                    # (velocity components through zero: absolute to 1e-7 of
+                    # the field's scale)
+                    np.testing.assert_allclose(
+                        se[key], sb[key], rtol=2e-5,
+                        atol=1e-7 * float(np.abs(sb[key]).max()),
+                        err_msg=f"{k} {key}")
+                for key in ("a_hydro", "u_dt"):
+                    scale = np.abs(sb[key]).max()
+                    assert np.abs(se[key] - sb[key]).max() <= 1e-4 * scale, (k, key)
+        # the default rule switched between the shapes
+        assert kinds == {0, 1}
