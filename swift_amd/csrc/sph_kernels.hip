@@ -84,17 +84,25 @@ __global__ __launch_bounds__(kBoundsBlock) void k_chunk_bounds(
 // every store instruction over the wave's whole span).
 constexpr int kRecBlock = 256;
 
+// Store words of a record type: 16 bytes, or 8 for records of 8-byte
+// multiples.
+template <class R>
+using RecWord = typename std::conditional<sizeof(R) % 16 == 0, uint4,
+                                          uint2>::type;
+
 template <class R>
 __device__ __forceinline__ void store_wave_records(R *dst, int i0, int cnt,
-                                                   const R &r, uint4 *lds) {
-  constexpr int Q = (int)(sizeof(R) / 16);
-  static_assert(sizeof(R) % 16 == 0, "records are whole 16-byte words");
+                                                   const R &r,
+                                                   RecWord<R> *lds) {
+  using W = RecWord<R>;
+  constexpr int Q = (int)(sizeof(R) / sizeof(W));
+  static_assert(sizeof(R) % 8 == 0, "records are whole 8-byte words");
   const int lane = threadIdx.x & 63;
-  const uint4 *src = reinterpret_cast<const uint4 *>(&r);
+  const W *src = reinterpret_cast<const W *>(&r);
 #pragma unroll
   for (int q = 0; q < Q; q++) lds[lane * Q + q] = src[q];
   __syncthreads();
-  uint4 *out = reinterpret_cast<uint4 *>(dst + i0);
+  W *out = reinterpret_cast<W *>(dst + i0);
   const int words = cnt * Q;
   for (int k = lane; k < words; k += 64) out[k] = lds[k];
 }
@@ -110,13 +118,14 @@ __device__ __forceinline__ void build_records(const DevParts &d, R *dst,
     return;
   }
   const int i = lo + blockIdx.x * kRecBlock + threadIdx.x;
-  __shared__ uint4 buf[kRecBlock * sizeof(R) / 16];
+  using W = RecWord<R>;
+  __shared__ W buf[kRecBlock * sizeof(R) / sizeof(W)];
   R r{};
   if (i < hi) r = make(i);
   const int w = threadIdx.x >> 6;
   const int i0 = lo + blockIdx.x * kRecBlock + (w << 6);
   store_wave_records(dst, i0, max(0, min(64, hi - i0)), r,
-                     buf + w * 64 * (int)(sizeof(R) / 16));
+                     buf + w * 64 * (int)(sizeof(R) / sizeof(W)));
 }
 
 __global__ __launch_bounds__(kRecBlock) void k_build_dens_rec(DevParts d,

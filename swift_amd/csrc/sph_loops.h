@@ -46,7 +46,9 @@ __device__ __forceinline__ DensRec dens_rec(const DevParts &d, int i) {
   r.vy = d.vy[i];
   r.vz = d.vz[i];
   r.m = d.m[i];
-  for (int k = 0; k < SWSPH_DENS_PAD; k++) r.pad[k] = 0.f;
+#if !SWSPH_COMPACT_RECORDS
+  r.pad[0] = r.pad[1] = 0.f;
+#endif
   return r;
 }
 
@@ -65,7 +67,9 @@ __device__ __forceinline__ GradRec grad_rec(const DevParts &d, int i) {
   r.c = d.c[i];
   r.alpha = d.alpha[i];
   r.pad0 = 0.f;
-  for (int k = 0; k < SWSPH_GRAD_PAD; k++) r.pad[k] = 0.f;
+#if !SWSPH_COMPACT_RECORDS
+  r.pad[0] = r.pad[1] = 0.f;
+#endif
   return r;
 }
 
@@ -85,7 +89,9 @@ __device__ __forceinline__ ForceRec force_rec(const DevParts &d, int i) {
   r.inv_m = 1.f / r.m;
   r.h_inv = 1.f / h;
   const float hi2 = r.h_inv * r.h_inv;
+#if !SWSPH_COMPACT_RECORDS
   r.h_inv4 = hi2 * hi2;
+#endif
   r.rho = d.rho[i];
   r.inv_rho = 1.f / r.rho;
   r.P = d.P[i];
@@ -97,8 +103,10 @@ __device__ __forceinline__ ForceRec force_rec(const DevParts &d, int i) {
   r.Pad = r.P * d.adiff[i];
   r.u = d.u[i];
   r.tb = (float)tb;
+#if !SWSPH_COMPACT_RECORDS
   r.pad0 = 0.f;
-  for (int k = 0; k < SWSPH_FORCE_PAD; k++) r.pad[k] = 0.f;
+  r.pad[0] = r.pad[1] = 0.f;
+#endif
   return r;
 }
 
@@ -332,7 +340,7 @@ struct ForceLoop {
     s.inv_m = r.inv_m;
     s.hg2 = r.hg2;
     s.h_inv = r.h_inv;
-    s.h_inv4 = r.h_inv4;
+    s.h_inv4 = (r.h_inv * r.h_inv) * (r.h_inv * r.h_inv);
     s.inv_rho = r.inv_rho;
     s.rho = r.rho;
     s.pr2 = r.pr2;
@@ -398,7 +406,12 @@ struct ForceLoop {
     kernel_deval(r * s.h_inv, wi, wi_dx);
     kernel_deval(r * j.h_inv, wj, wj_dx);
     const float wi_dr = s.h_inv4 * wi_dx;
+#if SWSPH_COMPACT_RECORDS
+    const float hj2 = j.h_inv * j.h_inv;
+    const float wj_dr = (hj2 * hj2) * wj_dx;
+#else
     const float wj_dr = j.h_inv4 * wj_dx;
+#endif
     const float dvdr =
         fmaf(s.vz - j.vz, dz, fmaf(s.vy - j.vy, dy, (s.vx - j.vx) * dx));
     const float mu = r_inv * fminf(dvdr, 0.f);
