@@ -192,6 +192,27 @@ int swsph_get_chunk(swsph_engine *e, int which, int *C);
  * particles per cell (64 / CI candidate streams per particle).
  * swsph_set_chunk(e, which, C) is swsph_set_tile_shape(e, which, C, C). */
 int swsph_set_tile_shape(swsph_engine *e, int which, int CI, int CJ);
+
+/* A task released by the scheduler: a self task on cell ci, or a pair task
+ * on the adjacent cells ci, cj (both directions, as runner_dopair1_* /
+ * runner_dopair2_*). */
+enum { SWSPH_TASK_SELF = 0, SWSPH_TASK_PAIR = 1 };
+typedef struct swsph_task {
+  int type;
+  int ci, cj;
+} swsph_task;
+/* Runs a bundle of released tasks of one loop (0 density, 1 gradient,
+ * 2 force) in one launch: the scheduler -> device task export of a
+ * task-based engine (scheduler_done releasing tasks, src/scheduler.c:1203;
+ * the hydro loop tasks of engine_make_hydroloop_tasks_mapper,
+ * src/engine_maketasks.c:3501).  The same interactions as
+ * swsph_doself1_density / swsph_dopair1_density ... for each task, merged per
+ * i-cell (one wave per chunk of every cell the bundle touches); a cell pair
+ * given twice is an error.  Bundles run in call order on the engine's
+ * stream, which carries the dependencies between them (density bundle ->
+ * swsph_do_ghost -> gradient bundle -> ...).  Status 0 on success. */
+int swsph_run_tasks(swsph_engine *e, int loop, const swsph_task *tasks,
+                    int ntasks);
 /* Tile shape of the passes of swsph_engine_step when its active particles
  * number fewer than per_cell per top-level cell on average (work-queue
  * schedules only): all three loops then use tiles of CI against chunks of

@@ -20,6 +20,7 @@
 #include <mutex>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/swsph.h"
@@ -152,6 +153,10 @@ struct swsph_engine {
   bool started = false;
   unsigned int *d_dmax = nullptr;
   unsigned long long *d_tiend = nullptr;  // [1] ti_end_min
+  // task bundles: device copies of the merged items and neighbour images
+  BundleItem *d_bitems = nullptr;
+  BundleNbr *d_bnbrs = nullptr;
+  size_t cap_bitems = 0, cap_bnbrs = 0;
   unsigned int *d_err = nullptr;          // [3] time-step errors, active,
                                           // active count before a step
   unsigned int *h_small = nullptr;        // pinned readback: dmax, hg2
@@ -1349,6 +1354,8 @@ int swsph_engine_destroy(swsph_engine *e) {
   if (e->d_dmax) (void)hipFree(e->d_dmax);
   if (e->d_tiend) (void)hipFree(e->d_tiend);
   if (e->d_err) (void)hipFree(e->d_err);
+  if (e->d_bitems) (void)hipFree(e->d_bitems);
+  if (e->d_bnbrs) (void)hipFree(e->d_bnbrs);
   if (e->h_small) (void)hipHostFree(e->h_small);
   if (e->h_tiend) (void)hipHostFree(e->h_tiend);
   if (e->h_err) (void)hipHostFree(e->h_err);
@@ -1613,6 +1620,100 @@ int swsph_init_cell(swsph_engine *e, int ci) {
     std::lock_guard<std::mutex> lk(e->mtx);                         \
     return task(e, kind, subset, ci, ci, false);                    \
   }
+
+// A bundle of released tasks in one launch (the scheduler -> device task
+// export): the directed interactions of all tasks are merged per i-cell into
+// a list of neighbour images, and every chunk of such an i-cell becomes one
+// work item.  Equivalent to running the tasks one by one (the same
+// interactions; FP32 sums in another order).
+int swsph_run_tasks(swsph_engine *e, int loop, const swsph_task *tasks,
+                    int ntasks) {
+  if (!e || (!tasks && ntasks > 0)) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (loop < 0 || loop > 2) return fail(1, "loop index %d out of range", loop);
+  if (ntasks < 0) return fail(1, "negative task count");
+  if (ntasks == 0) return 0;
+  if (e->need_particles) return fail(1, "no particles for the current grid");
+  const LoopKind k = (LoopKind)loop;
+  // neighbour images per i-cell, in task order
+  std::vector<int> cells;                       // i-cells with a list
+  std::vector<std::vector<BundleNbr>> lists;
+  std::unordered_map<int, int> slot;            // i-cell -> index in lists
+  auto add = [&](int ci, int cj, const double t[3]) -> int {
+    auto it = slot.find(ci);
+    int q;
+    if (it == slot.end()) {
+      q = (int)lists.size();
+      slot.emplace(ci, q);
+      cells.push_back(ci);
+      lists.emplace_back();
+    } else {
+      q = it->second;
+    }
+    for (const BundleNbr &b : lists[q])
+      if (b.cell == cj && b.t[0] == t[0] && b.t[1] == t[1] && b.t[2] == t[2])
+        return fail(1, "cell %d meets cell %d twice in one bundle", ci, cj);
+    BundleNbr b{};
+    b.cell = cj;
+    for (int a = 0; a < 3; a++) b.t[a] = t[a];
+    lists[q].push_back(b);
+    return 0;
+  };
+  int rc = 0;
+  for (int n = 0; n < ntasks; n++) {
+    const swsph_task &tk = tasks[n];
+    if ((rc = cell_ok(e, tk.ci))) return rc;
+    const double zero[3] = {0.0, 0.0, 0.0};
+    if (tk.type == SWSPH_TASK_SELF) {
+      if ((rc = add(tk.ci, tk.ci, zero))) return rc;
+    } else if (tk.type == SWSPH_TASK_PAIR) {
+      if ((rc = cell_ok(e, tk.cj))) return rc;
+      double t[3];
+      if (tk.ci == tk.cj || !adjacent(e, tk.ci, tk.cj, t))
+        return fail(1, "cells %d and %d are not neighbours", tk.ci, tk.cj);
+      const double mt[3] = {-t[0], -t[1], -t[2]};
+      if ((rc = add(tk.ci, tk.cj, t)) || (rc = add(tk.cj, tk.ci, mt))) return rc;
+    } else {
+      return fail(1, "task %d: unknown type %d", n, tk.type);
+    }
+  }
+  if ((rc = ensure_records(e))) return rc;
+  const swsph_engine::ChunkTable *tb = chunk_table(e, e->loop_c[loop], &rc);
+  if (!tb) return rc;
+  std::vector<BundleItem> items;
+  std::vector<BundleNbr> nbrs;
+  for (size_t q = 0; q < cells.size(); q++) {
+    const int c = cells[q];
+    const int off = (int)nbrs.size();
+    nbrs.insert(nbrs.end(), lists[q].begin(), lists[q].end());
+    for (int ch = tb->cell_chunk_h[c]; ch < tb->cell_chunk_h[c + 1]; ch++)
+      items.push_back(BundleItem{ch, off, (int)lists[q].size(), 0});
+  }
+  if (items.empty()) return 0;
+  if (items.size() > e->cap_bitems) {
+    if (e->d_bitems) (void)hipFree(e->d_bitems);
+    e->d_bitems = nullptr;
+    HIPCHK(hipMalloc((void **)&e->d_bitems, items.size() * sizeof(BundleItem)));
+    e->cap_bitems = items.size();
+  }
+  if (nbrs.size() > e->cap_bnbrs) {
+    if (e->d_bnbrs) (void)hipFree(e->d_bnbrs);
+    e->d_bnbrs = nullptr;
+    HIPCHK(hipMalloc((void **)&e->d_bnbrs, nbrs.size() * sizeof(BundleNbr)));
+    e->cap_bnbrs = nbrs.size();
+  }
+  // the host vectors must outlive the copies: wait for them
+  HIPCHK(hipMemcpyAsync(e->d_bitems, items.data(),
+                        items.size() * sizeof(BundleItem),
+                        hipMemcpyHostToDevice, e->st));
+  HIPCHK(hipMemcpyAsync(e->d_bnbrs, nbrs.data(), nbrs.size() * sizeof(BundleNbr),
+                        hipMemcpyHostToDevice, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  BundleArgs ba{e->d_bitems, e->d_bnbrs, (int)items.size()};
+  HIPCHK(sphk::launch_bundle(k, e->d, loop_args(e), chunk_args(tb), ba, e->st));
+  e->launches++;
+  return 0;
+}
 
 SELF_ENTRY(swsph_doself1_density, LoopKind::Density, false)
 TASK_ENTRY(swsph_dopair1_density, LoopKind::Density, false, true)

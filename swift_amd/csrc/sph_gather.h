@@ -296,7 +296,8 @@ __device__ __forceinline__ void run_chunk(
     const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
     const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
     int ich, bool stencil, int cj0, double tx0, double ty0, double tz0,
-    const NbrList &nl, const TileArgs *tq = nullptr, int tile = 0) {
+    const NbrList &nl, const TileArgs *tq = nullptr, int tile = 0,
+    const BundleNbr *nbl = nullptr, int nnb = 0) {
   using Sh = GatherLds<CI, CJ>;
   constexpr int S = Sh::S, RPC = Sh::RPC;
   constexpr int C = CI;
@@ -349,13 +350,19 @@ __device__ __forceinline__ void run_chunk(
   const float hg2i = wave_max(act ? rt2 : 0.f);
   int ncnt = 0;  // list entries found by this lane (BUILD)
 
-  // Neighbour table: lane q < nq describes neighbour q.
-  const int nq = stencil ? 27 : 1;
+  // Neighbour table: lane q < nq describes neighbour q (the 27-cell
+  // stencil, one cell, or the nnb <= 27 images of a task bundle item).
+  const int nq = nbl ? nnb : (stencil ? 27 : 1);
   int nchq = 0;
   if (lane < nq) {
     double tx = tx0, ty = ty0, tz = tz0;
     int cj = cj0;
-    if (stencil) {
+    if (nbl) {
+      cj = nbl[lane].cell;
+      tx = nbl[lane].t[0];
+      ty = nbl[lane].t[1];
+      tz = nbl[lane].t[2];
+    } else if (stencil) {
       int cx, cy, cz;
       cell_coords(cell_i, la.grid, cx, cy, cz);
       cj = neighbour(la.grid, cx, cy, cz, kStencilOrder[lane], tx, ty, tz);
@@ -606,6 +613,19 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_stencil(
   if (ich >= ca.nchunks) return;  // uniform per wave
   run_chunk<L, C, C, SUBSET, BUILD>(d, la, ca, rec, pos, ich, true, 0, 0.0,
                                     0.0, 0.0, nl);
+}
+
+// Task bundle: one wave per item (an i-chunk and its neighbour images).
+template <class L, int C>
+__global__ __launch_bounds__(64) void k_loop_bundle(
+    DevParts d, LoopArgs la, ChunkArgs ca,
+    const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
+    BundleArgs ba) {
+  const BundleItem it = ba.items[blockIdx.x];
+  const NbrList none{};
+  run_chunk<L, C, C, false, false, false, 1>(
+      d, la, ca, rec, pos, uniform(it.ich), false, 0, 0.0, 0.0, 0.0, none,
+      nullptr, 0, ba.nbrs + uniform(it.off), uniform(it.n));
 }
 
 // Per task: up to two directed (i-cell <- j-cell) interactions; blocks

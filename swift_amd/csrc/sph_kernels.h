@@ -176,6 +176,22 @@ struct TaskArgs {
   double t[6];
 };
 
+// A bundle of released self / pair tasks, merged per i-chunk: item k runs
+// i-chunk items[k].ich against the neighbour images nbrs[off .. off + n)
+// (at most 27: the j-cell and the translation applied to its positions).
+struct BundleNbr {
+  int cell, pad;
+  double t[3];
+};
+struct BundleItem {
+  int ich, off, n, pad;
+};
+struct BundleArgs {
+  const BundleItem *items;
+  const BundleNbr *nbrs;
+  int nitems;
+};
+
 struct GhostArgs {
   float eta_dim, eps, h_max, h_min;
   int max_active_bin;
@@ -263,6 +279,10 @@ hipError_t launch_queue(LoopKind kind, bool build_lists, const DevParts &d,
 hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
                        const LoopArgs &la, const ChunkArgs &ca,
                        const TaskArgs &ta, hipStream_t st);
+// One launch for a task bundle: one wave per item.
+hipError_t launch_bundle(LoopKind kind, const DevParts &d, const LoopArgs &la,
+                         const ChunkArgs &ca, const BundleArgs &ba,
+                         hipStream_t st);
 // Positions / search radii of the candidate test for particles [lo, hi).
 hipError_t launch_build_pos(const DevParts &d, const GridDesc &g, float pos_eps,
                             int lo, int hi, hipStream_t st);

@@ -923,6 +923,37 @@ hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
   return hipErrorInvalidValue;
 }
 
+template <class L>
+hipError_t bundle_any(const DevParts &d, const LoopArgs &la,
+                      const ChunkArgs &ca, const typename L::Rec *rec,
+                      const BundleArgs &ba, hipStream_t st) {
+  if (ba.nitems <= 0) return hipSuccess;
+  const dim3 g(ba.nitems), b(64);
+  const PosRec *p = (const PosRec *)d.pos;
+  switch (ca.C) {
+    case 8: hipLaunchKernelGGL((k_loop_bundle<L, 8>), g, b, 0, st, d, la, ca, rec, p, ba); break;
+    case 16: hipLaunchKernelGGL((k_loop_bundle<L, 16>), g, b, 0, st, d, la, ca, rec, p, ba); break;
+    case 32: hipLaunchKernelGGL((k_loop_bundle<L, 32>), g, b, 0, st, d, la, ca, rec, p, ba); break;
+    case 64: hipLaunchKernelGGL((k_loop_bundle<L, 64>), g, b, 0, st, d, la, ca, rec, p, ba); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_bundle(LoopKind kind, const DevParts &d, const LoopArgs &la,
+                         const ChunkArgs &ca, const BundleArgs &ba,
+                         hipStream_t st) {
+  switch (kind) {
+    case LoopKind::Density:
+      return bundle_any<DensityLoop>(d, la, ca, d.rd, ba, st);
+    case LoopKind::Gradient:
+      return bundle_any<GradientLoop>(d, la, ca, d.rg, ba, st);
+    case LoopKind::Force:
+      return bundle_any<ForceLoop>(d, la, ca, d.rf, ba, st);
+  }
+  return hipErrorInvalidValue;
+}
+
 hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_u64, dim3(nblk(n, 64)), dim3(64), 0, st, p, n);
   return hipGetLastError();

@@ -234,6 +234,18 @@ class Engine:
         check(lib().swsph_do_end_hydro_force(self._h, ci),
               "runner_do_end_hydro_force")
 
+    # -- task bundles (scheduler -> device export) ----------------------------
+    TASK_SELF, TASK_PAIR = 0, 1
+
+    def run_tasks(self, loop: str, tasks) -> None:
+        """Runs a bundle of released tasks of one loop in one launch: rows
+        (type, ci, cj) with type TASK_SELF (cj ignored) or TASK_PAIR; the
+        same interactions as the per-task entry points, merged per i-cell."""
+        t = np.ascontiguousarray(np.asarray(tasks, np.int32).reshape(-1, 3))
+        check(lib().swsph_run_tasks(self._h, self.LOOPS[loop],
+                                    t.ctypes.data_as(C.c_void_p), t.shape[0]),
+              "swsph_run_tasks")
+
     # -- whole passes --------------------------------------------------------
     def pass_plain(self) -> dict:
         s = PassStats()

@@ -1,0 +1,139 @@
+"""Task bundles (swsph_run_tasks): the scheduler -> device task export.
+
+A bundle of released self / pair tasks runs in one launch, the directed
+interactions merged per i-cell.  It must do exactly what the per-task entry
+points do for the same tasks (runner_doself1_branch_density /
+runner_dopair1_branch_density, the gradient instance, runner_doself2 /
+dopair2 force; src/runner_doiact_hydro.h:177-195): the same neighbour counts
+and the same fields to FP32 summation order, whether the tasks come as one
+bundle or as several bundles in any order.  The h tolerance is loose so
+that the ghost keeps the initial smoothing lengths and every path meets the
+same neighbour sets.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SELF, PAIR = 0, 1
+FIELDS = ["rho", "wcount", "div_v", "rot_v", "v_sig", "laplace_u",
+          "a_hydro", "u_dt", "h_dt"]
+COUNTS = ["n_density", "n_gradient", "n_force", "min_ngb_time_bin"]
+
+
+def _tasks(eng):
+    """Every self task and the 13 upper pair tasks of every cell."""
+    out = []
+    for c in range(eng.num_cells()):
+        cx, cy, cz = eng.cell_coords(c)
+        out.append((SELF, c, c))
+        for o in range(14, 27):
+            cj = eng.cell_id(cx + o // 9 - 1, cy + (o // 3) % 3 - 1,
+                             cz + o % 3 - 1)
+            out.append((PAIR, c, cj))
+    return out
+
+
+def _pass(eng, tasks, bundles):
+    """The SWIFT task sequence of one pass: init, density tasks, ghost,
+    gradient tasks, extra ghost, force tasks, end force; the loop tasks
+    either one by one (bundles = None) or as the given bundles."""
+    n = eng.num_cells()
+    for c in range(n):
+        eng.hydro_init_cell(c)
+    per_task = {
+        "density": (eng.runner_doself1_branch_density,
+                    eng.runner_dopair1_branch_density),
+        "gradient": (eng.runner_doself1_branch_gradient,
+                     eng.runner_dopair1_branch_gradient),
+        "force": (eng.runner_doself2_branch_force,
+                  eng.runner_dopair2_branch_force)}
+
+    def loop(name):
+        if bundles is None:
+            self_fn, pair_fn = per_task[name]
+            for t, ci, cj in tasks:
+                if t == SELF:
+                    self_fn(ci)
+                else:
+                    pair_fn(ci, cj)
+        else:
+            for b in bundles:
+                eng.run_tasks(name, [tasks[i] for i in b])
+
+    loop("density")
+    for c in range(n):
+        eng.runner_do_ghost(c)
+    loop("gradient")
+    for c in range(n):
+        eng.runner_do_extra_ghost(c)
+    loop("force")
+    for c in range(n):
+        eng.runner_do_end_hydro_force(c)
+    return eng.results(FIELDS + COUNTS + ["h"])
+
+
+@pytest.mark.parametrize("box", ["modulated", "clustered"])
+def test_bundles_match_per_task_entry_points(box):
+    from swift_amd import Engine, HydroProps, ics
+    props = HydroProps(ncell=(4, 4, 4), h_tolerance=1.0, max_iter=30)
+    if box == "modulated":
+        st = ics.modulated_box(16, seed=21, amp=0.4)
+    else:
+        st = ics.clustered_box(6000, seed=22, n_spheres=6, a_min=0.02,
+                               a_max=0.05)
+        # the inactive part brings the state of an all-active pass
+        with Engine(props) as eng:
+            eng.set_particles(st)
+            eng.pass_batched()
+            base = eng.results()
+        for k in ("rho", "pressure", "soundspeed", "f", "balsara",
+                  "alpha_visc", "alpha_diff"):
+            st[k] = base[k].astype(st[k].dtype)
+        ics.set_active_fraction(st, 0.6, seed=4)
+    res = {}
+    for mode in ("tasks", "one", "split"):
+        with Engine(props) as eng:
+            eng.set_particles(st)
+            tasks = _tasks(eng)
+            if mode == "tasks":
+                bundles = None
+            elif mode == "one":
+                bundles = [list(range(len(tasks)))]
+            else:  # three bundles, tasks in a random order
+                order = np.random.default_rng(5).permutation(len(tasks))
+                bundles = [list(order[k::3]) for k in range(3)]
+            res[mode] = _pass(eng, tasks, bundles)
+    ref = res["tasks"]
+    assert np.array_equal(ref["h"], st["h"])      # the ghost kept h
+    act = st["time_bin"] <= props.max_active_bin
+    assert ref["n_density"][act].min() > 0
+    for mode in ("one", "split"):
+        r = res[mode]
+        for k in COUNTS:
+            np.testing.assert_array_equal(r[k], ref[k], err_msg=f"{mode} {k}")
+        for k in FIELDS:
+            a, b = r[k][act].astype(np.float64), ref[k][act].astype(np.float64)
+            scale = np.abs(b).max()
+            assert np.abs(a - b).max() <= 2e-5 * scale, (mode, k)
+
+
+def test_bundle_errors():
+    from swift_amd import Engine, HydroProps, ics
+    from swift_amd._lib import SwsphError
+    st = ics.modulated_box(12, seed=2)
+    with Engine(HydroProps(ncell=(4, 4, 4))) as eng:
+        eng.set_particles(st)
+        a, b = eng.cell_id(0, 0, 0), eng.cell_id(1, 0, 0)
+        far = eng.cell_id(2, 2, 2)
+        with pytest.raises(SwsphError, match="twice"):
+            eng.run_tasks("density", [(PAIR, a, b), (PAIR, b, a)])
+        with pytest.raises(SwsphError, match="twice"):
+            eng.run_tasks("force", [(SELF, a, a), (SELF, a, a)])
+        with pytest.raises(SwsphError, match="not neighbours"):
+            eng.run_tasks("density", [(PAIR, a, far)])
+        with pytest.raises(SwsphError, match="unknown type"):
+            eng.run_tasks("gradient", [(7, a, a)])
+        eng.run_tasks("density", [])              # an empty bundle is no work
