@@ -188,8 +188,9 @@ int swsph_get_chunk(swsph_engine *e, int which, int *C);
  * and 2 of swsph_set_schedule) sized apart from the tiles: tiles of CI
  * i-particles against candidate chunks of CJ particles, CJ = CI or
  * CJ = CI / 2 for CI = 16, 32 (finer candidate culling under the same tile),
- * or the narrow tiles (4, 16), (4, 32), (8, 16) for steps with few active
- * particles per cell (64 / CI candidate streams per particle).
+ * the narrow tiles (4, 16), (4, 32), (8, 16) for steps with few active
+ * particles per cell (64 / CI candidate streams per particle), or (16, 32)
+ * (coarser chunks: fewer chunk visits, looser culling).
  * swsph_set_chunk(e, which, C) is swsph_set_tile_shape(e, which, C, C). */
 int swsph_set_tile_shape(swsph_engine *e, int which, int CI, int CJ);
 

@@ -1928,7 +1928,8 @@ int swsph_set_tile_shape(swsph_engine *e, int which, int CI, int CJ) {
   if (which < 0 || which > 2) return fail(1, "loop index %d out of range", which);
   const bool ok = (CJ == CI && (CI == 8 || CI == 16 || CI == 32 || CI == 64)) ||
                   (CJ * 2 == CI && (CI == 16 || CI == 32)) ||
-                  (CI == 4 && (CJ == 16 || CJ == 32)) || (CI == 8 && CJ == 16);
+                  (CI == 4 && (CJ == 16 || CJ == 32)) || (CI == 8 && CJ == 16) ||
+                  (CI == 16 && CJ == 32);
   if (!ok) return fail(1, "tile shape (%d, %d) not supported", CI, CJ);
   e->loop_c[which] = CI;
   e->loop_cj[which] = CJ;

@@ -818,6 +818,7 @@ hipError_t queue_any(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
     case 416: return queue_c<L, 4, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
     case 432: return queue_c<L, 4, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
     case 816: return queue_c<L, 8, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 1632: return queue_c<L, 16, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
   }
   return hipErrorInvalidValue;
 }

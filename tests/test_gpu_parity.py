@@ -148,7 +148,8 @@ def test_chunk_shapes(orc, C):
     _check(orc, props, st, res)
 
 
-@pytest.mark.parametrize("shape", [(16, 8), (32, 16), (8, 16), (4, 16), (4, 32)])
+@pytest.mark.parametrize("shape", [(16, 8), (32, 16), (8, 16), (4, 16), (4, 32),
+                                   (16, 32)])
 @pytest.mark.parametrize("schedule", [1, 2])
 def test_tile_shapes(orc, shape, schedule):
     """Work-queue tiles against finer candidate chunks, and narrow tiles
