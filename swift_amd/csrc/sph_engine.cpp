@@ -399,10 +399,17 @@ ChunkArgs chunk_args(const swsph_engine::ChunkTable *t) {
   return ca;
 }
 
+// Capacity of the chunk table whose chunks serve as i-chunks (grid
+// schedule, per-task entry points, task bundles): the tile capacity, or for
+// the narrow tiles of the work queue (fewer than 8) their candidate chunks.
+int chunk_c(const swsph_engine *e, int k) {
+  return e->loop_c[k] >= 8 ? e->loop_c[k] : e->loop_cj[k];
+}
+
 // Chunk capacity of the table a loop reads: CJ in the work-queue schedules,
-// else the tile capacity (i-chunks are chunks of the table).
+// else the i-chunk capacity.
 int table_c(const swsph_engine *e, int k) {
-  return e->schedule >= 1 ? e->loop_cj[k] : e->loop_c[k];
+  return e->schedule >= 1 ? e->loop_cj[k] : chunk_c(e, k);
 }
 
 // Tables no loop reads any more are dropped from the per-pass refresh
@@ -411,7 +418,7 @@ void prune_tables(swsph_engine *e) {
   for (auto &t : e->tables) {
     bool used = false;
     for (int k = 0; k < 3; k++)
-      used = used || t.C == table_c(e, k) || t.C == e->loop_c[k];
+      used = used || t.C == table_c(e, k) || t.C == chunk_c(e, k);
     if (!used) t.valid = false;
   }
 }
@@ -612,7 +619,7 @@ int time_end(swsph_engine *e, LoopKind k, hipEvent_t a) {
 int stencil(swsph_engine *e, LoopKind k, bool subset,
             const int *flag = nullptr) {
   int rc = 0;
-  const swsph_engine::ChunkTable *t = chunk_table(e, e->loop_c[(int)k], &rc);
+  const swsph_engine::ChunkTable *t = chunk_table(e, chunk_c(e, (int)k), &rc);
   if (!t) return rc;
   hipEvent_t a;
   if ((rc = time_begin(e, &a))) return rc;
@@ -746,7 +753,7 @@ int task(swsph_engine *e, LoopKind k, bool subset, int ci, int cj, bool both) {
   double t[3];
   if (!adjacent(e, ci, cj, t))
     return fail(1, "cells %d and %d are not neighbours", ci, cj);
-  const swsph_engine::ChunkTable *tb = chunk_table(e, e->loop_c[(int)k], &rc);
+  const swsph_engine::ChunkTable *tb = chunk_table(e, chunk_c(e, (int)k), &rc);
   if (!tb) return rc;
   auto nch = [&](int c) { return tb->cell_chunk_h[c + 1] - tb->cell_chunk_h[c]; };
   ta.icell[0] = ci;
@@ -1678,7 +1685,7 @@ int swsph_run_tasks(swsph_engine *e, int loop, const swsph_task *tasks,
     }
   }
   if ((rc = ensure_records(e))) return rc;
-  const swsph_engine::ChunkTable *tb = chunk_table(e, e->loop_c[loop], &rc);
+  const swsph_engine::ChunkTable *tb = chunk_table(e, chunk_c(e, loop), &rc);
   if (!tb) return rc;
   std::vector<BundleItem> items;
   std::vector<BundleNbr> nbrs;
@@ -2122,25 +2129,40 @@ int swsph_autotune(swsph_engine *e, int repeats, swsph_tune_result *out) {
     }
   }
   if (!rc && best_s >= 1) {
-    // Under the work queue: candidate chunks of half the tile size.
+    // Under the work queue: candidate chunks of half the tile size, each
+    // loop keeping its faster shape.  (The narrow tiles are not tried: the
+    // trials time eager passes, which launch one wave per tile, while a
+    // captured graph launches a grid for the largest possible tile count,
+    // four times larger for tiles of 4 -- on the clustered box the narrow
+    // shapes won 11 % of loop time eagerly and lost 6 % per replayed step.
+    // engine_step, which runs eager passes, takes them by its own rule.)
     e->schedule = best_s;
     e->queue_waves = 0;
-    bool any = false;
-    for (int k = 0; k < 3; k++) {
-      e->loop_c[k] = best_c[k];
-      const bool half = (best_c[k] == 16 || best_c[k] == 32) &&
-                        !(e->use_lists && k == 0);
-      e->loop_cj[k] = half ? best_c[k] / 2 : best_c[k];
-      any = any || half;
-    }
-    double ms[3], w = 0;
-    if (any && !(rc = time_config(ms, &w))) {
+    int best_ci[3] = {best_c[0], best_c[1], best_c[2]};
+    for (int trial = 0; trial < 1 && !rc; trial++) {
+      bool any = false;
+      for (int k = 0; k < 3; k++) {
+        int ci = best_c[k], cj = best_c[k];
+        if (!(e->use_lists && k == 0)) {
+          if (trial == 0 && (best_c[k] == 16 || best_c[k] == 32)) cj = ci / 2;
+          if (trial == 1) ci = 4, cj = 32;
+          if (trial == 2) ci = 8, cj = 16;
+        }
+        e->loop_c[k] = ci;
+        e->loop_cj[k] = cj;
+        any = any || ci != best_c[k] || cj != best_c[k];
+      }
+      double ms[3], w = 0;
+      if (!any || (rc = time_config(ms, &w))) continue;
       for (int k = 0; k < 3; k++)
-        if (e->loop_cj[k] != best_c[k] && ms[k] < best_ms[k]) {
+        if ((e->loop_c[k] != best_c[k] || e->loop_cj[k] != best_c[k]) &&
+            ms[k] < best_ms[k]) {
+          best_ci[k] = e->loop_c[k];
           best_cj[k] = e->loop_cj[k];
           best_ms[k] = ms[k];
         }
     }
+    for (int k = 0; k < 3; k++) best_c[k] = best_ci[k];
   }
   // apply the choice (or restore the settings after a failure) and the state
   e->schedule = rc ? sched0 : best_s;
