@@ -125,4 +125,29 @@ __device__ __forceinline__ void for_runs(int lo, int hi,
   }
 }
 
+// The same over the entries of a compact particle list (list[0 .. *count),
+// e.g. the work queue's list of the particles a loop processed), launched
+// with run_blocks(capacity, blockDim.x) blocks: entries past the count are
+// skipped, so a list much shorter than the particle slots costs little.
+template <class K, class F>
+__device__ __forceinline__ void for_runs_list(const int *__restrict__ list,
+                                              const int *__restrict__ count,
+                                              const K *__restrict__ key, F f) {
+  const int n = *count;
+  const int base = blockIdx.x * blockDim.x * kRuns + threadIdx.x;
+  if (base >= n) return;
+  int idx[kRuns];
+#pragma unroll
+  for (int u = 0; u < kRuns; u++) {
+    const int k = base + u * (int)blockDim.x;
+    idx[u] = k < n ? list[k] : -1;
+  }
+  K k[kRuns];
+#pragma unroll
+  for (int u = 0; u < kRuns; u++) k[u] = idx[u] >= 0 ? key[idx[u]] : K{};
+#pragma unroll
+  for (int u = 0; u < kRuns; u++)
+    if (idx[u] >= 0) f(idx[u], k[u]);
+}
+
 }  // namespace sph

@@ -121,6 +121,11 @@ struct swsph_engine {
   // average) run their pass with tiles of sparse_ci against chunks of
   // sparse_cj in all three loops; 0 disables.
   int sparse_ci = 4, sparse_cj = 32, sparse_per_cell = 16;
+  // the ghost / extra ghost / force records / end force of a pass sweep the
+  // work queue's list instead of every slot (set by engine_step when under
+  // half of the particles are active; an all-active pass sweeps slots,
+  // 1 % faster there)
+  bool sweep_list = false;
   // Device capacity (particles, cells): buffers are reused by later
   // swsph_set_particles calls that fit, without reallocation.
   size_t cap_n = 0;
@@ -1136,10 +1141,27 @@ int phase_start(swsph_engine *e) {
   return stencil(e, LoopKind::Density, false);
 }
 
-// One ghost round over all particles (runner_do_ghost).
+// The work queue's compact list of the particles its last tile build
+// selected (cell order), and its length on the device; null when the pass
+// does not run through the queue.  After a density round it holds the
+// particles that round processed -- those the ghost finishes -- and after
+// the gradient and force loops all active particles.
+const int *queue_list(const swsph_engine *e, const int **cnt) {
+  if (!e->sweep_list || e->schedule < 1 || e->use_lists || e->n == 0) {
+    *cnt = nullptr;
+    return nullptr;
+  }
+  *cnt = e->ts.scan + e->n;
+  return e->ts.alist;
+}
+
+// One ghost round over the particles of the last density round
+// (runner_do_ghost).
 int phase_ghost(swsph_engine *e) {
+  const int *cnt;
+  const int *list = queue_list(e, &cnt);
   HIPCHK(sphk::launch_ghost(e->d, ghost_args(e), 0, (int)e->n, e->d_redo,
-                            e->st));
+                            e->st, list, cnt));
   e->launches += 2;
   return 0;
 }
@@ -1172,6 +1194,8 @@ int phase_finish(swsph_engine *e) {
   const bool queue = e->schedule >= 1;
   // the records of the inactive particles are current (ensure_records)
   const int mab = e->props.max_active_bin;
+  // (the queue's list holds the particles of the last density round here,
+  // a subset of the active ones: the records sweep all slots)
   HIPCHK(sphk::launch_build_records(LoopKind::Gradient, e->d, 0, n, e->st, mab));
   e->launches++;
   if ((rc = refresh_bounds(e, 0, e->ncells))) return rc;
@@ -1185,13 +1209,18 @@ int phase_finish(swsph_engine *e) {
                          : stencil(e, LoopKind::Gradient, false))) {
     return rc;
   }
-  HIPCHK(sphk::launch_extra_ghost(e->d, ghost_args(e), 0, n, e->st));
-  HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, 0, n, e->st, mab));
+  // after the gradient loop the queue's list holds all active particles
+  const int *cnt;
+  const int *list = queue_list(e, &cnt);
+  HIPCHK(sphk::launch_extra_ghost(e->d, ghost_args(e), 0, n, e->st, list, cnt));
+  HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, 0, n, e->st, mab,
+                                    list, cnt));
   e->launches += 2;
   if ((rc = queue ? queue_loop(e, LoopKind::Force, 0, nullptr, false)
                   : stencil(e, LoopKind::Force, false)))
     return rc;
-  HIPCHK(sphk::launch_end_force(e->d, 0, n, e->props.max_active_bin, e->st));
+  HIPCHK(sphk::launch_end_force(e->d, 0, n, e->props.max_active_bin, e->st,
+                                list, cnt));
   e->launches++;
   return 0;
 }
@@ -2274,16 +2303,19 @@ int swsph_engine_step(swsph_engine *e, swsph_step_info *info) {
   }
   const float dx_max = e->dmax;
   set_active_bin(e, max_active_bin_at(e->ti_current));
-  // narrow tiles when the active particles are few per cell
+  // narrow tiles when the active particles are few per cell, and list
+  // sweeps when they are under half of all
   bool sparse = false;
-  if (e->sparse_per_cell > 0 && e->schedule >= 1) {
+  if (e->schedule >= 1) {
     HIPCHK(sphk::launch_count_active(e->d, (int)e->n, e->props.max_active_bin,
                                      e->d_err + 2, e->st));
     HIPCHK(hipMemcpyAsync(e->h_err + 2, e->d_err + 2, sizeof(unsigned int),
                           hipMemcpyDeviceToHost, e->st));
     HIPCHK(hipStreamSynchronize(e->st));
-    sparse = (double)e->h_err[2] <
-             (double)e->sparse_per_cell * (double)e->ncells;
+    const double nact = (double)e->h_err[2];
+    sparse = e->sparse_per_cell > 0 &&
+             nact < (double)e->sparse_per_cell * (double)e->ncells;
+    e->sweep_list = 2.0 * nact < (double)e->n;
   }
   const int c0[3] = {e->loop_c[0], e->loop_c[1], e->loop_c[2]};
   const int cj0[3] = {e->loop_cj[0], e->loop_cj[1], e->loop_cj[2]};
@@ -2304,6 +2336,7 @@ int swsph_engine_step(swsph_engine *e, swsph_step_info *info) {
       rc = pass_batched_locked(e, &ps);
     }
   }
+  e->sweep_list = false;
   if (sparse) {
     for (int k = 0; k < 3; k++) {
       e->loop_c[k] = c0[k];

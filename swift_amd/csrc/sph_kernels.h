@@ -303,11 +303,19 @@ hipError_t launch_init_density(const DevParts &d, int lo, int hi,
 // interactions evaluated for the processed particles, accumulated over the
 // rounds (the pass zeroes it).  [0..2] are zeroed by each launch.
 hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,
-                        unsigned long long *counters, hipStream_t st);
+                        unsigned long long *counters, hipStream_t st,
+                        const int *list = nullptr, const int *cnt = nullptr);
+// The per-particle kernels below take either the particle range [lo, hi)
+// or (list != nullptr) the particles list[0 .. *cnt) of a compact list whose
+// capacity is hi - lo.
 hipError_t launch_extra_ghost(const DevParts &d, const GhostArgs &ga, int lo,
-                              int hi, hipStream_t st);
+                              int hi, hipStream_t st,
+                              const int *list = nullptr,
+                              const int *cnt = nullptr);
 hipError_t launch_end_force(const DevParts &d, int lo, int hi,
-                            int max_active_bin, hipStream_t st);
+                            int max_active_bin, hipStream_t st,
+                            const int *list = nullptr,
+                            const int *cnt = nullptr);
 hipError_t launch_count(const DevParts &d, int n, int max_active_bin,
                         unsigned long long *out3, hipStream_t st);
 // *out = number of active particles of [0, n) (device-side).
@@ -316,7 +324,9 @@ hipError_t launch_count_active(const DevParts &d, int n, int max_active_bin,
 // Records of particles [lo, hi); with max_active_bin >= 0 only those of the
 // active particles.
 hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
-                                int hi, hipStream_t st, int max_active_bin = -1);
+                                int hi, hipStream_t st, int max_active_bin = -1,
+                                const int *list = nullptr,
+                                const int *cnt = nullptr);
 
 // ---- binning and time integration (sph_step.hip) ----
 // Keys (cell << 30 | in-cell Morton code) of n positions, component k of

@@ -110,11 +110,16 @@ __device__ __forceinline__ void store_wave_records(R *dst, int i0, int cnt,
 template <class R, class Make>
 __device__ __forceinline__ void build_records(const DevParts &d, R *dst,
                                               int lo, int hi, int mab,
+                                              const int *list, const int *cnt,
                                               Make make) {
   if (mab >= 0) {  // uniform; launched with run_blocks(hi - lo, kRecBlock)
-    for_runs(lo, hi, d.tb, [&](int i, int8_t tb) {
+    auto f = [&](int i, int8_t tb) {
       if (is_active(tb, mab)) dst[i] = make(i);
-    });
+    };
+    if (list)
+      for_runs_list(list, cnt, d.tb, f);
+    else
+      for_runs(lo, hi, d.tb, f);
     return;
   }
   const int i = lo + blockIdx.x * kRecBlock + threadIdx.x;
@@ -128,22 +133,22 @@ __device__ __forceinline__ void build_records(const DevParts &d, R *dst,
                      buf + w * 64 * (int)(sizeof(R) / sizeof(W)));
 }
 
-__global__ __launch_bounds__(kRecBlock) void k_build_dens_rec(DevParts d,
-                                                              int lo, int hi,
-                                                              int mab) {
-  build_records(d, d.rd, lo, hi, mab, [&](int i) { return dens_rec(d, i); });
+__global__ __launch_bounds__(kRecBlock) void k_build_dens_rec(
+    DevParts d, int lo, int hi, int mab, const int *list, const int *cnt) {
+  build_records(d, d.rd, lo, hi, mab, list, cnt,
+                [&](int i) { return dens_rec(d, i); });
 }
 
-__global__ __launch_bounds__(kRecBlock) void k_build_grad_rec(DevParts d,
-                                                              int lo, int hi,
-                                                              int mab) {
-  build_records(d, d.rg, lo, hi, mab, [&](int i) { return grad_rec(d, i); });
+__global__ __launch_bounds__(kRecBlock) void k_build_grad_rec(
+    DevParts d, int lo, int hi, int mab, const int *list, const int *cnt) {
+  build_records(d, d.rg, lo, hi, mab, list, cnt,
+                [&](int i) { return grad_rec(d, i); });
 }
 
-__global__ __launch_bounds__(kRecBlock) void k_build_force_rec(DevParts d,
-                                                               int lo, int hi,
-                                                               int mab) {
-  build_records(d, d.rf, lo, hi, mab, [&](int i) { return force_rec(d, i); });
+__global__ __launch_bounds__(kRecBlock) void k_build_force_rec(
+    DevParts d, int lo, int hi, int mab, const int *list, const int *cnt) {
+  build_records(d, d.rf, lo, hi, mab, list, cnt,
+                [&](int i) { return force_rec(d, i); });
 }
 
 // ------------------------------------------------------ per-particle work --
@@ -320,25 +325,37 @@ __device__ bool ghost_one(const DevParts &d, const GhostArgs &ga, int i,
 // order.
 constexpr int kGhostBlock = 256;
 __global__ __launch_bounds__(kGhostBlock) void k_ghost(
-    DevParts d, GhostArgs ga, int lo, int hi, unsigned long long *counters) {
+    DevParts d, GhostArgs ga, int lo, int hi, unsigned long long *counters,
+    const int *list, const int *cnt) {
   constexpr int W = kGhostBlock / 64;
   __shared__ int wcnt[3][W];
   __shared__ unsigned long long wwork[W];
   __shared__ int base;
+  // run u of this thread: particle slot i0 + u * kGhostBlock, or with a
+  // compact list the particle of list entry k0 + u * kGhostBlock
   const int i0 = lo + blockIdx.x * kGhostBlock * kRuns + threadIdx.x;
-  int redo[kRuns];
+  const int k0 = blockIdx.x * kGhostBlock * kRuns + threadIdx.x;
+  const int nl = list ? *cnt : 0;
+  int idx[kRuns], redo[kRuns];
 #pragma unroll
   for (int u = 0; u < kRuns; u++) {
-    const int i = i0 + u * kGhostBlock;
-    redo[u] = i < hi ? d.redo[i] : 0;
+    if (list) {
+      const int k = k0 + u * kGhostBlock;
+      idx[u] = k < nl ? list[k] : -1;
+    } else {
+      const int i = i0 + u * kGhostBlock;
+      idx[u] = i < hi ? i : -1;
+    }
   }
+#pragma unroll
+  for (int u = 0; u < kRuns; u++) redo[u] = idx[u] >= 0 ? d.redo[idx[u]] : 0;
   unsigned int again = 0u, relist = 0u;  // bit u: run u
   // density interactions evaluated for these particles in the last round
   unsigned long long work = 0ull;
 #pragma unroll
   for (int u = 0; u < kRuns; u++) {
     if (redo[u]) {
-      const int i = i0 + u * kGhostBlock;
+      const int i = idx[u];
       work += (unsigned long long)d.nd[i];
       bool rl = false;
       if (ghost_one(d, ga, i, rl)) again |= 1u << u;
@@ -384,7 +401,7 @@ __global__ __launch_bounds__(kGhostBlock) void k_ghost(
     for (int u = 0; u < kRuns; u++) {
       const bool ag = (again >> u) & 1u;
       const unsigned long long ba = __ballot(ag);
-      if (ag) d.sel[off + __popcll(ba & below)] = i0 + u * kGhostBlock;
+      if (ag) d.sel[off + __popcll(ba & below)] = idx[u];
       off += __popcll(ba);
     }
   }
@@ -398,8 +415,9 @@ __global__ void k_fill_f32(float *p, float v, int n) {
 // runner_do_extra_ghost (runner_ghost.c:1016-1111): hydro_end_gradient,
 // hydro_prepare_force (hydro.h:840-954, a = 1), hydro_reset_acceleration and
 // timestep_limiter_prepare_force for active particles in [lo, hi).
-__global__ void k_extra_ghost(DevParts d, GhostArgs ga, int lo, int hi) {
-  for_runs(lo, hi, d.tb, [&](int i, int8_t tb) {
+__global__ void k_extra_ghost(DevParts d, GhostArgs ga, int lo, int hi,
+                              const int *list, const int *cnt) {
+  auto body = [&](int i, int8_t tb) {
     if (!is_active(tb, ga.max_active_bin)) return;
     const float h = d.h[i];
     const float h_inv = 1.0f / h;
@@ -448,14 +466,23 @@ __global__ void k_extra_ghost(DevParts d, GhostArgs ga, int lo, int hi) {
     d.hdt[i] = 0.f;
     d.nf[i] = 0;
     d.mintb[i] = (int8_t)(kNumTimeBins + 1);
-  });
+  };
+  if (list)
+    for_runs_list(list, cnt, d.tb, body);
+  else
+    for_runs(lo, hi, d.tb, body);
 }
 
 // runner_do_end_hydro_force (runner_others.c:815-870) -> hydro_end_force.
-__global__ void k_end_force(DevParts d, int lo, int hi, int max_active_bin) {
-  for_runs(lo, hi, d.tb, [&](int i, int8_t tb) {
+__global__ void k_end_force(DevParts d, int lo, int hi, int max_active_bin,
+                            const int *list, const int *cnt) {
+  auto body = [&](int i, int8_t tb) {
     if (is_active(tb, max_active_bin)) d.hdt[i] *= d.h[i] * kDimInv;
-  });
+  };
+  if (list)
+    for_runs_list(list, cnt, d.tb, body);
+  else
+    for_runs(lo, hi, d.tb, body);
 }
 
 __global__ void k_zero_int(int *p, int n) {
@@ -988,7 +1015,9 @@ hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
 }
 
 hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
-                                int hi, hipStream_t st, int max_active_bin) {
+                                int hi, hipStream_t st, int max_active_bin,
+                                const int *list, const int *cnt) {
+  if (max_active_bin < 0) list = cnt = nullptr;  // all records: staged path
   if (hi <= lo) return hipSuccess;
   const int m = max_active_bin;
   // active records: kRuns runs per block; all records: one run, staged
@@ -996,13 +1025,13 @@ hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
       b(kRecBlock);
   switch (kind) {
     case LoopKind::Density:
-      hipLaunchKernelGGL(k_build_dens_rec, g, b, 0, st, d, lo, hi, m);
+      hipLaunchKernelGGL(k_build_dens_rec, g, b, 0, st, d, lo, hi, m, list, cnt);
       break;
     case LoopKind::Gradient:
-      hipLaunchKernelGGL(k_build_grad_rec, g, b, 0, st, d, lo, hi, m);
+      hipLaunchKernelGGL(k_build_grad_rec, g, b, 0, st, d, lo, hi, m, list, cnt);
       break;
     case LoopKind::Force:
-      hipLaunchKernelGGL(k_build_force_rec, g, b, 0, st, d, lo, hi, m);
+      hipLaunchKernelGGL(k_build_force_rec, g, b, 0, st, d, lo, hi, m, list, cnt);
       break;
   }
   return hipGetLastError();
@@ -1017,28 +1046,32 @@ hipError_t launch_init_density(const DevParts &d, int lo, int hi,
 }
 
 hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,
-                        unsigned long long *counters, hipStream_t st) {
+                        unsigned long long *counters, hipStream_t st,
+                        const int *list, const int *cnt) {
   // counters[3] accumulates over the rounds of a pass (zeroed by the pass)
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, counters, 3);
   if (hi > lo)
     hipLaunchKernelGGL(k_ghost, dim3(run_blocks(hi - lo, kGhostBlock)),
-                       dim3(kGhostBlock), 0, st, d, ga, lo, hi, counters);
+                       dim3(kGhostBlock), 0, st, d, ga, lo, hi, counters,
+                       list, cnt);
   return hipGetLastError();
 }
 
 hipError_t launch_extra_ghost(const DevParts &d, const GhostArgs &ga, int lo,
-                              int hi, hipStream_t st) {
+                              int hi, hipStream_t st, const int *list,
+                              const int *cnt) {
   if (hi <= lo) return hipSuccess;
   hipLaunchKernelGGL(k_extra_ghost, dim3(run_blocks(hi - lo, 256)), dim3(256), 0, st,
-                     d, ga, lo, hi);
+                     d, ga, lo, hi, list, cnt);
   return hipGetLastError();
 }
 
 hipError_t launch_end_force(const DevParts &d, int lo, int hi,
-                            int max_active_bin, hipStream_t st) {
+                            int max_active_bin, hipStream_t st, const int *list,
+                            const int *cnt) {
   if (hi <= lo) return hipSuccess;
   hipLaunchKernelGGL(k_end_force, dim3(run_blocks(hi - lo, 256)), dim3(256), 0, st, d,
-                     lo, hi, max_active_bin);
+                     lo, hi, max_active_bin, list, cnt);
   return hipGetLastError();
 }
 
