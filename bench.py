@@ -352,7 +352,8 @@ def main() -> int:
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not "
-              "supported (see DESIGN.md for how the pass would shard).",
+              "supported: the pass shards by space over top-level cells with "
+              "one-cell halos, which is not built here.",
               file=sys.stderr)
         return 2
 
