@@ -107,30 +107,65 @@ __device__ __forceinline__ void store_wave_records(R *dst, int i0, int cnt,
   for (int k = lane; k < words; k += 64) out[k] = lds[k];
 }
 
+// LDS written by lanes of one wave and read by others of the same wave: no
+// block barrier (the waves of a block take different branches here).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class R, class Make>
 __device__ __forceinline__ void build_records(const DevParts &d, R *dst,
                                               int lo, int hi, int mab,
                                               const int *list, const int *cnt,
                                               Make make) {
+  using W = RecWord<R>;
+  constexpr int Q = (int)(sizeof(R) / sizeof(W));
+  __shared__ W buf[kRecBlock * sizeof(R) / sizeof(W)];
   if (mab >= 0) {  // uniform; launched with run_blocks(hi - lo, kRecBlock)
-    auto f = [&](int i, int8_t tb) {
-      if (is_active(tb, mab)) dst[i] = make(i);
-    };
-    if (list)
-      for_runs_list(list, cnt, d.tb, f);
-    else
-      for_runs(lo, hi, d.tb, f);
+    if (list) {
+      for_runs_list(list, cnt, d.tb, [&](int i, int8_t tb) {
+        if (is_active(tb, mab)) dst[i] = make(i);
+      });
+      return;
+    }
+    // kRuns runs of the block's width; a run whose 64 particles of a wave
+    // are all active is staged in the wave's LDS and stored as contiguous
+    // words, otherwise the active ones store their records one by one
+    const int lane = threadIdx.x & 63;
+    W *mine = buf + (threadIdx.x >> 6) * 64 * Q;
+    const int base = lo + blockIdx.x * kRecBlock * kRuns + threadIdx.x;
+    int8_t t[kRuns];
+#pragma unroll
+    for (int u = 0; u < kRuns; u++) {
+      const int i = base + u * kRecBlock;
+      t[u] = i < hi ? d.tb[i] : (int8_t)kBinInhibited;
+    }
+    for (int u = 0; u < kRuns; u++) {
+      const int i = base + u * kRecBlock;
+      const bool act = i < hi && is_active(t[u], mab);
+      if (__ballot(act) == ~0ull) {  // uniform over the wave
+        const R r = make(i);
+        const W *src = reinterpret_cast<const W *>(&r);
+#pragma unroll
+        for (int q = 0; q < Q; q++) mine[lane * Q + q] = src[q];
+        wave_lds_sync();
+        W *out = reinterpret_cast<W *>(dst + (i - lane));
+        for (int k = lane; k < 64 * Q; k += 64) out[k] = mine[k];
+        wave_lds_sync();  // the next run reuses the wave's LDS
+      } else if (act) {
+        dst[i] = make(i);
+      }
+    }
     return;
   }
   const int i = lo + blockIdx.x * kRecBlock + threadIdx.x;
-  using W = RecWord<R>;
-  __shared__ W buf[kRecBlock * sizeof(R) / sizeof(W)];
   R r{};
   if (i < hi) r = make(i);
   const int w = threadIdx.x >> 6;
   const int i0 = lo + blockIdx.x * kRecBlock + (w << 6);
-  store_wave_records(dst, i0, max(0, min(64, hi - i0)), r,
-                     buf + w * 64 * (int)(sizeof(R) / sizeof(W)));
+  store_wave_records(dst, i0, max(0, min(64, hi - i0)), r, buf + w * 64 * Q);
 }
 
 __global__ __launch_bounds__(kRecBlock) void k_build_dens_rec(
