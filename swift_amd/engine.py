@@ -283,8 +283,9 @@ class Engine:
 
     def set_tile_shape(self, loop: str, CI: int, CJ: int) -> None:
         """Tiles of CI i-particles against candidate chunks of CJ particles
-        in the work-queue schedules (CJ = CI, or CI / 2 for CI = 16, 32, or
-        the narrow tiles (4, 16), (4, 32), (8, 16) for sparse activity)."""
+        in the work-queue schedules (CJ = CI, or CI / 2 for CI = 16, 32, the
+        narrow tiles (4, 16), (4, 32), (8, 16) for sparse activity, or
+        (16, 32))."""
         check(lib().swsph_set_tile_shape(self._h, self.LOOPS[loop], int(CI),
                                          int(CJ)), "swsph_set_tile_shape")
 
