@@ -1275,9 +1275,15 @@ int copy_state(swsph_engine *e, std::vector<float *> &buf, bool save) {
     buf.assign(f.size(), nullptr);
     for (auto &p : buf) HIPCHK(dalloc(&p, std::max(n, e->cap_n)));
   }
-  for (size_t k = 0; k < f.size(); k++)
-    HIPCHK(hipMemcpyAsync(save ? buf[k] : f[k], save ? f[k] : buf[k],
-                          sizeof(float) * n, hipMemcpyDeviceToDevice, e->st));
+  // all fields in one launch
+  sphk::CopySet cs{};
+  cs.m = (int)f.size();
+  if (cs.m > sphk::kMaxCopies) return fail(2, "too many state fields");
+  for (int k = 0; k < cs.m; k++) {
+    cs.src[k] = save ? f[k] : buf[k];
+    cs.dst[k] = save ? buf[k] : f[k];
+  }
+  HIPCHK(sphk::launch_copy_set(cs, (int)n, e->st));
   if (!save) e->stale_mab = e->last_pass_mab;
   return 0;
 }

@@ -318,6 +318,15 @@ hipError_t launch_end_force(const DevParts &d, int lo, int hi,
                             const int *cnt = nullptr);
 hipError_t launch_count(const DevParts &d, int n, int max_active_bin,
                         unsigned long long *out3, hipStream_t st);
+// Copies n floats from each src[k] to dst[k], k < m <= kMaxCopies, in one
+// launch (the pass-state save / restore).
+constexpr int kMaxCopies = 16;
+struct CopySet {
+  const float *src[kMaxCopies];
+  float *dst[kMaxCopies];
+  int m;
+};
+hipError_t launch_copy_set(const CopySet &cs, int n, hipStream_t st);
 // *out = number of active particles of [0, n) (device-side).
 hipError_t launch_count_active(const DevParts &d, int n, int max_active_bin,
                                unsigned int *out, hipStream_t st);

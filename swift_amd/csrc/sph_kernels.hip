@@ -575,6 +575,20 @@ __global__ void k_count(DevParts d, int n, int max_active_bin,
   }
 }
 
+// Block (x, k) copies 1024 floats of array k as float4 words (the buffers
+// are 256-byte aligned; the last partial word element by element).
+__global__ __launch_bounds__(256) void k_copy_set(sphk::CopySet cs, int n) {
+  const int k = blockIdx.y;
+  const int w = blockIdx.x * 256 + threadIdx.x;  // float4 word
+  const int i = 4 * w;
+  if (i + 3 < n) {
+    reinterpret_cast<float4 *>(cs.dst[k])[w] =
+        reinterpret_cast<const float4 *>(cs.src[k])[w];
+  } else {
+    for (int j = i; j < n; j++) cs.dst[k][j] = cs.src[k][j];
+  }
+}
+
 // Active particles of [0, n) at max_active_bin: one atomic per block.
 __global__ void k_count_active(const int8_t *__restrict__ tb, int n,
                                int max_active_bin, unsigned int *out) {
@@ -1107,6 +1121,14 @@ hipError_t launch_end_force(const DevParts &d, int lo, int hi,
   if (hi <= lo) return hipSuccess;
   hipLaunchKernelGGL(k_end_force, dim3(run_blocks(hi - lo, 256)), dim3(256), 0, st, d,
                      lo, hi, max_active_bin, list, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_set(const CopySet &cs, int n, hipStream_t st) {
+  if (n <= 0 || cs.m <= 0) return hipSuccess;
+  if (cs.m > kMaxCopies) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_copy_set, dim3(nblk(n, 1024), cs.m), dim3(256), 0, st,
+                     cs, n);
   return hipGetLastError();
 }
 
