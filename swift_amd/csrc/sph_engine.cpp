@@ -1919,9 +1919,14 @@ int swsph_save_state(swsph_engine *e) {
     e->saved.assign(src.size(), nullptr);
     for (auto &p : e->saved) HIPCHK(dalloc(&p, std::max(n, e->cap_n)));
   }
-  for (size_t k = 0; k < src.size(); k++)
-    HIPCHK(hipMemcpyAsync(e->saved[k], src[k], sizeof(float) * n,
-                          hipMemcpyDeviceToDevice, e->st));
+  sphk::CopySet cs{};
+  cs.m = (int)src.size();
+  if (cs.m > sphk::kMaxCopies) return fail(2, "too many state fields");
+  for (int k = 0; k < cs.m; k++) {
+    cs.src[k] = src[k];
+    cs.dst[k] = e->saved[k];
+  }
+  HIPCHK(sphk::launch_copy_set(cs, (int)n, e->st));
   e->saved_valid = true;
   e->saved_epoch = e->fields_epoch;
   return 0;
@@ -1934,9 +1939,14 @@ int swsph_restore_state(swsph_engine *e) {
     return fail(1, "no saved state for the current particles");
   std::vector<float *> dst = io_fields(e->d);
   const size_t n = (size_t)(e->n ? e->n : 1);
-  for (size_t k = 0; k < dst.size(); k++)
-    HIPCHK(hipMemcpyAsync(dst[k], e->saved[k], sizeof(float) * n,
-                          hipMemcpyDeviceToDevice, e->st));
+  sphk::CopySet cs{};
+  cs.m = (int)dst.size();
+  if (cs.m > sphk::kMaxCopies) return fail(2, "too many state fields");
+  for (int k = 0; k < cs.m; k++) {
+    cs.src[k] = e->saved[k];
+    cs.dst[k] = dst[k];
+  }
+  HIPCHK(sphk::launch_copy_set(cs, (int)n, e->st));
   if (e->saved_epoch != e->fields_epoch) {
     invalidate_records(e);  // fields changed outside passes since the save
     e->fields_epoch++;
