@@ -142,6 +142,7 @@ struct swsph_engine {
   bool capturing = false;
   hipGraphExec_t g_density = nullptr, g_finish = nullptr;
   std::vector<float *> tune_saved;  // pass state kept by the autotuner
+  size_t saved_cap = 0, tune_cap = 0;  // particles each copy set holds
   // Device binning (sph_step.hip): key / value buffers of the radix sort,
   // its temporary storage, and a staging buffer of 3 doubles per particle
   // for uploads in the caller's order and for permutations.
@@ -187,10 +188,16 @@ namespace {
 std::vector<float *> io_fields(DevParts &d) {
   return {d.h, d.rho, d.P, d.c, d.f, d.bal, d.alpha, d.adiff, d.divv_prev};
 }
+// Frees the swsph_save_state copies and the autotuner's copies: both are
+// sized for the particle capacity and go whenever the capacity changes.
 void free_saved(swsph_engine *e) {
   for (float *p : e->saved)
     if (p) (void)hipFree(p);
   e->saved.clear();
+  for (float *p : e->tune_saved)
+    if (p) (void)hipFree(p);
+  e->tune_saved.clear();
+  e->saved_cap = e->tune_cap = 0;
 }
 }  // namespace
 
@@ -1267,13 +1274,31 @@ void drop_graphs(swsph_engine *e) {
   e->g_density = e->g_finish = nullptr;
 }
 
-// Device copies of the pass state (h and the (io) fields).
+// A set of nf device buffers holding at least n particles each; cap is the
+// particle count the set was allocated for (reallocated when it is short).
+int ensure_copies(swsph_engine *e, std::vector<float *> &buf, size_t &cap,
+                  size_t nf, size_t n) {
+  if (!buf.empty() && cap >= n && buf.size() == nf) return 0;
+  for (float *p : buf)
+    if (p) (void)hipFree(p);
+  buf.assign(nf, nullptr);
+  cap = 0;
+  const size_t want = std::max(n, e->cap_n);
+  for (auto &p : buf) HIPCHK(dalloc(&p, want));
+  cap = want;
+  return 0;
+}
+
+// Device copies of the pass state (h and the (io) fields) into / out of
+// the autotuner's buffers.
 int copy_state(swsph_engine *e, std::vector<float *> &buf, bool save) {
   std::vector<float *> f = io_fields(e->d);
   const size_t n = (size_t)(e->n ? e->n : 1);
-  if (buf.empty()) {
-    buf.assign(f.size(), nullptr);
-    for (auto &p : buf) HIPCHK(dalloc(&p, std::max(n, e->cap_n)));
+  if (save) {
+    int rc = ensure_copies(e, buf, e->tune_cap, f.size(), n);
+    if (rc) return rc;
+  } else if (buf.size() != f.size() || e->tune_cap < n) {
+    return fail(2, "autotuner state copy does not hold the current particles");
   }
   // all fields in one launch
   sphk::CopySet cs{};
@@ -1915,9 +1940,9 @@ int swsph_save_state(swsph_engine *e) {
   std::lock_guard<std::mutex> lk(e->mtx);
   std::vector<float *> src = io_fields(e->d);
   const size_t n = (size_t)(e->n ? e->n : 1);
-  if (e->saved.empty()) {
-    e->saved.assign(src.size(), nullptr);
-    for (auto &p : e->saved) HIPCHK(dalloc(&p, std::max(n, e->cap_n)));
+  {
+    int rc = ensure_copies(e, e->saved, e->saved_cap, src.size(), n);
+    if (rc) return rc;
   }
   sphk::CopySet cs{};
   cs.m = (int)src.size();
@@ -1939,6 +1964,8 @@ int swsph_restore_state(swsph_engine *e) {
     return fail(1, "no saved state for the current particles");
   std::vector<float *> dst = io_fields(e->d);
   const size_t n = (size_t)(e->n ? e->n : 1);
+  if (e->saved_cap < n)
+    return fail(1, "no saved state for the current particles");
   sphk::CopySet cs{};
   cs.m = (int)dst.size();
   if (cs.m > sphk::kMaxCopies) return fail(2, "too many state fields");

@@ -85,41 +85,24 @@ __device__ __forceinline__ float wave_max(float v) {
 
 constexpr unsigned int kRowMask = (1u << 27) - 1u;
 
-// 64-bit hit masks per lane between two interaction phases (1 or 2): more
-// rows per phase even out the lanes' hit counts, at two registers a mask.
-#ifndef SWSPH_MASKS
-#define SWSPH_MASKS 2
-#endif
-constexpr int kMasks = SWSPH_MASKS;
+// Two 64-bit hit masks per lane between two interaction phases: 128 rows
+// per phase even out the lanes' hit counts better than 64 (-2 % per pass).
+constexpr int kMasks = 2;
 
-// Candidate pre-test of the density and gradient loops in packed FP16 (two
-// rows per instruction) when a chunk has an even number of rows; 0 keeps the
-// FP32 pre-test everywhere.
-#ifndef SWSPH_F16_PRETEST
-#define SWSPH_F16_PRETEST 1
-#endif
+// The candidate pre-test of the density and gradient loops runs in packed
+// FP16 (two rows per instruction) when a chunk has an even number of rows.
 typedef _Float16 half2v __attribute__((ext_vector_type(2)));
 
-// Waves per block of the batched kernels.  Each wave works on its own chunk
-// or tile with its own LDS.  Several waves per block lift the per-CU
-// workgroup limit off the occupancy, but measured slower on the 128^3 box
-// than one wave per block.
-#ifndef SWSPH_WAVES_PER_BLOCK
-#define SWSPH_WAVES_PER_BLOCK 1
-#endif
-constexpr int kWaves = SWSPH_WAVES_PER_BLOCK;
+// One wave per block in the batched kernels: each wave works on its own
+// chunk or tile with its own LDS (several waves per block measured slower on
+// the 128^3 box).
+constexpr int kWaves = 1;
 
-// Launch bounds of the batched neighbour kernels: block size and, unless
-// SWSPH_NO_OCC_TARGET is defined, the loop's occupancy target (its own one
-// for the one-tile-per-wave kernel).
-#ifdef SWSPH_NO_OCC_TARGET
-#define SWSPH_LOOP_BOUNDS(L) __launch_bounds__(64 * kWaves)
-#define SWSPH_TILE_BOUNDS(L) __launch_bounds__(64 * kWaves)
-#else
-#define SWSPH_LOOP_BOUNDS(L) __launch_bounds__(64 * kWaves, L::kWavesPerSimd)
-#define SWSPH_TILE_BOUNDS(L) \
-  __launch_bounds__(64 * kWaves, L::kTileWavesPerSimd)
-#endif
+// Launch bounds of the batched neighbour kernels: one wave per block and
+// the loop's occupancy target (its own one for the one-tile-per-wave
+// kernel).
+#define SWSPH_LOOP_BOUNDS(L) __launch_bounds__(64, L::kWavesPerSimd)
+#define SWSPH_TILE_BOUNDS(L) __launch_bounds__(64, L::kTileWavesPerSimd)
 
 // A value every lane of the wave holds (e.g. read from LDS at a common
 // address), moved to a scalar register so that branches on it stay uniform.
@@ -137,13 +120,7 @@ __device__ __forceinline__ double uniform(double v) {
 // through LDS (a wave's LDS operations execute in issue order; this stops
 // the compiler from moving them across).
 __device__ __forceinline__ void wave_sync() {
-#if SWSPH_WAVES_PER_BLOCK == 1
   __syncthreads();  // the block is the wave
-#else
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#endif
 }
 
 // Block b of a batched grid over n work items, wave w of it: each XCD (the
@@ -200,9 +177,6 @@ struct GatherLds {
 // hit's record is loaded before the current hit is evaluated, so its latency
 // overlaps the interaction body); measured slower for the gradient loop's
 // larger records and register budget, so the others walk one hit at a time.
-#ifndef SWSPH_WALK_PREFETCH
-#define SWSPH_WALK_PREFETCH 1
-#endif
 template <class L, int CI, int CJ, bool BUILD>
 __device__ __forceinline__ void interact_rows(
     const typename L::Rec *__restrict__ rec, const GatherLds<CI, CJ> &sh,
@@ -212,21 +186,15 @@ __device__ __forceinline__ void interact_rows(
   constexpr int S = GatherLds<CI, CJ>::S, RPC = GatherLds<CI, CJ>::RPC;
   // next row of this lane (ascending), taken out of the masks
   auto pop = [&]() -> int {
-    int row;
-    if (kMasks == 1) {
-      row = m0 ? __builtin_ctzll(m0) : 0;
-      m0 &= m0 - 1ull;
-    } else {
-      const bool lo = m0 != 0ull;
-      const unsigned long long mm = lo ? m0 : m1;
-      row = (mm ? __builtin_ctzll(mm) : 0) + (lo ? 0 : 64);
-      const unsigned long long rest = mm & (mm - 1ull);
-      m0 = lo ? rest : m0;
-      m1 = lo ? m1 : rest;
-    }
+    const bool lo = m0 != 0ull;
+    const unsigned long long mm = lo ? m0 : m1;
+    const int row = (mm ? __builtin_ctzll(mm) : 0) + (lo ? 0 : 64);
+    const unsigned long long rest = mm & (mm - 1ull);
+    m0 = lo ? rest : m0;
+    m1 = lo ? m1 : rest;
     return row;
   };
-  auto any = [&]() { return kMasks == 1 ? m0 != 0ull : (m0 | m1) != 0ull; };
+  auto any = [&]() { return (m0 | m1) != 0ull; };
   // record index and image of a row
   auto locate = [&](int row, int &q) -> int {
     const unsigned int info = sh.cinfo[row / RPC];
@@ -253,7 +221,7 @@ __device__ __forceinline__ void interact_rows(
       L::interact(r2, dx, dy, dz, st, r, acc);
     }
   };
-  if (!SWSPH_WALK_PREFETCH || !L::kPrefetchWalk) {
+  if (!L::kPrefetchWalk) {
     while (any()) {
       int q;
       const int j = locate(pop(), q);
@@ -290,8 +258,7 @@ __device__ __forceinline__ void interact_rows(
 // neighbour cells.  `stencil` selects the 27 neighbours of the i-chunk's
 // cell; otherwise the single cell `cj0` with translation (tx0, ty0, tz0).
 // i-chunks come from the chunk table itself (CI == CJ) unless TILE.
-template <class L, int CI, int CJ, bool SUBSET, bool BUILD, bool TILE = false,
-          int W = kWaves>
+template <class L, int CI, int CJ, bool SUBSET, bool BUILD, bool TILE = false>
 __device__ __forceinline__ void run_chunk(
     const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
     const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
@@ -306,8 +273,7 @@ __device__ __forceinline__ void run_chunk(
   static_assert(!BUILD || (CI == kListChunk && CJ == kListChunk &&
                            !L::kSymmetric),
                 "lists are built by the density loop in list tiles");
-  __shared__ Sh sh_all[W];
-  Sh &sh = sh_all[W == 1 ? 0 : threadIdx.x >> 6];
+  __shared__ Sh sh;
   const int lane = threadIdx.x & 63;
   const int g = lane % C;
   const int s = lane / C;
@@ -434,8 +400,7 @@ __device__ __forceinline__ void run_chunk(
   // rejects a neighbour of the pair contract.
   // (the force loop's symmetric test needs a widened radius per candidate
   // as well, which costs more than the packed rows save: it stays FP32)
-  constexpr bool kF16 =
-      SWSPH_F16_PRETEST && (RPC % 2 == 0) && !L::kSymmetric;
+  constexpr bool kF16 = (RPC % 2 == 0) && !L::kSymmetric;
   constexpr float kU16 = 1.f / 2048.f;
   float invR = 0.f, eps16 = 0.f;
   _Float16 ixh = 0, iyh = 0, izh = 0, rt2h = -1;
@@ -569,7 +534,7 @@ __device__ __forceinline__ void run_chunk(
             nib |= (pass && jl != selfl) ? ((Nib)1 << rr) : (Nib)0;
           }
         }
-        if (kMasks == 1 || nrow < 64)  // uniform
+        if (nrow < 64)  // uniform
           m0 |= (unsigned long long)nib << nrow;
         else
           m1 |= (unsigned long long)nib << (nrow - 64);
@@ -623,7 +588,7 @@ __global__ __launch_bounds__(64) void k_loop_bundle(
     BundleArgs ba) {
   const BundleItem it = ba.items[blockIdx.x];
   const NbrList none{};
-  run_chunk<L, C, C, false, false, false, 1>(
+  run_chunk<L, C, C, false, false, false>(
       d, la, ca, rec, pos, uniform(it.ich), false, 0, 0.0, 0.0, 0.0, none,
       nullptr, 0, ba.nbrs + uniform(it.off), uniform(it.n));
 }
@@ -643,7 +608,7 @@ __global__ __launch_bounds__(64) void k_loop_task(
   }
   const int ich = ca.cell_chunk[ta.icell[q]] + b;
   const NbrList none{};
-  run_chunk<L, C, C, SUBSET, false, false, 1>(d, la, ca, rec, pos, ich, false,
+  run_chunk<L, C, C, SUBSET, false, false>(d, la, ca, rec, pos, ich, false,
                                               ta.jcell[q], ta.t[3 * q],
                                               ta.t[3 * q + 1], ta.t[3 * q + 2],
                                               none);

@@ -11,48 +11,27 @@
 // position (and a negative radius) so that every test rejects them.
 // Record sizes: the hit walks gather records of scattered j from L2, and
 // fewer bytes per record measured faster than cache-line-aligned strides
-// (64 B density / 128 B force records: +6 % / +5 % loop time).  The
-// default (SWSPH_COMPACT_RECORDS = 1) packs 8-byte aligned records with no
-// padding, density 40 B and gradient 56 B, and a force record without h^-4
-// (two multiplies per pair instead), 96 B: -2.7 % per pass against 48 / 64 /
-// 112 B records (0 selects those).
-#ifndef SWSPH_COMPACT_RECORDS
-#define SWSPH_COMPACT_RECORDS 1
-#endif
-#if SWSPH_COMPACT_RECORDS
-#define SWSPH_REC_ALIGN 8
-#else
-#define SWSPH_REC_ALIGN 16
-#endif
-struct __attribute__((aligned(SWSPH_REC_ALIGN))) DensRec {
+// (64 B density / 128 B force records: +6 % / +5 % loop time).  Records are
+// 8-byte aligned with no padding: density 40 B, gradient 56 B, and a force
+// record without h^-4 (two multiplies per pair instead), 96 B (-2.7 % per
+// pass against 48 / 64 / 112 B records with h^-4 and padding).
+struct __attribute__((aligned(8))) DensRec {
   double x, y, z;
   float vx, vy, vz, m;
-#if !SWSPH_COMPACT_RECORDS
-  float pad[2];
-#endif
 };
-struct __attribute__((aligned(SWSPH_REC_ALIGN))) GradRec {
+struct __attribute__((aligned(8))) GradRec {
   double x, y, z;
   float vx, vy, vz, mrho;  // m_j / rho_j
   float u, c, alpha, pad0;
-#if !SWSPH_COMPACT_RECORDS
-  float pad[2];
-#endif
 };
 struct __attribute__((aligned(16))) ForceRec {
   double x, y, z;
   float hg2;               // (gamma_K h_j)^2
   float vx, vy, vz;
   float m, inv_m, h_inv;
-#if !SWSPH_COMPACT_RECORDS
-  float h_inv4;
-#endif
   float rho, inv_rho, pr2, f;  // pr2 = P / rho^2
   float bal, c, alpha, P;
   float Pad, u, tb;            // Pad = P * alpha_diff
-#if !SWSPH_COMPACT_RECORDS
-  float pad0, pad[2];
-#endif
 };
 
 // Candidate record of the FP32 pre-test: the position relative to the

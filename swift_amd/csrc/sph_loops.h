@@ -6,22 +6,14 @@
 #include "sph_device.h"
 #include "sph_kernels.h"
 
-// Occupancy targets (waves per SIMD) of the batched neighbour kernels.
-#ifndef SWSPH_OCC_DENSITY
-#define SWSPH_OCC_DENSITY 7
-#endif
-#ifndef SWSPH_OCC_GRADIENT
-#define SWSPH_OCC_GRADIENT 7
-#endif
-#ifndef SWSPH_OCC_FORCE
-#define SWSPH_OCC_FORCE 5
-#endif
-// The one-tile-per-wave kernel needs fewer registers than the grid and
+// Occupancy targets (waves per SIMD) of the batched neighbour kernels.  The
+// one-tile-per-wave kernel needs fewer registers than the grid and
 // persistent forms: its force loop fits 6 waves per SIMD without spilling
 // (the others spill 16-31 VGPRs there).
-#ifndef SWSPH_OCC_FORCE_TILES
-#define SWSPH_OCC_FORCE_TILES 6
-#endif
+constexpr int kOccDensity = 7;
+constexpr int kOccGradient = 7;
+constexpr int kOccForce = 5;
+constexpr int kOccForceTiles = 6;
 
 namespace sphl {
 using namespace sph;
@@ -46,9 +38,6 @@ __device__ __forceinline__ DensRec dens_rec(const DevParts &d, int i) {
   r.vy = d.vy[i];
   r.vz = d.vz[i];
   r.m = d.m[i];
-#if !SWSPH_COMPACT_RECORDS
-  r.pad[0] = r.pad[1] = 0.f;
-#endif
   return r;
 }
 
@@ -67,9 +56,6 @@ __device__ __forceinline__ GradRec grad_rec(const DevParts &d, int i) {
   r.c = d.c[i];
   r.alpha = d.alpha[i];
   r.pad0 = 0.f;
-#if !SWSPH_COMPACT_RECORDS
-  r.pad[0] = r.pad[1] = 0.f;
-#endif
   return r;
 }
 
@@ -88,10 +74,6 @@ __device__ __forceinline__ ForceRec force_rec(const DevParts &d, int i) {
   r.m = d.m[i];
   r.inv_m = 1.f / r.m;
   r.h_inv = 1.f / h;
-  const float hi2 = r.h_inv * r.h_inv;
-#if !SWSPH_COMPACT_RECORDS
-  r.h_inv4 = hi2 * hi2;
-#endif
   r.rho = d.rho[i];
   r.inv_rho = 1.f / r.rho;
   r.P = d.P[i];
@@ -103,10 +85,6 @@ __device__ __forceinline__ ForceRec force_rec(const DevParts &d, int i) {
   r.Pad = r.P * d.adiff[i];
   r.u = d.u[i];
   r.tb = (float)tb;
-#if !SWSPH_COMPACT_RECORDS
-  r.pad0 = 0.f;
-  r.pad[0] = r.pad[1] = 0.f;
-#endif
   return r;
 }
 
@@ -139,8 +117,8 @@ struct DensI {
 
 struct DensityLoop {
   // occupancy target of the batched kernels (waves per SIMD)
-  static constexpr int kWavesPerSimd = SWSPH_OCC_DENSITY;
-  static constexpr int kTileWavesPerSimd = SWSPH_OCC_DENSITY;
+  static constexpr int kWavesPerSimd = kOccDensity;
+  static constexpr int kTileWavesPerSimd = kOccDensity;
   using Rec = DensRec;
   using Acc = DensAcc;
   using I = DensI;
@@ -241,8 +219,8 @@ struct GradI {
 
 struct GradientLoop {
   // occupancy target of the batched kernels (waves per SIMD)
-  static constexpr int kWavesPerSimd = SWSPH_OCC_GRADIENT;
-  static constexpr int kTileWavesPerSimd = SWSPH_OCC_GRADIENT;
+  static constexpr int kWavesPerSimd = kOccGradient;
+  static constexpr int kTileWavesPerSimd = kOccGradient;
   using Rec = GradRec;
   using Acc = GradAcc;
   using I = GradI;
@@ -324,8 +302,8 @@ struct ForceI {
 
 struct ForceLoop {
   // occupancy target of the batched kernels (waves per SIMD)
-  static constexpr int kWavesPerSimd = SWSPH_OCC_FORCE;
-  static constexpr int kTileWavesPerSimd = SWSPH_OCC_FORCE_TILES;
+  static constexpr int kWavesPerSimd = kOccForce;
+  static constexpr int kTileWavesPerSimd = kOccForceTiles;
   using Rec = ForceRec;
   using Acc = ForceAcc;
   using I = ForceI;
@@ -406,12 +384,8 @@ struct ForceLoop {
     kernel_deval(r * s.h_inv, wi, wi_dx);
     kernel_deval(r * j.h_inv, wj, wj_dx);
     const float wi_dr = s.h_inv4 * wi_dx;
-#if SWSPH_COMPACT_RECORDS
     const float hj2 = j.h_inv * j.h_inv;
     const float wj_dr = (hj2 * hj2) * wj_dx;
-#else
-    const float wj_dr = j.h_inv4 * wj_dx;
-#endif
     const float dvdr =
         fmaf(s.vz - j.vz, dz, fmaf(s.vy - j.vy, dy, (s.vx - j.vx) * dx));
     const float mu = r_inv * fminf(dvdr, 0.f);

@@ -204,6 +204,17 @@ void worker(swsph_server *s) {
       // fit the budget (or nothing else runs, so that any request can run).
       s->cv_work.wait(lk, [&] {
         if (s->stop) return true;
+        // a step of a box that no longer exists fails at once instead of
+        // blocking the queue behind it
+        while (!s->queue.empty() && s->queue.front()->box &&
+               s->boxes.find(s->queue.front()->box) == s->boxes.end()) {
+          auto dead = s->queue.front();
+          s->queue.pop_front();
+          dead->err = "resident box was closed";
+          dead->info.t_done_ms = now_ms();
+          dead->done = true;
+          s->cv_done.notify_all();
+        }
         if (s->queue.empty() || s->inflight >= s->opts.max_inflight)
           return false;
         const auto &h = s->queue.front();
@@ -411,6 +422,8 @@ int swsph_server_close_box(swsph_server *s, int64_t box) {
     auto it = s->boxes.find(box);
     if (it == s->boxes.end()) return sfail(1, "no resident box with that id");
     if (it->second->busy) return sfail(1, "box is being stepped");
+    for (const auto &q : s->queue)
+      if (q->box == box) return sfail(1, "box has queued steps");
     b = it->second;
     s->boxes.erase(it);
     s->bytes_in_use -= b->bytes;
