@@ -5,6 +5,7 @@ bench.py's cpu_baseline leg, as the checker.  The product never imports it.
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 import subprocess
@@ -33,25 +34,44 @@ def port_lib():
     return _port
 
 
+def _load(path):
+    L = C.CDLL(path)
+    L.orc_kernel_gamma.restype = C.c_float
+    L.orc_kernel_gamma2.restype = C.c_float
+    L.orc_kernel_root.restype = C.c_float
+    L.orc_kernel_norm.restype = C.c_float
+    L.orc_cbrt32.restype = C.c_float
+    L.orc_cbrt32.argtypes = [C.c_float]
+    L.orc_kernel_deval32.argtypes = [C.c_float, C.POINTER(C.c_float),
+                                     C.POINTER(C.c_float)]
+    L.orc_kernel_deval64.argtypes = [C.c_double, C.POINTER(C.c_double),
+                                     C.POINTER(C.c_double)]
+    for fn in ("orc_iact_density32", "orc_iact_gradient32",
+               "orc_iact_force32"):
+        getattr(L, fn).argtypes = [C.c_float, C.c_void_p, C.c_float,
+                                   C.c_float, C.c_void_p, C.c_void_p, C.c_int]
+    return L
+
+
+@contextlib.contextmanager
+def using(path: str):
+    """Temporarily route every call of this module to another build of the
+    oracle (the planted-mistake builds of the pinning tests)."""
+    global _lib
+    saved = _lib
+    _lib = _load(path)
+    try:
+        yield
+    finally:
+        _lib = saved
+
+
 def lib():
     global _lib
     if _lib is None:
         if not os.path.exists(_LIB):
             build()
-        _lib = C.CDLL(_LIB)
-        _lib.orc_kernel_gamma.restype = C.c_float
-        _lib.orc_kernel_gamma2.restype = C.c_float
-        _lib.orc_kernel_root.restype = C.c_float
-        _lib.orc_kernel_norm.restype = C.c_float
-        _lib.orc_kernel_deval32.argtypes = [C.c_float, C.POINTER(C.c_float),
-                                            C.POINTER(C.c_float)]
-        _lib.orc_kernel_deval64.argtypes = [C.c_double, C.POINTER(C.c_double),
-                                            C.POINTER(C.c_double)]
-        for fn in ("orc_iact_density32", "orc_iact_gradient32",
-                   "orc_iact_force32"):
-            getattr(_lib, fn).argtypes = [C.c_float, C.c_void_p, C.c_float,
-                                          C.c_float, C.c_void_p, C.c_void_p,
-                                          C.c_int]
+        _lib = _load(_LIB)
     return _lib
 
 
@@ -180,6 +200,40 @@ def count_bruteforce(props, st: dict, mode: int) -> np.ndarray:
     lib().orc_count_bruteforce(C.byref(cfg), C.byref(p), mode,
                                counts.ctypes.data_as(C.c_void_p))
     return counts
+
+
+def contract_flips(props, st: dict) -> dict:
+    """Density-mask decisions on which the reference's pair contract and a
+    single rounding of the FP64 difference disagree (27-cell candidates)."""
+    out = np.zeros(4, np.int64)
+    cfg = make_config(props)
+    p, _keep = _parts(st)
+    lib().orc_contract_flips(C.byref(cfg), C.byref(p),
+                             out.ctypes.data_as(C.c_void_p))
+    return {"ref_only": int(out[0]), "fp64_only": int(out[1]),
+            "candidates": int(out[2]), "both": int(out[3])}
+
+
+class OrcFrame(C.Structure):
+    _fields_ = [("self", C.c_int), ("oi", C.c_double * 3),
+                ("oj", C.c_double * 3)]
+
+
+def pair_frame(loc_i, loc_j, dim) -> OrcFrame:
+    f = OrcFrame()
+    a = (C.c_double * 3)(*loc_i)
+    b = (C.c_double * 3)(*loc_j)
+    d = (C.c_double * 3)(*dim)
+    lib().orc_pair_frame(a, b, d, C.byref(f))
+    return f
+
+
+def pair_dx32(xi, xj, frame: OrcFrame):
+    dx = (C.c_float * 3)()
+    r2 = C.c_float()
+    lib().orc_pair_dx32((C.c_double * 3)(*xi), (C.c_double * 3)(*xj),
+                        C.byref(frame), dx, C.byref(r2))
+    return np.array(dx[:], np.float32), np.float32(r2.value)
 
 
 def kernel_deval32(u: float):

@@ -19,6 +19,15 @@
  */
 #include "sphenix_oracle.h"
 
+/* Planted mistakes, for the tests that prove each pinning check can fail:
+ * tests/test_oracle_pins.py builds this file a second time per mistake with
+ * -DORC_MUTATION=k (into oracle/_mut/); the checker itself is always built
+ * with ORC_MUTATION 0, where every MUT(k, good, bad) is `good`. */
+#ifndef ORC_MUTATION
+#define ORC_MUTATION 0
+#endif
+#define MUT(k, good, bad) (ORC_MUTATION == (k) ? (bad) : (good))
+
 #include <float.h>
 #include <math.h>
 #include <stdio.h>
@@ -91,17 +100,86 @@ void orc_kernel_deval64(double u, double *W, double *dW_dx) {
   *dW_dx = dw * (double)KCONST * (double)KG_INV_DIM1;
 }
 
-/* The FP32 pair contract (see header). */
-void orc_pair_dx32(const double xi[3], const double xj[3], const double t[3],
+/* Frame of the pair of top-level cells with corners loc_i (cell of i) and
+ * loc_j (cell of j): space_getsid_and_swap_cells (space_getsid.h:47-80)
+ * finds the shift that brings cell j next to cell i (loc differences beyond
+ * half the box wrap) and the sort id of the offset; runner_flip swaps the
+ * cells so that the second cell of the pair ("cj", the upper cell) lies in
+ * the positive half-space of the offset (sort_part.h:59-61).  Both
+ * directions of the pair then use cj's corner: particles of cj relative to
+ * cj->loc, those of ci relative to cj->loc + shift
+ * (runner_doiact_functions_hydro.h:1327-1341, 1446-1465, 1672-1674;
+ * tools.c:218-246).  The same cell (sort id 13) is a self pair. */
+void orc_pair_frame(const double loc_i[3], const double loc_j[3],
+                    const double dim[3], orc_frame *f) {
+  double shift[3], d[3];
+  for (int k = 0; k < 3; k++) {
+    d[k] = loc_j[k] - loc_i[k];
+    if (d[k] < -dim[k] / 2)
+      shift[k] = dim[k];
+    else if (d[k] > dim[k] / 2)
+      shift[k] = -dim[k];
+    else
+      shift[k] = 0.0;
+    d[k] += shift[k];
+  }
+  int sid = 0;
+  for (int k = 0; k < 3; k++)
+    sid = 3 * sid + ((d[k] < 0.0) ? 0 : ((d[k] > 0.0) ? 2 : 1));
+  f->self = sid == 13;
+  if (f->self) {
+    for (int k = 0; k < 3; k++) f->oi[k] = f->oj[k] = 0.0;
+  } else if (sid > 13) {
+    /* no flip: ci = cell of i, cj = cell of j */
+    for (int k = 0; k < 3; k++) {
+      f->oi[k] = loc_j[k] + shift[k];
+      f->oj[k] = loc_j[k];
+    }
+  } else {
+    /* flip: ci = cell of j, cj = cell of i, shift negated */
+    for (int k = 0; k < 3; k++) {
+      f->oi[k] = loc_i[k];
+      f->oj[k] = loc_i[k] + (-shift[k]);
+    }
+  }
+}
+
+/* The FP32 pair contract: the expression of the reference's loops and of
+ * its brute-force checker for the pair's frame.  Self pairs round the FP64
+ * difference once (DOSELF1 :2459, DOSELF2 :2784, self_all_density
+ * tools.c:617-619); cell pairs round both cell-frame offsets, then subtract
+ * in FP32 (DOPAIR1 :1327-1346, DOPAIR2 :1748-1781, pairs_all_density
+ * tools.c:232-246).  r2 sums the squares as gcc contracts
+ * dx0*dx0 + dx1*dx1 + dx2*dx2 under -ffp-contract=fast. */
+void orc_pair_dx32(const double xi[3], const double xj[3], const orc_frame *f,
                    float dx[3], float *r2) {
   for (int k = 0; k < 3; k++) {
-    const double xs = xi[k] - t[k];
-    dx[k] = (float)(xs - xj[k]);
+    if (f->self) {
+      dx[k] = (float)(xi[k] - xj[k]);
+    } else {
+      const float pix = (float)(xi[k] - f->oi[k]);
+      const float pjx = (float)(xj[k] - f->oj[k]);
+      dx[k] = pix - pjx;
+    }
   }
   *r2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
 }
 
+/* The contract of the first build of this oracle: one rounding of the FP64
+ * difference for every pair, dx = (float)((x_i - t) - x_j).  Kept only to
+ * count the mask decisions on which the two contracts differ. */
+static void pair_dx32_fp64(const double xi[3], const double xj[3],
+                           const double t[3], float dx[3], float *r2) {
+  for (int k = 0; k < 3; k++) dx[k] = (float)((xi[k] - t[k]) - xj[k]);
+  *r2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
+}
+
 static inline float hig2_of(float h) { return (h * h) * KG2; }
+
+/* pow_inv_dimension for 3D (dimension.h:98-102): the C library's cbrtf. */
+float orc_cbrt32(float x) {
+  return MUT(17, cbrtf(x), expf(logf(x) / 3.f));
+}
 
 /* ------------------------------------------------------------------------ */
 /* Single-pair FP32 interactions on a flat particle record                  */
@@ -174,7 +252,8 @@ void orc_iact_gradient32(float r2, const float dx[3], float hi, float hj,
   const float dvdr_Hubble = dvdr + a2_Hubble * r2;
   const float omega_ij = fminf(dvdr_Hubble, 0.f);
   const float mu_ij = fac_mu * r_inv * omega_ij;
-  const float new_v_sig = pi[P_C] + pj[P_C] - VISC_BETA * mu_ij;
+  const float new_v_sig =
+      pi[P_C] + pj[P_C] - MUT(13, VISC_BETA, 2.f) * mu_ij;
   pi[P_VSIG] = fmaxf(pi[P_VSIG], new_v_sig);
   float wi, wi_dx;
   const float ui = r / hi;
@@ -230,8 +309,8 @@ void orc_iact_force32(float r2, const float dx[3], float hi, float hj,
   const float f_ji = 1.f - pj[P_F] / mi;
   const float rho_ij = rhoi + rhoj;
   const float alpha = pi[P_ALPHA] + pj[P_ALPHA];
-  const float visc =
-      -0.25f * alpha * v_sig * mu_ij * (pi[P_BAL] + pj[P_BAL]) / rho_ij;
+  const float visc = MUT(12, -0.25f, -0.5f) * alpha * v_sig * mu_ij *
+                     (pi[P_BAL] + pj[P_BAL]) / rho_ij;
   const float visc_acc_term =
       0.5f * visc * (wi_dr * f_ij + wj_dr * f_ji) * r_inv;
   const float P_over_rho2_i = pressurei / (rhoi * rhoi) * f_ij;
@@ -248,7 +327,7 @@ void orc_iact_force32(float r2, const float dx[3], float hi, float hj,
       (pressurei * pi[P_ADIFF] + pressurej * pj[P_ADIFF]) /
       (pressurei + pressurej);
   const float v_diff =
-      alpha_diff * 0.5f *
+      alpha_diff * MUT(14, 0.5f, 1.f) *
       (sqrtf(2.f * fabsf(pressurei - pressurej) / rho_ij) +
        fabsf(fac_mu * r_inv * dvdr_Hubble));
   const float diff_du_term = v_diff * (pi[P_U] - pj[P_U]) *
@@ -290,8 +369,13 @@ typedef struct grid {
 #endif
 } grid;
 
+/* space_parts_get_cell_index (space_cell_index.c:114-128): the cell of a
+ * position in [0, dim) is (int)(x * iwidth), iwidth = 1 / (dim / cdim)
+ * (space_regrid.c:221-222); the clamp only guards positions the caller
+ * failed to wrap. */
 static int cell_coord(double x, double inv_w, int n) {
-  int c = (int)floor(x * inv_w);
+  int c = (int)(x * inv_w);
+  if (x < 0.0) c = 0;
   if (c < 0) c = 0;
   if (c >= n) c = n - 1;
   return c;
@@ -319,7 +403,7 @@ static int grid_build(const orc_config *cfg, const orc_parts *p, grid *g) {
   g->index = (int *)malloc(sizeof(int) * (size_t)(p->n > 0 ? p->n : 1));
   if (!g->cell_of || !g->start || !g->index) return 1;
   double inv_w[3];
-  for (int k = 0; k < 3; k++) inv_w[k] = (double)g->nc[k] / cfg->dim[k];
+  for (int k = 0; k < 3; k++) inv_w[k] = 1.0 / g->w[k];
   for (int i = 0; i < p->n; i++) {
     int c[3];
     for (int k = 0; k < 3; k++)
@@ -427,10 +511,12 @@ static void grid_hmax(const orc_parts *p, grid *g) {
 
 
 /* The 27 neighbour cells of cell cid with the image translation t applied
- * to positions of j (x_j + t is the image nearest to cell cid). */
+ * to positions of j (x_j + t is the image nearest to cell cid), and the
+ * frame of the pair contract for (cell cid, neighbour). */
 typedef struct nbr {
   int cell;
   double t[3];
+  orc_frame fr;
 } nbr;
 
 static void neighbours27(const grid *g, const double dim[3], int cid,
@@ -458,6 +544,14 @@ static void neighbours27(const grid *g, const double dim[3], int cid,
           cc[k] = v;
         }
         out[m].cell = (cc[0] * g->nc[1] + cc[1]) * g->nc[2] + cc[2];
+        /* cell corners as space_regrid sets them: loc = index * width
+         * (space_regrid.c:307) */
+        double li[3], lj[3];
+        for (int k = 0; k < 3; k++) {
+          li[k] = c[k] * g->w[k];
+          lj[k] = cc[k] * g->w[k];
+        }
+        orc_pair_frame(li, lj, dim, &out[m].fr);
         m++;
       }
 }
@@ -580,7 +674,7 @@ int orc_pass_fp64(const orc_config *cfg, const orc_parts *p, orc_out64 *o) {
           const int j = g.index[jj];
           if (j == i || is_inhibited(p->time_bin[j])) continue;
           float dxf[3], r2f;
-          orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], b->t, dxf, &r2f);
+          orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], &b->fr, dxf, &r2f);
           if (!(r2f < hig2)) continue;
           double dx[3];
           for (int k = 0; k < 3; k++)
@@ -714,7 +808,7 @@ int orc_pass_fp64(const orc_config *cfg, const orc_parts *p, orc_out64 *o) {
           const int j = g.index[jj];
           if (j == i || is_inhibited(p->time_bin[j])) continue;
           float dxf[3], r2f;
-          orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], b->t, dxf, &r2f);
+          orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], &b->fr, dxf, &r2f);
           if (!(r2f < hig2)) continue;
           double dx[3];
           for (int k = 0; k < 3; k++)
@@ -725,13 +819,14 @@ int orc_pass_fp64(const orc_config *cfg, const orc_parts *p, orc_out64 *o) {
               ((double)p->v[3 * i] - p->v[3 * j]) * dx[0] +
               ((double)p->v[3 * i + 1] - p->v[3 * j + 1]) * dx[1] +
               ((double)p->v[3 * i + 2] - p->v[3 * j + 2]) * dx[2];
-          const double mu = r_inv * (dvdr < 0 ? dvdr : 0.0);
-          const double nv = c_all[i] + c_all[j] - 3.0 * mu;
+          const double mu =
+              r_inv * MUT(8, (dvdr < 0 ? dvdr : 0.0), (dvdr > 0 ? dvdr : 0.0));
+          const double nv = c_all[i] + c_all[j] - MUT(7, 3.0, 2.0) * mu;
           if (nv > vsig) vsig = nv;
           double wi, wi_dx;
           orc_kernel_deval64(r / h, &wi, &wi_dx);
           const double lt = (double)p->m[j] * ((double)p->u[i] - p->u[j]) *
-                            r_inv * wi_dx / rho_all[j];
+                            MUT(6, r_inv, 1.0) * wi_dx / rho_all[j];
           lapu += lt;
           s_lapu += fabs(lt);
           if (alpha_all[j] > avmax) avmax = alpha_all[j];
@@ -740,8 +835,8 @@ int orc_pass_fp64(const orc_config *cfg, const orc_parts *p, orc_out64 *o) {
       }
       const double h_inv = 1.0 / h;
       const double hid1 = h_inv * h_inv * h_inv * h_inv;
-      lapu *= 2.0 * hid1;
-      o->s_laplace_u[i] = s_lapu * 2.0 * hid1;
+      lapu *= MUT(5, 2.0, 1.0) * hid1;
+      o->s_laplace_u[i] = s_lapu * MUT(5, 2.0, 1.0) * hid1;
       o->v_sig[i] = vsig;
       o->laplace_u[i] = lapu;
       o->alpha_visc_max_ngb[i] = avmax;
@@ -756,13 +851,15 @@ int orc_pass_fp64(const orc_config *cfg, const orc_parts *p, orc_out64 *o) {
       const double div_v = o->div_v[i];
       const double div_v_dt =
           dt_alpha == 0.0 ? 0.0 : (div_v - (double)p->div_v_prev[i]) / dt_alpha;
-      const double S = div_v < 0.0 ? ksp * ksp * fmax(0.0, -div_v_dt) : 0.0;
+      const double S = div_v < 0.0 ? MUT(9, ksp * ksp, h * h) * fmax(0.0, -div_v_dt)
+                                   : 0.0;
       const double alpha_loc = (double)cfg->alpha_max * S / (c * c + S);
       double alpha = alpha_all[i];
       if (alpha_loc > alpha)
         alpha = alpha_loc;
       else {
-        const double tr = dt_alpha * sct_inv * (double)cfg->visc_length;
+        const double tr =
+            dt_alpha * sct_inv * MUT(10, (double)cfg->visc_length, 1.0);
         alpha += alpha_loc * tr;
         alpha /= (1.0 + tr);
       }
@@ -770,7 +867,8 @@ int orc_pass_fp64(const orc_config *cfg, const orc_parts *p, orc_out64 *o) {
       o->alpha_visc[i] = alpha;
       o->div_v_dt[i] = div_v_dt;
       const double dtpi = vsig * ksp_inv;
-      const double sqrt_u_inv = 1.0 / sqrt((double)p->u[i]);
+      const double sqrt_u_inv =
+          MUT(11, 1.0 / sqrt((double)p->u[i]), 1.0 / (double)p->u[i]);
       double adt = (double)cfg->diff_beta * ksp * lapu * sqrt_u_inv;
       adt -= (adiff_all[i] - (double)cfg->diff_alpha_min) * dtpi;
       double na = adiff_all[i] + adt * dt_alpha;
@@ -813,7 +911,7 @@ int orc_pass_fp64(const orc_config *cfg, const orc_parts *p, orc_out64 *o) {
           const int j = g.index[jj];
           if (j == i || is_inhibited(p->time_bin[j])) continue;
           float dxf[3], r2f;
-          orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], b->t, dxf, &r2f);
+          orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], &b->fr, dxf, &r2f);
           const float hj = p->h[j];
           if (!(r2f < hig2 || r2f < hig2_of(hj))) continue;
           double dx[3];
@@ -834,13 +932,14 @@ int orc_pass_fp64(const orc_config *cfg, const orc_parts *p, orc_out64 *o) {
               ((double)p->v[3 * i + 2] - p->v[3 * j + 2]) * dx[2];
           const double omega = dvdr < 0 ? dvdr : 0.0;
           const double mu = r_inv * omega;
-          const double vsig = c_all[i] + c_all[j] - 3.0 * mu;
+          const double vsig = c_all[i] + c_all[j] - MUT(16, 3.0, 2.0) * mu;
           const double f_ij = 1.0 - fi / mj;
           const double f_ji = 1.0 - f_all[j] / mi;
           const double rho_ij = rhoi + rhoj;
           const double alpha = alpha_all[i] + alpha_all[j];
-          const double visc =
-              -0.25 * alpha * vsig * mu * (bal_all[i] + bal_all[j]) / rho_ij;
+          const double visc = -0.25 * alpha * vsig * mu *
+                              (bal_all[i] + bal_all[j]) /
+                              MUT(2, rho_ij, 2.0 * rhoi);
           const double visc_acc = 0.5 * visc * (wi_dr * f_ij + wj_dr * f_ji) * r_inv;
           const double Pri = Pi / (rhoi * rhoi) * f_ij;
           const double Prj = Pj / (rhoj * rhoj) * f_ji;
@@ -850,13 +949,17 @@ int orc_pass_fp64(const orc_config *cfg, const orc_parts *p, orc_out64 *o) {
             s_a[k] += fabs(mj * dx[k] * r_inv) *
                       (fabs(Pri * wi_dr) + fabs(Prj * wj_dr) + fabs(visc_acc * r));
           }
-          const double sph_du = Pri * dvdr * r_inv * wi_dr;
-          const double visc_du = 0.5 * visc_acc * dvdr;
-          const double ad = (Pi * adiff_all[i] + Pj * adiff_all[j]) / (Pi + Pj);
+          const double sph_du = Pri * dvdr * r_inv * MUT(15, wi_dr, wj_dr);
+          const double visc_du = MUT(1, 0.5, 1.0) * visc_acc * dvdr;
+          const double ad =
+              (Pi * adiff_all[i] + Pj * MUT(3, adiff_all[j], adiff_all[i])) /
+              (Pi + Pj);
           const double v_diff =
               ad * 0.5 * (sqrt(2.0 * fabs(Pi - Pj) / rho_ij) + fabs(r_inv * dvdr));
-          const double diff_du = v_diff * ((double)p->u[i] - p->u[j]) *
-                                 (f_ij * wi_dr / rhoi + f_ji * wj_dr / rhoj);
+          const double diff_du =
+              v_diff *
+              MUT(4, (double)p->u[i] - p->u[j], (double)p->u[j] - p->u[i]) *
+              (f_ij * wi_dr / rhoi + f_ji * wj_dr / rhoj);
           udt += (sph_du + visc_du + diff_du) * mj;
           s_udt += (fabs(sph_du) + fabs(visc_du) + fabs(diff_du)) * mj;
           hdt -= mj * dvdr * r_inv / rhoj * wi_dr;
@@ -911,7 +1014,7 @@ static void density32(const orc_config *cfg, const grid *g, const orc_parts *p,
       const int j = jl[jj];
       if (j == i || is_inhibited(p->time_bin[j])) continue;
       float dx[3], r2;
-      orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], b->t, dx, &r2);
+      orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], &b->fr, dx, &r2);
       if (!(r2 < hig2)) continue;
       const float r = sqrtf(r2);
       const float mj = p->m[j];
@@ -1030,7 +1133,8 @@ int orc_pass_fp32(const orc_config *cfg, orc_parts *p, orc_out32 *o) {
         if (fabsf(h_new - h_old) > eps * h_old) {
           if ((h_new == left && h_old == right) ||
               (h_old == left && h_new == right))
-            p->h[i] = cbrtf(0.5f * (left * left * left + right * right * right));
+            p->h[i] =
+                orc_cbrt32(0.5f * (left * left * left + right * right * right));
           else
             p->h[i] = h_new;
           if (p->h[i] < cfg->h_max && p->h[i] > cfg->h_min) continue;
@@ -1129,7 +1233,7 @@ int orc_pass_fp32(const orc_config *cfg, orc_parts *p, orc_out32 *o) {
           const int j = jl[jj];
           if (j == i || is_inhibited(p->time_bin[j])) continue;
           float dx[3], r2;
-          orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], b->t, dx, &r2);
+          orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], &b->fr, dx, &r2);
           if (!(r2 < hig2)) continue;
           const float r = sqrtf(r2);
           const float r_inv = r ? 1.0f / r : 0.0f;
@@ -1242,7 +1346,7 @@ int orc_pass_fp32(const orc_config *cfg, orc_parts *p, orc_out32 *o) {
           const int j = jl[jj];
           if (j == i || is_inhibited(p->time_bin[j])) continue;
           float dx[3], r2;
-          orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], b->t, dx, &r2);
+          orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], &b->fr, dx, &r2);
           const float hj = p->h[j];
           if (!(r2 < hig2 || r2 < hig2_of(hj))) continue;
           pj[P_V0] = p->v[3 * j];
@@ -1279,35 +1383,100 @@ int orc_pass_fp32(const orc_config *cfg, orc_parts *p, orc_out32 *o) {
 }
 
 /* ------------------------------------------------------------------------ */
-/* O(N^2) brute-force counts (pairs_all_density / pairs_all_force style,
- * tools.c:207-570), minimal-image translation instead of cells.           */
+/* O(N^2) brute-force counts: every pair of particles, in the reference's   */
+/* brute-force checkers' own expression (self_all_density tools.c:590-650, */
+/* pairs_all_density / pairs_all_force tools.c:207-470): a pair in one      */
+/* top-level cell rounds the FP64 difference, any other pair works in the   */
+/* frame of its two cells (orc_pair_frame) and folds dx with nearest()      */
+/* (periodic.h:66-72).  No neighbour list, no stencil.                      */
 /* ------------------------------------------------------------------------ */
 int orc_count_bruteforce(const orc_config *cfg, const orc_parts *p, int mode,
                          int32_t *counts) {
+  grid g;
+  if (grid_build(cfg, p, &g)) return 1;
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < p->n; i++) {
     counts[i] = 0;
     if (!is_active(cfg, p->time_bin[i]) || is_inhibited(p->time_bin[i]))
       continue;
     const float hig2 = hig2_of(p->h[i]);
+    const int ci = g.cell_of[i];
+    const int cix[3] = {ci / (g.nc[2] * g.nc[1]), (ci / g.nc[2]) % g.nc[1],
+                        ci % g.nc[2]};
+    double loc_i[3];
+    for (int k = 0; k < 3; k++) loc_i[k] = cix[k] * g.w[k];
     int c = 0;
     for (int j = 0; j < p->n; j++) {
       if (j == i || is_inhibited(p->time_bin[j])) continue;
-      double t[3];
-      for (int k = 0; k < 3; k++) {
-        /* nearest() (periodic.h:66-72) applied to x_j - x_i picks t */
-        const double d = p->x[3 * j + k] - p->x[3 * i + k];
-        t[k] = d > 0.5 * cfg->dim[k] ? -cfg->dim[k]
-                                     : (d < -0.5 * cfg->dim[k] ? cfg->dim[k] : 0.0);
-      }
+      const int cj = g.cell_of[j];
+      const int cjx[3] = {cj / (g.nc[2] * g.nc[1]), (cj / g.nc[2]) % g.nc[1],
+                          cj % g.nc[2]};
+      double loc_j[3];
+      for (int k = 0; k < 3; k++) loc_j[k] = cjx[k] * g.w[k];
+      orc_frame fr;
+      orc_pair_frame(loc_i, loc_j, cfg->dim, &fr);
       float dx[3], r2;
-      orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], t, dx, &r2);
+      orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], &fr, dx, &r2);
+      if (!fr.self) {
+        for (int k = 0; k < 3; k++) {
+          const double b = cfg->dim[k];
+          dx[k] = (float)(dx[k] > 0.5 * b ? dx[k] - b
+                                          : (dx[k] < -0.5 * b ? dx[k] + b
+                                                              : (double)dx[k]));
+        }
+        r2 = fmaf(dx[2], dx[2], fmaf(dx[1], dx[1], dx[0] * dx[0]));
+      }
       const int hit = mode == 0 ? (r2 < hig2)
                                 : (r2 < hig2 || r2 < hig2_of(p->h[j]));
       c += hit;
     }
     counts[i] = c;
   }
+  grid_free(&g);
+  return 0;
+}
+
+/* Mask decisions of the density loop (r2 < (gamma_K h_i)^2 over the 27-cell
+ * stencil) on which the reference's contract (orc_pair_dx32) and the single
+ * FP64-difference rounding (float)((x_i - t) - x_j) disagree: out[0]
+ * neighbours only under the reference's, out[1] only under the other,
+ * out[2] candidate pairs examined, out[3] pairs inside under both. */
+int orc_contract_flips(const orc_config *cfg, const orc_parts *p,
+                       int64_t out[4]) {
+  grid g;
+  if (grid_build(cfg, p, &g)) return 1;
+  int64_t a = 0, b = 0, tot = 0, both = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(+ : a, b, tot, both)
+  for (int cid = 0; cid < g.ncells; cid++) {
+    nbr nb[27];
+    neighbours27(&g, cfg->dim, cid, nb);
+    for (int ii = g.start[cid]; ii < g.start[cid + 1]; ii++) {
+      const int i = g.index[ii];
+      if (!is_active(cfg, p->time_bin[i]) || is_inhibited(p->time_bin[i]))
+        continue;
+      const float hig2 = hig2_of(p->h[i]);
+      for (int q = 0; q < 27; q++) {
+        const nbr *bq = &nb[q];
+        for (int jj = g.start[bq->cell]; jj < g.start[bq->cell + 1]; jj++) {
+          const int j = g.index[jj];
+          if (j == i || is_inhibited(p->time_bin[j])) continue;
+          float dx[3], r2, dx2[3], r22;
+          orc_pair_dx32(&p->x[3 * i], &p->x[3 * j], &bq->fr, dx, &r2);
+          pair_dx32_fp64(&p->x[3 * i], &p->x[3 * j], bq->t, dx2, &r22);
+          const int in1 = r2 < hig2, in2 = r22 < hig2;
+          a += in1 && !in2;
+          b += in2 && !in1;
+          both += in1 && in2;
+          tot++;
+        }
+      }
+    }
+  }
+  out[0] = a;
+  out[1] = b;
+  out[2] = tot;
+  out[3] = both;
+  grid_free(&g);
   return 0;
 }
 

@@ -18,18 +18,26 @@
  *    smoothing-length Newton-Raphson / bisection ghost with subset re-runs
  *    (runner_ghost.c:1113-1628).
  *
- * The FP32 pair contract (shared, by specification, with the HIP kernels):
- *   t      = periodic image translation of particle j (0 or +-dim per axis),
- *            chosen from the top-level cell offset of j's cell w.r.t. i's;
- *   dx_k   = (float)((x_i,k - t_k) - x_j,k)     (two FP64 roundings, then one
- *                                                 FP64->FP32 rounding)
- *   r2     = fmaf(dx2, dx2, fmaf(dx1, dx1, dx0 * dx0))
+ * The FP32 pair contract (shared, by specification, with the HIP kernels)
+ * is the expression of the reference's own loops and brute-force checkers
+ * for top-level cells (cells of 1/cdim of the box; a particle's cell is
+ * (int)(x * iwidth), space_cell_index.c:114-128):
+ *   same cell   dx_k = (float)(x_i,k - x_j,k)             (DOSELF1 :2459,
+ *                                                  self_all_density tools.c:617)
+ *   other cells dx_k = (float)(x_i,k - oi_k) - (float)(x_j,k - oj_k), where
+ *               oi, oj are the corner of the "upper" cell of the pair (the
+ *               second cell after space_getsid_and_swap_cells, space_getsid.h:
+ *               47-80) with the periodic shift applied on the side of the
+ *               other cell                       (DOPAIR1 :1327-1346, DOPAIR2
+ *                                          :1672-1781, pairs_all_* tools.c:218)
+ *   r2     = fmaf(dx2, dx2, fmaf(dx1, dx1, dx0 * dx0))  (the sum of squares as
+ *            gcc contracts it under -ffp-contract=fast)
  *   H2_i   = (h_i * h_i) * kernel_gamma2        (two FP32 roundings)
  *   density / gradient neighbour : r2 < H2_i
  *   force neighbour              : r2 < H2_i || r2 < H2_j
- * For t = 0 this is the reference's self-loop expression
- * (runner_doiact_functions_hydro.h:2459, tools.c:621) with the sum of
- * squares contracted into FMAs as gcc does under -ffp-contract=fast.
+ * The reference's ghost re-runs (DOPAIR_SUBSET :891-914, DOSELF_SUBSET
+ * :1135-1163) use the two expressions the other way round; the brute-force
+ * checkers, which define the neighbour sets here, do not.
  */
 #ifndef SWSPH_ORACLE_H
 #define SWSPH_ORACLE_H
@@ -110,8 +118,20 @@ float orc_kernel_root(void);
 float orc_kernel_norm(void);
 void orc_kernel_deval32(float u, float *w, float *dw_dx);
 void orc_kernel_deval64(double u, double *w, double *dw_dx);
-void orc_pair_dx32(const double xi[3], const double xj[3], const double t[3],
+float orc_cbrt32(float x);
+/* Frame of a pair of top-level cells (corners loc_i of i's cell, loc_j of
+ * j's cell): self != 0 for one cell; else x_i is taken relative to oi and
+ * x_j relative to oj, the upper cell's corner (space_getsid.h:47-80). */
+typedef struct orc_frame {
+  int self;
+  double oi[3], oj[3];
+} orc_frame;
+void orc_pair_frame(const double loc_i[3], const double loc_j[3],
+                    const double dim[3], orc_frame *f);
+void orc_pair_dx32(const double xi[3], const double xj[3], const orc_frame *f,
                    float dx[3], float *r2);
+int orc_contract_flips(const orc_config *cfg, const orc_parts *p,
+                       int64_t out[4]);
 
 /* Single-pair interactions in FP32, restating hydro_iact.h.  The particle
  * record is a flat float array, see ORC_P_* in sphenix_oracle.c. */
@@ -127,9 +147,9 @@ void orc_iact_force32(float r2, const float dx[3], float hi, float hj,
 int orc_pass_fp64(const orc_config *cfg, const orc_parts *p, orc_out64 *out);
 int orc_pass_fp32(const orc_config *cfg, orc_parts *p, orc_out32 *out);
 
-/* O(N^2) brute-force neighbour counts with the contract, image chosen by
- * the minimal-image rule instead of cells (pins the cell-based search on
- * small inputs).  mode 0 = density (r2 < H2_i), 1 = force. */
+/* O(N^2) brute-force neighbour counts over every pair of particles, in the
+ * reference checkers' expression (pins the 27-cell search on small inputs).
+ * mode 0 = density (r2 < H2_i), 1 = force. */
 int orc_count_bruteforce(const orc_config *cfg, const orc_parts *p, int mode,
                          int32_t *counts);
 

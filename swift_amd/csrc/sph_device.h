@@ -61,15 +61,55 @@ __device__ __forceinline__ float kernel_dw(float u) {
   return (q * (om * om * om)) * cdW;
 }
 
-// The FP32 pair contract: dx = (float)((x_i - t) - x_j) per axis, computed
-// from FP64 positions, and r2 as two FMAs.  xs = x_i - t is formed once per
-// (i, neighbour cell) by the caller.
-__device__ __forceinline__ float pair_r2(double xs, double ys, double zs,
-                                         double xj, double yj, double zj,
+// Frame of a pair of adjacent top-level cells, as space_getsid_and_swap_cells
+// finds it (space_getsid.h:47-80): the sort id of the cells' offset and the
+// swap (runner_flip, sort_part.h:59-61) that makes the second cell of the
+// pair the one in the positive half-space of the offset.  The pair loops
+// take both cells' particles relative to that upper cell's corner, with the
+// periodic shift on the other cell's side (runner_doiact_functions_hydro.h:
+// 1327-1341, 1672-1674; tools.c:218-220).  For cells adjacent on a grid of
+// >= 3 cells per axis the wrapped loc difference of getsid is the stencil
+// offset o times the width and its shift is the image translation t of j's
+// cell, so, with loc = index * width (space_regrid.c:307):
+//   sort id 13 (o = 0)           the same cell: a self pair;
+//   sort id > 13 (no swap)       x_i relative to oi = loc_j + t,
+//                                x_j relative to oj = loc_j;
+//   sort id < 13 (swap)          x_i relative to oi = loc_i,
+//                                x_j relative to oj = loc_i + (-t).
+// Writes fo = {oi[3], oj[3]}; returns true for a self pair.
+__device__ __forceinline__ bool pair_frame(const int o[3], const int ci[3],
+                                           const int cj[3], const double t[3],
+                                           const double w[3], double *fo) {
+  const int sid = 9 * (o[0] + 1) + 3 * (o[1] + 1) + (o[2] + 1);
+  const bool up_j = sid > 13;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double li = (double)ci[k] * w[k], lj = (double)cj[k] * w[k];
+    fo[k] = up_j ? lj + t[k] : li;
+    fo[3 + k] = up_j ? lj : li + (-t[k]);
+  }
+  return sid == 13;
+}
+
+// The FP32 pair contract, the expression of the reference's loops and
+// brute-force checkers: in one cell (self) the FP64 difference rounded once
+// (DOSELF1 :2459, DOSELF2 :2784, tools.c:617-619); across cells both
+// cell-frame offsets rounded to float and subtracted in float (DOPAIR1
+// :1327-1346, DOPAIR2 :1748-1781, tools.c:232-246); r2 as two FMAs, the
+// sum of squares as gcc contracts it.  fo = {oi[3], oj[3]} of pair_frame.
+__device__ __forceinline__ float pair_r2(bool self, double xi, double yi,
+                                         double zi, double xj, double yj,
+                                         double zj, const double *fo,
                                          float &dx, float &dy, float &dz) {
-  dx = (float)(xs - xj);
-  dy = (float)(ys - yj);
-  dz = (float)(zs - zj);
+  if (self) {
+    dx = (float)(xi - xj);
+    dy = (float)(yi - yj);
+    dz = (float)(zi - zj);
+  } else {
+    dx = (float)(xi - fo[0]) - (float)(xj - fo[3]);
+    dy = (float)(yi - fo[1]) - (float)(yj - fo[4]);
+    dz = (float)(zi - fo[2]) - (float)(zj - fo[5]);
+  }
   return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
 }
 
@@ -78,10 +118,10 @@ __device__ __forceinline__ float search_radius2(float h) {
 }
 
 // Radius of the FP32 pre-test for an exact squared radius R.  The pre-test
-// distance is formed from float positions relative to top-level cell
-// corners; each component differs from the pair contract's
-// (float)((x_i - t) - x_j) by at most eps (both are within a few ulp of the
-// real difference, and all terms are bounded by two cell widths).  Then
+// distance is formed from float positions relative to the candidate's
+// top-level cell corner; each component differs from the pair contract's
+// by at most eps (both are within a few float roundings of the real
+// difference, and all terms are bounded by a few cell widths).  Then
 // |r_f^2 - r^2| <= eps (2 sqrt(3) r + 3 eps) + 2 x 2^-22 r^2 (rounding of
 // both squares), so r^2 < R implies r_f^2 < pretest_radius2(R, eps).
 __device__ __forceinline__ float pretest_radius2(float R, float eps) {

@@ -927,7 +927,9 @@ int bin_sort(swsph_engine *e, const double *px, const double *py,
              const double *pz, int stride, int *vals_out) {
   const int ni = (int)e->n;
   double inv_w[3];
-  for (int k = 0; k < 3; k++) inv_w[k] = (double)e->grid.nc[k] / e->grid.dim[k];
+  // iwidth as space_regrid sets it (space_regrid.c:221-222)
+  for (int k = 0; k < 3; k++)
+    inv_w[k] = 1.0 / (e->grid.dim[k] / e->grid.nc[k]);
   HIPCHK(sphk::launch_bin_keys(px, py, pz, stride, e->grid, inv_w, ni,
                                e->d_key[0], e->d_val[0], e->st));
   int bits = 30;

@@ -139,15 +139,6 @@ __host__ __device__ inline int xcd_blocks(int n) {
   return 8 * ((per + kWaves - 1) / kWaves);
 }
 
-// Image code 9 (sx + 1) + 3 (sy + 1) + (sz + 1) of a translation.
-__device__ __forceinline__ unsigned int image_code(double tx, double ty,
-                                                   double tz) {
-  const int sx = tx < 0.0 ? 0 : (tx > 0.0 ? 2 : 1);
-  const int sy = ty < 0.0 ? 0 : (ty > 0.0 ? 2 : 1);
-  const int sz = tz < 0.0 ? 0 : (tz > 0.0 ? 2 : 1);
-  return (unsigned int)(9 * sx + 3 * sy + sz);
-}
-
 // LDS of one wave: the neighbour table, the compacted surviving chunks, the
 // staged candidate positions and the chunk table of the current 64 rows.
 // CI: i-particles per wave (S = 64 / CI candidate streams each); CJ:
@@ -165,7 +156,8 @@ struct GatherLds {
   float qoffc[27 * 3];  // neighbour image corners relative to the tile centre
   double ttab[27 * 3];
   double qorg[27 * 3];  // corner of each neighbour's image
-  unsigned int tcode[27];
+  double fo[27 * 6];    // pair-contract frame {oi, oj} of each neighbour
+  unsigned int ocode[27];  // stencil offset code of each neighbour (lists)
   int qk0[27], qpre[28], qj0[27], qj1[27], qcell[27];
   unsigned int plist[64];
   unsigned int cinfo[SLOTS];
@@ -182,7 +174,7 @@ __device__ __forceinline__ void interact_rows(
     const typename L::Rec *__restrict__ rec, const GatherLds<CI, CJ> &sh,
     const typename L::I &st, int s, unsigned long long m0,
     unsigned long long m1, typename L::Acc &acc, const NbrList &nl, int i,
-    float rt2, int &ncnt) {
+    float rt2, int &ncnt, unsigned int selfm) {
   constexpr int S = GatherLds<CI, CJ>::S, RPC = GatherLds<CI, CJ>::RPC;
   // next row of this lane (ascending), taken out of the masks
   auto pop = [&]() -> int {
@@ -204,16 +196,14 @@ __device__ __forceinline__ void interact_rows(
   using Rec = typename L::Rec;
   // Evaluates hit (j, q) with record r.
   auto eval = [&](int j, int q, const Rec &r) {
-    const double xs = st.x - sh.ttab[3 * q];
-    const double ys = st.y - sh.ttab[3 * q + 1];
-    const double zs = st.z - sh.ttab[3 * q + 2];
     float dx, dy, dz;
-    const float r2 = pair_r2(xs, ys, zs, r.x, r.y, r.z, dx, dy, dz);
+    const float r2 = pair_r2((selfm >> q) & 1u, st.x, st.y, st.z, r.x, r.y,
+                             r.z, &sh.fo[6 * q], dx, dy, dz);
     if (BUILD) {
       if (r2 < rt2) {
         if (ncnt < nl.Ks)
           nl.idx[((size_t)ncnt * nl.npad + i) * kListStreams + s] =
-              (sh.tcode[q] << 27) | (unsigned int)j;
+              (sh.ocode[q] << 27) | (unsigned int)j;
         ncnt++;
         if (L::test(r2, st, &r)) L::interact(r2, dx, dy, dz, st, r, acc);
       }
@@ -320,24 +310,38 @@ __device__ __forceinline__ void run_chunk(
   // stencil, one cell, or the nnb <= 27 images of a task bundle item).
   const int nq = nbl ? nnb : (stencil ? 27 : 1);
   int nchq = 0;
+  bool qself = false;
   if (lane < nq) {
     double tx = tx0, ty = ty0, tz = tz0;
     int cj = cj0;
+    int cx, cy, cz;
+    cell_coords(cell_i, la.grid, cx, cy, cz);
     if (nbl) {
       cj = nbl[lane].cell;
       tx = nbl[lane].t[0];
       ty = nbl[lane].t[1];
       tz = nbl[lane].t[2];
     } else if (stencil) {
-      int cx, cy, cz;
-      cell_coords(cell_i, la.grid, cx, cy, cz);
       cj = neighbour(la.grid, cx, cy, cz, kStencilOrder[lane], tx, ty, tz);
     }
     int jx, jy, jz;
     cell_coords(cj, la.grid, jx, jy, jz);
-    sh.qorg[3 * lane] = (double)jx * (la.grid.dim[0] / la.grid.nc[0]) + tx;
-    sh.qorg[3 * lane + 1] = (double)jy * (la.grid.dim[1] / la.grid.nc[1]) + ty;
-    sh.qorg[3 * lane + 2] = (double)jz * (la.grid.dim[2] / la.grid.nc[2]) + tz;
+    const double w[3] = {la.grid.dim[0] / la.grid.nc[0],
+                         la.grid.dim[1] / la.grid.nc[1],
+                         la.grid.dim[2] / la.grid.nc[2]};
+    const double t[3] = {tx, ty, tz};
+    const int ci3[3] = {cx, cy, cz}, cj3[3] = {jx, jy, jz};
+    // stencil offset of j's cell (adjacent cells, wrapped)
+    int o[3];
+    for (int k = 0; k < 3; k++) {
+      int v = cj3[k] - ci3[k];
+      v = v > 1 ? v - la.grid.nc[k] : (v < -1 ? v + la.grid.nc[k] : v);
+      o[k] = v;
+      sh.qorg[3 * lane + k] = (double)cj3[k] * w[k] + t[k];
+    }
+    double fo[6];
+    qself = pair_frame(o, ci3, cj3, t, w, fo);
+    for (int k = 0; k < 6; k++) sh.fo[6 * lane + k] = fo[k];
     const int k0 = ca.cell_chunk[cj], k1 = ca.cell_chunk[cj + 1];
     sh.qk0[lane] = k0;
     sh.qj0[lane] = la.cell_start[cj];
@@ -346,9 +350,11 @@ __device__ __forceinline__ void run_chunk(
     sh.ttab[3 * lane] = tx;
     sh.ttab[3 * lane + 1] = ty;
     sh.ttab[3 * lane + 2] = tz;
-    sh.tcode[lane] = image_code(tx, ty, tz);
+    sh.ocode[lane] = stencil ? (unsigned int)kStencilOrder[lane] : 13u;
     nchq = k1 - k0;
   }
+  // neighbours that are the i-cell itself (self-pair contract), as bits
+  const unsigned int selfm = (unsigned int)uniform((int)(unsigned int)__ballot(qself));
   int incl = nchq;  // inclusive prefix over lanes 0..31
 #pragma unroll
   for (int off = 1; off < 32; off <<= 1) {
@@ -542,7 +548,7 @@ __device__ __forceinline__ void run_chunk(
         if (nrow == 64 * kMasks) {
           wave_sync();
           interact_rows<L, CI, CJ, BUILD>(rec, sh, st, s, m0, m1, acc, nl, i,
-                                          rt2, ncnt);
+                                          rt2, ncnt, selfm);
           m0 = m1 = 0ull;
           nrow = 0;
         }
@@ -553,7 +559,7 @@ __device__ __forceinline__ void run_chunk(
   if (nrow > 0) {
     wave_sync();
     interact_rows<L, CI, CJ, BUILD>(rec, sh, st, s, m0, m1, acc, nl, i, rt2,
-                                    ncnt);
+                                    ncnt, selfm);
   }
   L::template combine<S>(acc);
   if (act && s == 0) L::store_acc(d, i, acc);
@@ -702,7 +708,12 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_list(
   const int n = act ? min(nl.cnt[i * S + s], nl.Ks) : 0;
   const unsigned int *ip = nl.idx + (size_t)i * S + s;
   const size_t stride = (size_t)nl.npad * S;
-  const double dxm = la.grid.dim[0], dym = la.grid.dim[1], dzm = la.grid.dim[2];
+  // the i-cell's coordinates
+  int ci3[3] = {0, 0, 0};
+  if (act) cell_coords(d.cell[i], la.grid, ci3[0], ci3[1], ci3[2]);
+  const double w[3] = {la.grid.dim[0] / la.grid.nc[0],
+                       la.grid.dim[1] / la.grid.nc[1],
+                       la.grid.dim[2] / la.grid.nc[2]};
   // Software pipeline: list entries are fetched two ahead and records one
   // ahead of the entry being evaluated, so the gathers overlap the bodies.
   unsigned int e1 = n > 0 ? ip[0] : 0u;
@@ -715,18 +726,20 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_list(
     e1 = e2;
     if (k + 2 < n) e2 = ip[(size_t)(k + 2) * stride];
     if (k + 1 < n) r1 = rec[e1 & kRowMask];
-    // image code 9 a + 3 b + c, a, b, c in {0, 1, 2} (translation a - 1,
-    // b - 1, c - 1 box lengths)
-    const int tc = (int)(e >> 27);
-    const int a = (tc * 57) >> 9;
-    const int rem = tc - 9 * a;
-    const int b = (rem * 11) >> 5;
-    const int c = rem - 3 * b;
-    const double xs = st.x - (double)(a - 1) * dxm;
-    const double ys = st.y - (double)(b - 1) * dym;
-    const double zs = st.z - (double)(c - 1) * dzm;
+    // stencil offset code of j's cell: the neighbour cell, its image
+    // translation and the pair-contract frame
+    const int oc = (int)(e >> 27);
+    double t[3];
+    const int cj = neighbour(la.grid, ci3[0], ci3[1], ci3[2], oc, t[0], t[1],
+                             t[2]);
+    int cj3[3];
+    cell_coords(cj, la.grid, cj3[0], cj3[1], cj3[2]);
+    const int o[3] = {oc / 9 - 1, (oc / 3) % 3 - 1, oc % 3 - 1};
+    double fo[6];
+    const bool self = pair_frame(o, ci3, cj3, t, w, fo);
     float dx, dy, dz;
-    const float r2 = pair_r2(xs, ys, zs, jr.x, jr.y, jr.z, dx, dy, dz);
+    const float r2 =
+        pair_r2(self, st.x, st.y, st.z, jr.x, jr.y, jr.z, fo, dx, dy, dz);
     if (L::test(r2, st, &jr)) L::interact(r2, dx, dy, dz, st, jr, acc);
   }
   L::template combine<S>(acc);

@@ -115,9 +115,9 @@ struct DevParts {
 // the density re-runs of the ghost and the gradient loop, which then test
 // only these.  Stream s (of kListStreams) of particle i owns slots
 // idx[(k * npad + i) * kListStreams + s], k < cnt[i * kListStreams + s];
-// an entry is (image code << 27) | j with image code
-// 9 (sx + 1) + 3 (sy + 1) + (sz + 1) for the periodic translation
-// (sx dim_x, sy dim_y, sz dim_z) of the j-cell.
+// an entry is (offset code << 27) | j with offset code
+// 9 (ox + 1) + 3 (oy + 1) + (oz + 1) for the stencil offset (ox, oy, oz) of
+// j's cell from i's cell.
 constexpr int kListStreams = 4;
 constexpr int kListChunk = 64 / kListStreams;
 // Smallest tile (i-particles per wave) of the work-queue schedules; the

@@ -58,8 +58,9 @@ __device__ __forceinline__ unsigned long long spread10(unsigned int v) {
 }
 
 // Key of particle i from its position (component k at p[k][stride * i]),
-// the cell rule of the host binning: floor(x * inv_w) clamped to the grid,
-// 10 Morton bits per axis from (x * inv_w - cell) * 1024.
+// the cell rule of space_parts_get_cell_index (space_cell_index.c:114-128):
+// (int)(x * iwidth) with iwidth = 1 / (dim / cdim), clamped to the grid;
+// 10 Morton bits per axis from (x * iwidth - cell) * 1024.
 __global__ void k_bin_keys(const double *px, const double *py,
                            const double *pz, int stride, GridDesc g,
                            double iw0, double iw1, double iw2, int n,
@@ -74,7 +75,7 @@ __global__ void k_bin_keys(const double *px, const double *py,
   unsigned long long m = 0ull;
   for (int k = 0; k < 3; k++) {
     const double s = x[k] * iw[k];
-    int v = (int)floor(s);
+    int v = (int)s;
     v = v < 0 ? 0 : (v >= g.nc[k] ? g.nc[k] - 1 : v);
     c[k] = v;
     const double f = (s - (double)v) * 1024.0;
@@ -100,6 +101,9 @@ __global__ void k_wrap(DevParts d, GridDesc g, int n) {
       x -= g.dim[k];
     else if (x < 0.0)
       x += g.dim[k];
+    // a position wrapped onto the upper face by rounding goes to 0
+    // (space_cell_index.c:113-122)
+    if (x == g.dim[k]) x = 0.0;
     p[k][i] = x;
   }
 }

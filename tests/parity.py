@@ -120,3 +120,25 @@ def assert_alpha(res: dict, ref: dict, pre: dict, props, time_base: float,
         e = np.abs(np.asarray(res[k], np.float64) - ref[k])
         bad = active & (e > b)
         assert not bad.any(), (k, int(bad.sum()), float((e / b)[active].max()))
+
+
+def contract_dx(xi, xj, ci, cj, w, t=None):
+    """The reference's pair expression (tools.c:218-246, 617-619;
+    runner_doiact_functions_hydro.h:1327-1346, 2459) for i in cell ci and j
+    in the adjacent cell cj (integer cell coordinates, j's image translated
+    by t): self pairs round the FP64 difference once; otherwise both
+    particles are taken relative to the corner of the upper cell (the one
+    whose offset from the other has its first non-zero component positive),
+    with the translation on the lower cell's side."""
+    if t is None:
+        t = np.zeros_like(xi)
+    o = cj - ci
+    o = np.where(o > 1, -1, np.where(o < -1, 1, o))  # wrapped offset
+    first = np.where(o[:, 0] != 0, o[:, 0], np.where(o[:, 1] != 0, o[:, 1], o[:, 2]))
+    self_ = np.all(o == 0, axis=1)
+    up_j = first > 0
+    li, lj = ci * w, cj * w
+    oi = np.where(up_j[:, None], lj + t, li)
+    oj = np.where(up_j[:, None], lj, li + (-t))
+    pair = (xi - oi).astype(np.float32) - (xj - oj).astype(np.float32)
+    return np.where(self_[:, None], (xi - xj).astype(np.float32), pair).astype(np.float32)

@@ -251,8 +251,9 @@ def test_task_entry_points_27cells(orc):
     chk = {k: np.array(v, copy=True) for k, v in st.items()}
     ref = orc.pass_fp64(props, chk)
     # particles of the main cell
-    inv_w = np.array(props.ncell) / np.array(props.dim)
-    cc = np.clip(np.floor(st["x"] * inv_w).astype(int), 0, np.array(props.ncell) - 1)
+    # the reference's cell rule: (int)(x * iwidth), iwidth = 1 / (dim / cdim)
+    inv_w = 1.0 / (np.array(props.dim) / np.array(props.ncell))
+    cc = np.clip((st["x"] * inv_w).astype(int), 0, np.array(props.ncell) - 1)
     in_main = (cc[:, 0] == cx) & (cc[:, 1] == cy) & (cc[:, 2] == cz)
     assert in_main.sum() > 0
     assert np.array_equal(res["n_density"][in_main], ref["n_density"][in_main])

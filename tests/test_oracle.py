@@ -275,3 +275,46 @@ def test_fp32_restatement_close_to_fp64(orc):
         assert rel_err(o32[k], ref[k], ref[sk] if sk else None) <= tol, k
     e = rel_err(o32["a_hydro"], ref["a_hydro"], ref["s_a_hydro"])
     assert e <= 1e-4
+
+
+@pytest.mark.parametrize("nc,dim", [(3, 1.0), (8, 1.0), (96, 1.0), (5, 1e3)])
+def test_pair_contract_is_the_reference_expression(orc, nc, dim):
+    """The oracle's pair frames (the getsid rule on cell corners,
+    space_getsid.h:47-80) and contract give, bit for bit, the expression of
+    the reference's brute-force checkers restated independently in numpy
+    from the stencil offset (tests/parity.py contract_dx), for pairs in
+    every one of the 27 neighbour positions, periodic images included."""
+    from parity import contract_dx
+    rng = np.random.default_rng(nc)
+    n = 3000
+    w = dim / nc
+    ci = rng.integers(0, nc, (n, 3))
+    o = rng.integers(-1, 2, (n, 3))
+    v = ci + o
+    cj = np.mod(v, nc)
+    t = np.where(v < 0, -dim, np.where(v >= nc, dim, 0.0))
+    xi = (ci + rng.uniform(0, 1, (n, 3))) * w
+    xj = (cj + rng.uniform(0, 1, (n, 3))) * w
+    # particles on cell faces too
+    face = rng.uniform(size=n) < 0.2
+    xj[face] = (cj[face] + rng.integers(0, 2, (face.sum(), 3)) * 0.999999) * w
+    want = contract_dx(xi, xj, ci, cj, w, t)
+    for k in range(n):
+        fr = orc.pair_frame(ci[k] * w, cj[k] * w, (dim,) * 3)
+        assert fr.self == int(np.all(o[k] == 0))
+        dx, r2 = orc.pair_dx32(xi[k], xj[k], fr)
+        assert dx.tobytes() == want[k].tobytes(), (k, o[k], dx, want[k])
+
+
+def test_contract_flips_are_rare(orc):
+    """The reference's expression and a single rounding of the FP64
+    difference, (float)((x_i - t) - x_j), disagree on a handful of mask
+    decisions at r ~ H: count them on the 32^3 lattice (8^3 cells) at its
+    converged h.  The neighbour sets are bit-exact only under the contract
+    the oracle and the GPU share."""
+    st = ics.lattice_box(32, seed=0)
+    props = HydroProps(ncell=(8, 8, 8))
+    orc.pass_fp32(props, st)
+    f = orc.contract_flips(props, st)
+    assert f["both"] > 1_800_000
+    assert f["ref_only"] + f["fp64_only"] <= 1e-5 * f["both"]

@@ -1,10 +1,12 @@
 """The FP32 candidate pre-test of the neighbour loops never rejects a pair
 that the pair contract accepts.
 
-The loops form a cheap distance from float positions relative to top-level
-cell corners, and compare it with a widened radius (pretest_radius2 in
-swift_amd/csrc/sph_device.h); the exact FP64 contract is applied afterwards
-to every pair that passes.  This test emulates both computations with IEEE
+The loops form a cheap distance from float positions relative to the
+candidate's top-level cell corner, and compare it with a widened radius
+(pretest_radius2 in swift_amd/csrc/sph_device.h); the exact pair contract --
+the reference's expression: the FP64 difference rounded once inside a cell,
+cell-frame offsets of the pair's upper cell rounded and subtracted in float
+across cells -- is applied afterwards to every pair that passes.  This test emulates both computations with IEEE
 float32/float64 arithmetic in numpy on adversarial pairs -- separations
 within 1e-6 of the search radius, particles at cell corners, radii from
 1e-3 to 1 cell width, boxes from 1e-3 to 1e3 -- and checks the implication
@@ -14,6 +16,8 @@ r^2 < R  =>  r_f^2 < R'.  It also checks that the bound is not vacuous
 from __future__ import annotations
 
 import numpy as np
+
+from parity import contract_dx
 
 f32 = np.float32
 
@@ -54,8 +58,8 @@ def _case(rng, dim, nc, hfrac, n=200000):
     ok = np.all((cj >= 0) & (cj <= 2), axis=1)
     xi, xj, cj, R = xi[ok], xj[ok], cj[ok], R[ok]
     ci = ci[ok]
-    # contract: dx = (float)((x_i - t) - x_j), t = 0 here (no wrap)
-    d = (xi - xj).astype(f32)
+    # contract (t = 0 here: no wrap)
+    d = contract_dx(xi, xj, ci, cj, w)
     r2 = _r2(d[:, 0], d[:, 1], d[:, 2])
     # pre-test: i relative to the j-cell corner, j relative to its corner
     oj = cj * w
@@ -123,7 +127,8 @@ def _f16_pretest_case(rng, n_tiles=150):
         t = f16(dd[:, 0] ** 2).astype(np.float64)
         t = f16(dd[:, 1] ** 2 + t).astype(np.float64)
         r2b = f16(dd[:, 2] ** 2 + t).astype(np.float64)
-        dc = (xi[k] - xj).astype(f32)
+        dc = contract_dx(xi[k], xj, np.floor(xi[k] / w).astype(np.int64),
+                         np.floor(xj / w).astype(np.int64), w)
         inside = _r2(dc[:, 0], dc[:, 1], dc[:, 2]) < hg2[k]
         rn = np.sqrt(hg2[k] * invR * invR) + 2 * e
         bnd = f16(np.minimum(rn * rn * (1 + 8 * u) * (1 + 2.0 ** -10),
