@@ -1,33 +1,48 @@
 #!/usr/bin/env python3
 """Benchmark of the SPHENIX neighbour pass on one MI355X.
 
-Workload (default): the GPU-scale drop-in case -- a periodic unit cube of
-128^3 = 2,097,152 gas particles on a jittered lattice, displaced in x so that
-rho ~ 1 + 0.5 sin(2 pi x), Gaussian velocities (sigma 0.1), u ~ U[0.5, 1.5],
-all particles active, 32^3 top-level cells, Wendland C2, eta = 1.2348.
+Headline workload: the clustered, uneven box -- 256^3 = 16,777,216 gas
+particles in a periodic unit cube, 70 % drawn from 1000 Plummer spheres
+(centres uniform, scale radii log-uniform in [0.002, 0.02], member counts
+Poisson) and 30 % uniform, equal masses, u log-uniform in [0.1, 100]; h
+converged by an all-active pass, then 10 % of the particles active by a
+random time-bin mask; 48^3 top-level cells (1 to ~5000 particles per cell,
+h spread over ~80x).  The batched pass runs the device work queue with
+autotuned tile shapes and is replayed from captured HIP graphs.
 
 A step is one full pass over that box: density -> ghost (smoothing-length
 iteration with subset re-runs) -> gradient -> extra ghost -> force -> end
-force, starting each time from the same initial state (restored device to
-device, inside the step).  The metric is SPH pair interactions/s counted as
-SWIFT's counters count them: directed density interactions (j != i,
-r < H_i) plus directed force interactions (r < max(H_i, H_j)) of the
-converged pass; ghost re-runs are overhead.  value = batched interactions/s.
+force, starting each time from the same state (restored device to device,
+inside the step).  The metric is SPH pair interactions/s counted as SWIFT's
+counters count them: directed density interactions (j != i, r < H_i) plus
+directed force interactions (r < max(H_i, H_j)) of the converged pass;
+ghost re-runs are overhead.  value = batched interactions/s.
 
 Beside it the line reports the plain per-task drop-in (one launch per
 self/pair task, per-cell ghosts), p50/p99 step latency, the roofline of the
-dominant kernel from HIP events on the engine stream, and the CPU baseline
-(the oracle's FP32 restatement of the pass, threaded over the host cores).
+dominant kernel from HIP events on the engine stream, the CPU baseline (the
+oracle's FP32 restatement of the pass with sorted candidate lists, built
+with -march=native on this host, threaded and pinned over the host cores,
+median of 5 after a warm-up, on the same clustered state), and side lines
+for the all-active modulated 128^3 box, the serving stream and (optional)
+the 512^3 multi-step run.
 """
 from __future__ import annotations
 
-import argparse
-import json
 import os
-import sys
-import time
 
-import numpy as np
+# The CPU baseline's OpenMP threads are pinned one per core; libgomp reads
+# these when it is first loaded, so they are set before anything loads it.
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+import argparse  # noqa: E402
+import json  # noqa: E402
+import subprocess  # noqa: E402
+import sys  # noqa: E402
+import time  # noqa: E402
+
+import numpy as np  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -42,39 +57,257 @@ FLOP_GRADIENT = 61
 FLOP_FORCE = 161
 PEAK_FP32_TFLOPS = 157.3      # MI355X FP32 vector peak (packed issue)
 PEAK_HBM_GBS = 8000.0
+LOOPS = ["density_loop", "gradient_loop", "force_loop"]
+IO_FIELDS = ["h", "rho", "pressure", "soundspeed", "f", "balsara",
+             "alpha_visc", "alpha_diff"]
 
 
 def _pct(a, q):
     return float(np.percentile(np.asarray(a), q))
 
 
-def _read_profile_value(path, kernel_key, name):
-    """A `kernel_key: name = value` line of a committed rocprofv3 summary."""
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        line = line.strip()
-        if line.startswith(kernel_key + ":"):
-            k, v = line[len(kernel_key) + 1:].split("=")
-            if k.strip() == name:
-                return float(v)
-    return None
-
-
-def _read_traffic(path, kernel_key):
-    """HBM bytes per launch from a committed rocprofv3 --pmc summary
-    (FETCH_SIZE and WRITE_SIZE in KiB per dispatch, copy-kernel scaled)."""
-    if not os.path.exists(path):
-        return None
+def _profile_values(path, key):
+    """`key: name = value` lines of a committed rocprofv3 summary."""
     vals = {}
+    if not os.path.exists(path):
+        return vals
     for line in open(path):
         line = line.strip()
-        if line.startswith(kernel_key + ":"):
-            k, v = line[len(kernel_key) + 1:].split("=")
+        if line.startswith(key + ":"):
+            k, v = line[len(key) + 1:].split("=")
             vals[k.strip()] = float(v)
-    if "hbm_bytes_per_launch" in vals:
-        return vals["hbm_bytes_per_launch"]
-    return None
+    return vals
+
+
+def clustered_state(n_side: int, ncell: int, seed: int = 0):
+    """The clustered box with h and the (io) state of a converged all-active
+    pass, then 10 % of the particles active."""
+    from swift_amd import Engine, HydroProps, ics
+    N = n_side ** 3
+    t0 = time.perf_counter()
+    st = ics.clustered_box(N, seed=seed, n_spheres=1000, a_min=0.002,
+                           a_max=0.02)
+    props = HydroProps(ncell=(ncell,) * 3)
+    with Engine(props) as eng:
+        eng.set_particles(st)
+        pre = eng.pass_batched()
+        res = eng.results(IO_FIELDS)
+    for k, v in res.items():
+        st[k] = v
+    ics.set_active_fraction(st, 0.1, seed=seed + 1)
+    print(f"bench: clustered {n_side}^3 box ready ({time.perf_counter() - t0:.1f} s,"
+          f" pre-pass {pre['ghost_rounds']} rounds)", file=sys.stderr, flush=True)
+    workload = (f"clustered box {n_side}^3 ({N} particles): 70 % in 1000 Plummer "
+                "spheres (scale radii log-uniform [0.002, 0.02]) + 30 % uniform, "
+                "u log-uniform [0.1, 100], h from a converged all-active pass, "
+                f"10 % active (random time-bin mask), {ncell}^3 top-level cells, "
+                "density+ghost+gradient+force")
+    return props, st, workload
+
+
+def modulated_state(n_side: int, ncell: int):
+    from swift_amd import HydroProps, ics
+    st = ics.modulated_box(n_side, seed=0, amp=0.5)
+    props = HydroProps(ncell=(ncell,) * 3)
+    workload = (f"modulated periodic box {n_side}^3 ({n_side ** 3} particles), "
+                f"rho ~ 1+0.5 sin(2 pi x), {ncell}^3 top-level cells, all active, "
+                "density+ghost+gradient+force")
+    return props, st, workload
+
+
+def measure(props, st, workload: str, steps: int, warmup: int, tune: bool,
+            graph: bool, plain: bool, traffic_key: str) -> dict:
+    """Times `steps` passes (after `warmup`), then the per-loop kernel times
+    of as many eager passes with HIP events on the engine stream, then one
+    plain per-task pass.  Returns the measurement and the roofline of the
+    dominant loop."""
+    from swift_amd import Engine
+    eng = Engine(props)
+    eng.set_particles(st)
+    eng.save_state()
+    tuning = None
+    if tune:
+        tuning = eng.autotune(repeats=2)
+        print(f"bench: autotuned {tuning}", file=sys.stderr, flush=True)
+    run = eng.pass_graph if graph else eng.pass_batched
+
+    def step():
+        eng.restore_state()
+        return run()
+
+    for _ in range(warmup):
+        step()
+    eng.sync()
+    times = []
+    stats = None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ts = time.perf_counter()
+        stats = step()          # returns after the pass has completed
+        times.append(time.perf_counter() - ts)
+    eng.sync()
+    total = time.perf_counter() - t0
+    print(f"bench: {steps} timed steps in {total:.3f} s", file=sys.stderr,
+          flush=True)
+    inter = stats["n_density"] + stats["n_force"]
+
+    # Per-loop kernel times: HIP events recorded on the engine stream around
+    # every neighbour-loop kernel launch (the tile build excluded) of the
+    # same number of eager passes with the same kernels (a captured graph
+    # carries no per-launch events).
+    eng.set_timing(True)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        eng.restore_state()
+        eager_stats = eng.pass_batched()
+    eng.sync()
+    eager_total = time.perf_counter() - t1
+    ms, nl = eng.timing()
+    eng.set_timing(False)
+    assert eager_stats["n_density"] == stats["n_density"]
+    kernels = {}
+    for k in range(3):
+        avg = ms[k] / max(nl[k], 1)
+        entry = {"avg_ms": avg, "launches_per_step": nl[k] / steps,
+                 "ms_per_step": ms[k] / steps,
+                 "share_of_step": ms[k] / (1e3 * total)}
+        if k == 0:
+            # every density launch of the pass: the first round and the
+            # ghost's re-runs, charged with the interactions they evaluated
+            n_eval = stats["n_density_all"]
+            flop = FLOP_DENSITY
+        elif k == 1:
+            n_eval = stats["n_gradient"]
+            flop = FLOP_GRADIENT
+        else:
+            n_eval = stats["n_force"]
+            flop = FLOP_FORCE
+        entry["interactions_per_step"] = n_eval
+        entry["tflops"] = n_eval * flop * steps / (ms[k] * 1e-3) / 1e12
+        kernels[LOOPS[k]] = entry
+    dom = int(np.argmax(ms))
+    name = LOOPS[dom]
+    avg_ms = kernels[name]["avg_ms"]
+    achieved = kernels[name]["tflops"]
+    prof = _profile_values(os.path.join(ROOT, "profiles", "pmc_traffic.txt"),
+                           f"{name}_{traffic_key}")
+    traffic = prof.get("hbm_bytes_per_launch")
+    valu = _profile_values(os.path.join(ROOT, "profiles", "pmc_valu.md"),
+                           f"{name}_{traffic_key}")
+    roofline = {"bound": "valu", "kernel": name, "achieved": achieved,
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                "hbm_gbs": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
+                "valu_busy": valu.get("valu_busy"),
+                "valu_insts_per_interaction": valu.get("valu_insts_per_interaction"),
+                "avg_launch_ms": avg_ms,
+                "flop_per_interaction": [FLOP_DENSITY, FLOP_GRADIENT,
+                                         FLOP_FORCE][dom]}
+    out = {"workload": workload, "particles": int(st["h"].shape[0]),
+           "active": int((st["time_bin"] <= props.max_active_bin).sum()),
+           "value": steps * inter / total, "unit": "interactions/s",
+           "ms_per_step": 1e3 * total / steps,
+           "p50_step_ms": 1e3 * _pct(times, 50),
+           "p99_step_ms": 1e3 * _pct(times, 99),
+           "interactions_per_step": {"density": stats["n_density"],
+                                     "force": stats["n_force"],
+                                     "gradient": stats["n_gradient"]},
+           "ghost_rounds": stats["ghost_rounds"],
+           "pass": "graph replay" if graph else "eager batched",
+           "eager_ms_per_step": 1e3 * eager_total / steps,
+           "launches_per_step": stats["launches"],
+           "tuning": tuning, "kernels": kernels, "roofline": roofline}
+    if plain:
+        eng.restore_state()
+        eng.sync()
+        ts = time.perf_counter()
+        ps = eng.pass_plain()
+        eng.sync()
+        pdt = time.perf_counter() - ts
+        pinter = ps["n_density"] + ps["n_force"]
+        out["plain"] = {"value": pinter / pdt, "unit": "interactions/s",
+                        "ms_per_step": 1e3 * pdt,
+                        "launches_per_step": ps["launches"],
+                        # the plain task order sums in another order, so h
+                        # and borderline neighbour counts may differ slightly
+                        "count_rel_diff": abs(pinter - inter) / max(inter, 1)}
+        out["batched_over_plain"] = out["value"] / out["plain"]["value"]
+        print(f"bench: plain pass {pdt:.2f} s", file=sys.stderr, flush=True)
+    eng.close()
+    return out
+
+
+def _native_port():
+    """The CPU baseline library built for this host (-march=native); the
+    prebuilt x86-64-v3 build if the host compiler is missing."""
+    path = os.path.join(ROOT, "oracle", "libcpuport_native.so")
+    try:
+        # always rebuilt (-B): a copy built on another host may not run here
+        subprocess.run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle"),
+                        "native"], check=True, capture_output=True, timeout=120)
+        return path, "-march=native"
+    except (OSError, subprocess.SubprocessError):
+        return None, "-march=x86-64-v3 (prebuilt)"
+
+
+def cpu_baseline(props, st, threads: int, small, runs: int = 5,
+                 budget_s: float = 6.0) -> dict:
+    """The oracle's FP32 restatement of the same pass on the host cores:
+    one warm-up pass, then `runs` timed passes, median.  When the warm-up
+    takes longer than `budget_s`, the timed passes keep a random subset of
+    the active particles (the same box, fewer active) so the leg stays
+    within a few tens of seconds.  The single-thread figure runs the same
+    construction at a smaller size (`small` = (props, state)), median of 3."""
+    from oracle import oracle as orc
+    from swift_amd import ics
+    port, flags = _native_port()
+
+    def one(p, state, nthreads):
+        cs = ics.copy_state(state)
+        ts = time.perf_counter()
+        o = orc.pass_fp32(p, cs, nthreads=nthreads, port=port or True)
+        dt = time.perf_counter() - ts
+        ci = int(o["n_density"].astype(np.int64).sum()) + int(
+            o["n_force"].astype(np.int64).sum())
+        return ci / dt, dt
+
+    def median(p, state, nthreads, k):
+        r = sorted(one(p, state, nthreads) for _ in range(k))
+        return r[len(r) // 2], [v for v, _ in r]
+
+    def n_active(p, state):
+        return int((state["time_bin"] <= p.max_active_bin).sum())
+
+    _, tw = one(props, st, threads)
+    sample = st
+    if tw > budget_s:
+        sample = ics.copy_state(st)
+        act = np.flatnonzero(sample["time_bin"] <= props.max_active_bin)
+        rng = np.random.default_rng(5)
+        off = rng.permutation(act)[int(round(budget_s / tw * act.size)):]
+        sample["time_bin"][off] = ics.NUM_TIME_BINS + 1
+    (med, med_dt), all_rates = median(props, sample, threads, runs)
+    sp, ss = small
+    one(sp, ss, 1)
+    (med1, dt1), _ = median(sp, ss, 1, 3)
+    n, nact = st["h"].shape[0], n_active(props, sample)
+    n1 = ss["h"].shape[0]
+    return {"value": med, "unit": "interactions/s", "cores": threads,
+            "kind": "port",
+            "sample": f"median of {runs} full passes (density+ghost+gradient+"
+                      f"force, one warm-up before) over the headline's clustered "
+                      f"{round(n ** (1 / 3))}^3 state with {nact} active particles"
+                      f" ({100.0 * nact / n:.2f} %), {med_dt:.2f} s wall each on "
+                      f"{threads} threads pinned one per core (the oracle's FP32 "
+                      "restatement with sorted per-axis candidate lists, "
+                      f"-O3 -ffast-math -funroll-loops {flags}, OpenMP over cells)",
+            "runs_interactions_per_s": all_rates,
+            "single_thread": {"value": med1, "unit": "interactions/s",
+                              "sample": f"median of 3 passes (one warm-up before) "
+                                        f"over the same construction at "
+                                        f"{round(n1 ** (1 / 3))}^3 ({n_active(sp, ss)}"
+                                        f" active), {dt1:.2f} s wall each on 1 "
+                                        "thread"}}
 
 
 def serving_bench(duration_s: float, rate: float, seed: int = 0,
@@ -157,70 +390,6 @@ def serving_bench(duration_s: float, rate: float, seed: int = 0,
             "mean_stage_ms": {k: float(np.mean(v)) for k, v in stages.items()},
             "requests_per_s": len(infos) / wall,
             "interactions_per_s": inter / wall, "wall_s": wall}
-
-
-def clustered_bench(n_side: int, ncell: int, steps: int, plain: bool,
-                    seed: int = 0) -> dict:
-    """The clustered, uneven workload: n_side^3 particles, 70 % in 1000
-    Plummer spheres (scale radii log-uniform in [0.002, 0.02]) and 30 %
-    uniform, u log-uniform in [0.1, 100], h from a converged all-active
-    pre-pass, then 10 % of the particles active (random time-bin mask)."""
-    from swift_amd import Engine, HydroProps, ics
-    N = n_side ** 3
-    t0 = time.perf_counter()
-    st = ics.clustered_box(N, seed=seed, n_spheres=1000, a_min=0.002,
-                           a_max=0.02)
-    props = HydroProps(ncell=(ncell,) * 3)
-    eng = Engine(props)
-    eng.set_particles(st)
-    pre = eng.pass_batched()
-    res = eng.results(["h", "rho", "pressure", "soundspeed", "f", "balsara",
-                       "alpha_visc", "alpha_diff"])
-    for k, v in res.items():
-        st[k] = v
-    ics.set_active_fraction(st, 0.1, seed=seed + 1)
-    eng.set_particles(st)
-    eng.save_state()
-    print(f"bench: clustered box ready ({time.perf_counter() - t0:.1f} s, "
-          f"pre-pass {pre['ghost_rounds']} rounds)", file=sys.stderr,
-          flush=True)
-    tuning = eng.autotune(repeats=2)
-    times = []
-    for k in range(steps + 1):
-        eng.restore_state()
-        eng.sync()
-        ts = time.perf_counter()
-        s = eng.pass_graph()
-        times.append(time.perf_counter() - ts)
-    times = times[1:]  # the first replay captures the graphs
-    inter = s["n_density"] + s["n_force"]
-    h = res["h"]
-    out = {"workload": f"clustered box {n_side}^3 ({N} particles), 1000 Plummer "
-                       "spheres + 30 % uniform, h from a converged all-active "
-                       f"pass, 10 % active, {ncell}^3 top-level cells",
-           "particles": N, "active": int((st["time_bin"] <= 56).sum()),
-           "h_max_over_min": float(h.max() / h.min()),
-           "value": inter / float(np.mean(times)), "unit": "interactions/s",
-           "ms_per_step": 1e3 * float(np.mean(times)),
-           "p50_step_ms": 1e3 * _pct(times, 50),
-           "p99_step_ms": 1e3 * _pct(times, 99),
-           "interactions_per_step": {"density": s["n_density"],
-                                     "force": s["n_force"]},
-           "ghost_rounds": s["ghost_rounds"], "tuning": tuning}
-    if plain:
-        eng.restore_state()
-        eng.sync()
-        ts = time.perf_counter()
-        ps = eng.pass_plain()
-        eng.sync()
-        pdt = time.perf_counter() - ts
-        pinter = ps["n_density"] + ps["n_force"]
-        out["plain"] = {"value": pinter / pdt, "unit": "interactions/s",
-                        "ms_per_step": 1e3 * pdt,
-                        "launches_per_step": ps["launches"]}
-        out["batched_over_plain"] = out["value"] / (pinter / pdt)
-    eng.close()
-    return out
 
 
 def _hbm_used_bytes() -> int:
@@ -318,18 +487,19 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=128, help="particles per axis")
-    ap.add_argument("--ncell", type=int, default=32, help="top-level cells per axis")
-    ap.add_argument("--plain-steps", type=int, default=1)
+    ap.add_argument("--n", type=int, default=256,
+                    help="particles per axis of the clustered headline box")
+    ap.add_argument("--ncell", type=int, default=48,
+                    help="top-level cells per axis of the headline box")
+    ap.add_argument("--modulated-n", type=int, default=128,
+                    help="particles per axis of the all-active modulated box "
+                         "(side line; 0: skip)")
+    ap.add_argument("--modulated-ncell", type=int, default=32)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0,
-                    help="host threads for the CPU baseline (0: min(16, cores))")
-    ap.add_argument("--cpu-n", type=int, default=0,
-                    help="particles per axis of the CPU sample (0: same as --n)")
-    ap.add_argument("--cpu-n1", type=int, default=0,
-                    help="particles per axis of the single-thread CPU sample "
-                         "(0: half of the threaded sample's)")
+                    help="host threads of the CPU baseline (0: OMP_NUM_THREADS "
+                         "if set, else every core this process may run on)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default schedule and tile shapes")
     ap.add_argument("--graph", type=int, default=1,
@@ -337,9 +507,6 @@ def main() -> int:
     ap.add_argument("--serving-s", type=float, default=5.0,
                     help="seconds of the serving stream (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=200.0)
-    ap.add_argument("--clustered-n", type=int, default=256,
-                    help="particles per axis of the clustered workload (0: skip)")
-    ap.add_argument("--clustered-ncell", type=int, default=48)
     ap.add_argument("--evolve-n", type=int, default=0,
                     help="particles per axis of the multi-step clustered run "
                          "(0: skip; 512 is the full-size case)")
@@ -357,161 +524,54 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
-    from swift_amd import Engine, HydroProps, ics
-
-    st = ics.modulated_box(args.n, seed=0, amp=0.5)
-    props = HydroProps(ncell=(args.ncell,) * 3)
-    eng = Engine(props)
-    eng.set_particles(st)
-    eng.save_state()
-    tuning = None
-    if not args.no_tune:
-        tuning = eng.autotune(repeats=2)
-        print(f"bench: autotuned {tuning}", file=sys.stderr, flush=True)
-    run = eng.pass_graph if args.graph else eng.pass_batched
-
-    def step():
-        eng.restore_state()
-        return run()
-
-    for _ in range(args.warmup):
-        step()
-    print(f"bench: warmup done ({args.warmup} steps)", file=sys.stderr, flush=True)
-    eng.sync()
-    times = []
-    stats = None
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        ts = time.perf_counter()
-        stats = step()          # returns after the pass has completed
-        times.append(time.perf_counter() - ts)
-    eng.sync()
-    total = time.perf_counter() - t0
-    print(f"bench: {args.steps} timed steps in {total:.3f} s", file=sys.stderr,
-          flush=True)
-    inter = stats["n_density"] + stats["n_force"]
-    value = args.steps * inter / total
-
-    # Per-loop kernel times: HIP events recorded on the engine stream around
-    # every neighbour-loop kernel launch (the tile build excluded) of the
-    # same number of eager passes with the same kernels (a captured graph
-    # carries no per-launch events).
-    eng.set_timing(True)
-    t1 = time.perf_counter()
-    for _ in range(args.steps):
-        eng.restore_state()
-        eager_stats = eng.pass_batched()
-    eng.sync()
-    eager_total = time.perf_counter() - t1
-    ms, nl = eng.timing()
-    eng.set_timing(False)
-    assert eager_stats["n_density"] == stats["n_density"]
-
-    # Roofline of the dominant neighbour-loop kernel.
-    names = ["density_loop", "gradient_loop", "force_loop"]
-    dom = int(np.argmax(ms))
-    kernels = {}
-    for k in range(3):
-        avg = ms[k] / max(nl[k], 1)
-        entry = {"avg_ms": avg, "launches_per_step": nl[k] / args.steps,
-                 "share_of_step": ms[k] / (1e3 * total)}
-        if k == 0:
-            # every density launch of the pass: the first round and the
-            # ghost's re-runs, charged with the interactions they evaluated
-            entry["interactions_evaluated"] = stats["n_density_all"]
-            entry["tflops"] = (stats["n_density_all"] * FLOP_DENSITY * args.steps
-                               / (ms[0] * 1e-3) / 1e12)
-        if k == 1:
-            entry["tflops"] = stats["n_gradient"] * FLOP_GRADIENT / (avg * 1e-3) / 1e12
-        if k == 2:
-            entry["tflops"] = stats["n_force"] * FLOP_FORCE / (avg * 1e-3) / 1e12
-        kernels[names[k]] = entry
-    if dom == 0:
-        flops = kernels["density_loop"]["tflops"] * 1e12
-        avg_ms = kernels["density_loop"]["avg_ms"]
-    elif dom == 1:
-        flops = kernels["gradient_loop"]["tflops"] * 1e12
-        avg_ms = kernels["gradient_loop"]["avg_ms"]
-    else:
-        flops = kernels["force_loop"]["tflops"] * 1e12
-        avg_ms = kernels["force_loop"]["avg_ms"]
-    achieved = flops / 1e12
-    traffic = _read_traffic(os.path.join(ROOT, "profiles", "pmc_traffic.txt"),
-                            f"{names[dom]}_{args.n}")
-    valu_busy = _read_profile_value(
-        os.path.join(ROOT, "profiles", "pmc_valu_128cube.md"),
-        f"{names[dom]}_{args.n}", "valu_busy")
-    roofline = {"bound": "valu", "kernel": names[dom], "achieved": achieved,
-                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
-                "hbm_gbs": (traffic / (avg_ms * 1e-3) / 1e9) if traffic else None,
-                "valu_busy": valu_busy,
-                "avg_launch_ms": avg_ms,
-                "flop_per_interaction": [FLOP_DENSITY, FLOP_GRADIENT,
-                                         FLOP_FORCE][dom]}
-
+    props, st, workload = clustered_state(args.n, args.ncell)
+    head = measure(props, st, workload, args.steps, args.warmup,
+                   not args.no_tune, bool(args.graph), not args.no_plain,
+                   f"clustered{args.n}")
     out = {
         "metric": "SPH pair interactions/s (density+force) batched vs plain on GPU; p50/p99 step ms",
-        "value": value,
+        "value": head["value"],
         "unit": "interactions/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * total / args.steps,
-        "p50_step_ms": 1e3 * _pct(times, 50),
-        "p99_step_ms": 1e3 * _pct(times, 99),
+        "ms_per_step": head["ms_per_step"],
+        "p50_step_ms": head["p50_step_ms"],
+        "p99_step_ms": head["p99_step_ms"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32 (fp64 positions)",
         "data": "synthetic",
-        "config": {"workload": f"modulated periodic box {args.n}^3 "
-                               f"({args.n ** 3} particles), rho ~ 1+0.5 sin(2 pi x), "
-                               f"{args.ncell}^3 top-level cells, all active, "
-                               "density+ghost+gradient+force",
-                   "particles": args.n ** 3, "top_cells": args.ncell ** 3,
+        "config": {"workload": workload, "particles": head["particles"],
+                   "active": head["active"], "top_cells": args.ncell ** 3,
                    "parallelism": "single GPU"},
-        "interactions_per_step": {"density": stats["n_density"],
-                                  "force": stats["n_force"],
-                                  "gradient": stats["n_gradient"]},
-        "ghost_rounds": stats["ghost_rounds"],
-        "pass": "graph replay" if args.graph else "eager batched",
-        "eager_ms_per_step": 1e3 * eager_total / args.steps,
-        "tuning": tuning,
-        "launches_per_step": stats["launches"],
-        "kernels": kernels,
-        "roofline": roofline,
+        "roofline": head["roofline"],
     }
+    for k in ("interactions_per_step", "ghost_rounds", "pass",
+              "eager_ms_per_step", "launches_per_step", "tuning", "kernels",
+              "plain", "batched_over_plain"):
+        if k in head:
+            out[k] = head[k]
 
-    if not args.no_plain and args.plain_steps > 0:
-        ptimes = []
-        pstats = None
-        for _ in range(args.plain_steps):
-            eng.restore_state()
-            eng.sync()
-            ts = time.perf_counter()
-            pstats = eng.pass_plain()
-            eng.sync()
-            ptimes.append(time.perf_counter() - ts)
-        pinter = pstats["n_density"] + pstats["n_force"]
-        pval = pinter / float(np.mean(ptimes))
-        out["plain"] = {"value": pval, "unit": "interactions/s",
-                        "ms_per_step": 1e3 * float(np.mean(ptimes)),
-                        "launches_per_step": pstats["launches"],
-                        "steps": args.plain_steps,
-                        # the plain task order sums in another order, so h
-                        # and borderline neighbour counts may differ slightly
-                        "same_counts": pinter == inter,
-                        "count_rel_diff": abs(pinter - inter) / max(inter, 1)}
-        out["batched_over_plain"] = value / pval
-        print(f"bench: plain pass {np.mean(ptimes):.2f} s", file=sys.stderr,
-              flush=True)
-    eng.close()
+    if not args.no_cpu:
+        threads = (args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", 0))
+                   or len(os.sched_getaffinity(0)))
+        # the single-thread sample: the same construction at 64^3
+        small = clustered_state(64, 12)[:2]
+        out["cpu_baseline"] = cpu_baseline(props, st, threads, small)
+        print(f"bench: cpu baseline {out['cpu_baseline']['value']:.3e} "
+              "interactions/s", file=sys.stderr, flush=True)
+    del st
 
-    if args.clustered_n > 0:
-        out["clustered"] = clustered_bench(args.clustered_n, args.clustered_ncell,
-                                           args.steps, not args.no_plain)
-        print(f"bench: clustered {out['clustered']['ms_per_step']:.1f} ms/step",
+    if args.modulated_n > 0:
+        mp, ms_, mw = modulated_state(args.modulated_n, args.modulated_ncell)
+        out["modulated"] = measure(mp, ms_, mw, args.steps, args.warmup,
+                                   not args.no_tune, bool(args.graph),
+                                   not args.no_plain,
+                                   f"modulated{args.modulated_n}")
+        del ms_
+        print(f"bench: modulated {out['modulated']['ms_per_step']:.2f} ms/step",
               file=sys.stderr, flush=True)
 
     if args.evolve_n > 0:
@@ -524,37 +584,6 @@ def main() -> int:
         out["serving"] = serving_bench(args.serving_s, args.serving_rate)
         print(f"bench: serving {out['serving']['requests']} requests",
               file=sys.stderr, flush=True)
-
-    if not args.no_cpu:
-        from oracle import oracle as orc
-        ncpu = os.cpu_count() or 1
-        threads = args.cpu_threads or min(16, ncpu)
-        cn = args.cpu_n or args.n
-
-        def cpu_pass(n_side, nthreads):
-            cs = ics.modulated_box(n_side, seed=0, amp=0.5)
-            cp = HydroProps(ncell=(max(3, args.ncell * n_side // args.n),) * 3)
-            ts = time.perf_counter()
-            o = orc.pass_fp32(cp, cs, nthreads=nthreads, port=True)
-            dt = time.perf_counter() - ts
-            ci = int(o["n_density"].astype(np.int64).sum()) + int(
-                o["n_force"].astype(np.int64).sum())
-            return ci / dt, dt
-
-        cval, dt = cpu_pass(cn, threads)
-        cn1 = args.cpu_n1 or max(8, cn // 2)
-        cval1, dt1 = cpu_pass(cn1, 1)
-        out["cpu_baseline"] = {
-            "value": cval, "unit": "interactions/s", "cores": threads,
-            "kind": "port",
-            "sample": f"one full pass (density+ghost+gradient+force) over the "
-                      f"{cn}^3 modulated box, {dt:.1f} s wall on {threads} "
-                      "host threads (the oracle's FP32 restatement with sorted "
-                      "per-axis candidate lists, -O3 -ffast-math, OpenMP over "
-                      "cells)",
-            "single_thread": {"value": cval1, "unit": "interactions/s",
-                              "sample": f"{cn1}^3 modulated box, {dt1:.1f} s "
-                                        "wall on 1 thread"}}
     print(json.dumps(out))
     return 0
 

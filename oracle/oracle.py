@@ -18,20 +18,20 @@ _LIB = os.path.join(_HERE, "liborc.so")
 # candidate lists and SWIFT's production flags (timed, never the checker).
 _PORT = os.path.join(_HERE, "libcpuport.so")
 _lib = None
-_port = None
+_port = {}
 
 
 def build() -> None:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def port_lib():
-    global _port
-    if _port is None:
-        if not os.path.exists(_PORT):
+def port_lib(path: str = _PORT):
+    """The CPU-baseline build at `path` (default: the prebuilt one)."""
+    if path not in _port:
+        if path == _PORT and not os.path.exists(_PORT):
             build()
-        _port = C.CDLL(_PORT)
-    return _port
+        _port[path] = C.CDLL(path)
+    return _port[path]
 
 
 def _load(path):
@@ -172,7 +172,8 @@ def pass_fp32(props, st: dict, nthreads: int = 0, port: bool = False) -> dict:
     """FP32 restatement of the full pass including the ghost; updates the
     (io) fields of ``st`` in place (h converged) and returns the outputs.
     ``port`` runs the CPU-baseline build (sorted candidate lists, -O3
-    -ffast-math) instead of the oracle proper."""
+    -ffast-math) instead of the oracle proper: True for the prebuilt one, or
+    the path of another build of it."""
     n = st["h"].shape[0]
     out = {}
     o = OrcOut32()
@@ -182,8 +183,11 @@ def pass_fp32(props, st: dict, nthreads: int = 0, port: bool = False) -> dict:
         setattr(o, k, out[k].ctypes.data)
     cfg = make_config(props, nthreads)
     p, _keep = _parts(st)
-    rc = (port_lib() if port else lib()).orc_pass_fp32(C.byref(cfg), C.byref(p),
-                                                       C.byref(o))
+    if port:
+        L = port_lib(port) if isinstance(port, str) else port_lib()
+    else:
+        L = lib()
+    rc = L.orc_pass_fp32(C.byref(cfg), C.byref(p), C.byref(o))
     if rc != 0:
         raise RuntimeError(f"orc_pass_fp32 failed ({rc})")
     for k in ("rho", "pressure", "soundspeed", "f", "balsara", "alpha_visc",

@@ -725,6 +725,16 @@ int orc_pass_fp64(const orc_config *cfg, const orc_parts *p, orc_out64 *o) {
       o->s_div_v[i] = s_dv * hid1 * rho_inv;
       for (int k = 0; k < 3; k++)
         o->s_rot_v[3 * i + k] = s_rot[k] * hid1 * rho_inv;
+      if (cnt == 0 && hi >= cfg->h_max) {
+        /* The ghost ends a particle that finds no neighbour at h_max in
+         * hydro_part_has_no_neighbours (runner_ghost.c:1217-1227, 1432-1442;
+         * hydro.h:791-825): rho, wcount as above (self term only), the
+         * h-derivatives zeroed. */
+        rho_dh = 0.0;
+        wc_dh = 0.0;
+        o->s_rho_dh[i] = 0.0;
+        o->s_wcount_dh[i] = 0.0;
+      }
       o->rho[i] = rho;
       o->rho_dh[i] = rho_dh;
       o->wcount[i] = wc;

@@ -358,3 +358,52 @@ def test_sparse_tile_shape_steps():
                     assert np.abs(se[key] - sb[key]).max() <= 1e-4 * scale, (k, key)
         # the default rule switched between the shapes
         assert kinds == {0, 1}
+
+
+def test_clustered_four_bins_steps_rebuild(orc):
+    """The multi-step clustered construction at a size the oracle checks:
+    24^3 particles, 1000 Plummer spheres (scale radii in [0.002, 0.02]) and
+    a uniform background, u log-uniform in [0.1, 100], on a fixed 4-level
+    time-bin hierarchy (1.5 / 4.5 / 19 / 75 % in bins 1-4, so 1.5, 6, 25 or
+    100 % of the particles are active), boosted by a bulk velocity so that
+    particles leave their cells' stencils within a few steps.  Every step is
+    replayed through the entry points against the oracle, re-binning
+    included, and engine_step must equal that sequence bit for bit."""
+    from swift_amd import Engine, HydroProps, ics
+    st = ics.clustered_box(24 ** 3, seed=4, n_spheres=1000, a_min=0.002,
+                           a_max=0.02)
+    st["v"] += np.asarray((150.0, 80.0, -40.0), np.float32)
+    ics.set_bin_hierarchy(st, {1: 0.015, 2: 0.045, 3: 0.19, 4: 0.75}, seed=5)
+    # 6^3 cells of width 0.167 against a largest kernel support of ~0.15:
+    # a drift of ~0.007 outside a cell leaves the stencil
+    props = HydroProps(ncell=(6, 6, 6), time_base=2e-6)
+    integ = _integ(fixed=True)
+    oi = orc.integ(props.time_base, integ)
+    with Engine(props) as a, Engine(props) as b:
+        for e in (a, b):
+            e.set_particles(st)
+            e.set_integration(integ)
+        a.set_sparse_shape(4, 32, 0)
+        ti = te = 0
+        rebuilt = 0
+        fractions = set()
+        for k in range(10):
+            info = a.step()
+            ti_old, ti = ti, (0 if k == 0 else te)
+            if info["rebuilt"]:
+                rebuilt += 1
+                g = _state(b)
+                orc.drift(g, oi, ti_old, ti)
+                b.runner_do_drift_part(-1, ti_old, ti)
+                b.rebuild()
+                _same(_state(b), {**g, "x": _wrap(g["x"])}, STATE, what="drift")
+                te, na = _rest_of_step(orc, b, props, st["m"], oi, ti)
+            else:
+                te, na = _manual_step(orc, b, props, st["m"], oi, ti_old, ti,
+                                      k == 0)
+            assert info["ti_end_min"] == te and info["n_active"] == na
+            fractions.add(round(na / st["h"].shape[0], 2))
+            _same(_state(a), _state(b), STATE, rtol=0, what=f"step {k}")
+        assert rebuilt >= 1
+        # every level of the hierarchy was active at some step
+        assert len(fractions) >= 4, fractions
