@@ -122,35 +122,91 @@ int swsph_cell_count(swsph_engine *e, int cid, int *count);
 int swsph_sync(swsph_engine *e);
 
 /* ---- drop-in task entry points ------------------------------------------
- * Each replaces the named SWIFT function for one top-level cell (pair).  */
+ * Each replaces the task function runner_main.c calls for a self or pair
+ * task of the hydro loops (src/runner_main.c:215-300): runner_dosub_self1_-
+ * density(r, ci, below_h_max = 0, 1) and its siblings
+ * (src/runner_doiact_functions_hydro.h:2933-3092), on a top-level cell
+ * (pair).  The reference's call recurses through the cell's tree and
+ * processes every particle at the tree level its h selects: the depth
+ * windows limit_h_min / limit_h_max of DOSELF1_BRANCH / DOPAIR1_BRANCH
+ * (src/runner_doiact_hydro.h:176-184) split the particles between levels
+ * so that each interaction is done once.  The device holds no sub-cells:
+ * one call here does the interactions of every level of the top-level cell
+ * at once, which is what the reference's recursion adds up to.  A task on a
+ * split sub-cell (depth > 0, after scheduler_splittasks) has no counterpart
+ * and the SWIFT-side shim refuses it.
+ *
+ * Completion: calls enqueue their work on the engine's stream and return;
+ * work of later calls on the same engine runs after it (the stream carries
+ * the dependencies of the task graph between device tasks).  Before host
+ * code reads a task's results -- the point where SWIFT's scheduler_done
+ * releases dependants that read struct part (src/scheduler.c:1203-1222) --
+ * swsph_cell_write_back waits for the engine and writes the results into
+ * the caller's particle memory. */
 
 /* hydro_init_part on the active particles of the cell (called from the
  * drift/init in SWIFT, src/hydro/SPHENIX/hydro.h:564). */
 int swsph_init_cell(swsph_engine *e, int ci);
-/* runner_doself1_branch_density (src/runner_doiact_hydro.h:177). */
+/* runner_dosub_self1_density / DOSELF1_BRANCH (src/runner_doiact_hydro.h:
+ * 176-177; src/runner_main.c:222). */
 int swsph_doself1_density(swsph_engine *e, int ci);
-/* runner_dopair1_branch_density (src/runner_doiact_hydro.h:181). */
+/* runner_dosub_pair1_density / DOPAIR1_BRANCH (src/runner_doiact_hydro.h:
+ * 181-182; src/runner_main.c:281): both directions. */
 int swsph_dopair1_density(swsph_engine *e, int ci, int cj);
-/* runner_doself_subset_branch_density / runner_dopair_subset_branch_density
- * (src/runner_doiact_hydro.h:197-210): density for the particles of ci that
- * the ghost flagged for another round. */
-int swsph_doself_subset_density(swsph_engine *e, int ci);
-int swsph_dopair_subset_density(swsph_engine *e, int ci, int cj);
-/* runner_do_ghost (src/runner.h:94, src/runner_ghost.c:1113): iterates h to
- * convergence with subset re-runs over the cell's 27 density interactions. */
-int swsph_do_ghost(swsph_engine *e, int ci);
-/* runner_doself1_branch_gradient / runner_dopair1_branch_gradient
- * (src/runner_doiact_hydro.h:177-185, gradient instance). */
+/* runner_doself_subset_branch_density(r, ci, parts, ind, count) and
+ * runner_dopair_subset_branch_density(r, ci, parts_i, ind, count, cj)
+ * (src/runner_doiact_hydro.h:197-210): the density of `count` particles of
+ * cell ci against ci (resp. cj) only.  As in the reference, parts is the
+ * cell's particle array inside the caller's array and ind indexes into
+ * it: particle k of the subset is the caller's particle first + ind[k]
+ * (first = ci->hydro.parts - s->parts).  The particles must belong to ci
+ * (status 1 otherwise, nothing is run); inactive ones are skipped.  Waits
+ * for the index upload and the cell check before returning. */
+int swsph_doself_subset_density(swsph_engine *e, int ci, int64_t first,
+                                const int *ind, int count);
+int swsph_dopair_subset_density(swsph_engine *e, int ci, int64_t first,
+                                const int *ind, int count, int cj);
+/* runner_do_ghost(r, c, offset, ntasks, timer) (src/runner.h:94,
+ * src/runner_ghost.c:1113): iterates h to convergence with subset re-runs
+ * over the cell's 27 density interactions, for the share `offset` of
+ * `ntasks` ghost tasks of the cell (the reference takes particles
+ * offset, offset + ntasks, ... of the cell; here every ntasks-th particle
+ * of the cell's device order from offset on -- each particle in exactly
+ * one share).  Waits for its own convergence test, like the reference. */
+int swsph_do_ghost(swsph_engine *e, int ci, int offset, int ntasks);
+/* runner_dosub_self1_gradient / runner_dosub_pair1_gradient (gradient
+ * instance of src/runner_doiact_hydro.h:176-185; src/runner_main.c:228,
+ * 287). */
 int swsph_doself1_gradient(swsph_engine *e, int ci);
 int swsph_dopair1_gradient(swsph_engine *e, int ci, int cj);
 /* runner_do_extra_ghost (src/runner.h:96, src/runner_ghost.c:1016). */
 int swsph_do_extra_ghost(swsph_engine *e, int ci);
-/* runner_doself2_branch_force / runner_dopair2_branch_force
- * (src/runner_doiact_hydro.h:187-195). */
+/* runner_dosub_self2_force / runner_dosub_pair2_force (src/runner_doiact_-
+ * hydro.h:187-195; src/runner_main.c:232, 291). */
 int swsph_doself2_force(swsph_engine *e, int ci);
 int swsph_dopair2_force(swsph_engine *e, int ci, int cj);
 /* runner_do_end_hydro_force (src/runner.h:110, src/runner_others.c:815). */
 int swsph_do_end_hydro_force(swsph_engine *e, int ci);
+
+/* A destination in the caller's memory for one field: the value of the
+ * caller's particle p goes to (char *)base + p * stride, its components
+ * (vectors) at consecutive elements from there -- an array of structs such
+ * as struct part (base = &parts[0].rho, stride = sizeof(struct part)) or a
+ * plain array (stride = the element size times the components).  Element
+ * types as for swsph_get_field. */
+typedef struct swsph_field_dst {
+  int field;          /* enum swsph_field */
+  void *base;
+  int64_t stride;     /* bytes; |stride| >= the field's element bytes */
+} swsph_field_dst;
+/* Waits for all work enqueued on the engine, then writes the listed fields
+ * of the particles of cell ci (-1: of all particles) into the caller's
+ * memory: the results are in the caller's particles when it returns, as
+ * SWIFT's are in struct part when a task ends (src/scheduler.c:1203-1222).
+ * No counterpart in the reference, whose tasks write struct part in
+ * place. */
+int swsph_cell_write_back(swsph_engine *e, int ci,
+                          const swsph_field_dst *dst, int nfields);
 
 /* ---- whole passes -------------------------------------------------------
  * Plain: the C task loop of the reference's dependency order (all density

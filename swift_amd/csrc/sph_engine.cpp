@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -181,6 +182,14 @@ struct swsph_engine {
   static constexpr int kNoMab = -1000;
   int stale_mab = kNoMab;
   int last_pass_mab = kNoMab;
+  // Subsets of the caller's particles (swsph_do*_subset_density): flags per
+  // slot, the device slot of every caller index (valid for the binning
+  // generation inv_gen), the uploaded indices, and an error counter;
+  // allocated on first use.  perm_gen counts binnings.
+  int *d_subflag = nullptr, *d_inv = nullptr, *d_subind = nullptr;
+  unsigned int *d_suberr = nullptr;
+  size_t sub_cap = 0, subind_cap = 0;
+  uint64_t perm_gen = 1, inv_gen = 0;
 };
 
 namespace {
@@ -280,9 +289,14 @@ void free_dev(swsph_engine *e) {
                   e->ts.tile_cell, e->ts.ntiles, e->ts.heads,
                   e->d_cell_start, e->d_tiles, e->d.vfx, e->d.vfy, e->d.vfz,
                   e->d.ufull, e->d.perm, e->d_key[0], e->d_key[1],
-                  e->d_val[0], e->d_val[1], e->d_sort_tmp, e->d_stage};
+                  e->d_val[0], e->d_val[1], e->d_sort_tmp, e->d_stage,
+                  e->d_subflag, e->d_inv, e->d_subind, e->d_suberr};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  e->d_subflag = e->d_inv = e->d_subind = nullptr;
+  e->d_suberr = nullptr;
+  e->sub_cap = e->subind_cap = 0;
+  e->inv_gen = 0;
   e->d = DevParts{};
   e->ts = TileScratch{};
   e->nl.idx = nullptr;
@@ -757,7 +771,8 @@ int collect_timing(swsph_engine *e) {
   return 0;
 }
 
-int task(swsph_engine *e, LoopKind k, bool subset, int ci, int cj, bool both) {
+int task(swsph_engine *e, LoopKind k, bool subset, int ci, int cj, bool both,
+         const int *flag = nullptr) {
   int rc;
   if ((rc = cell_ok(e, ci)) || (rc = cell_ok(e, cj))) return rc;
   if ((rc = ensure_records(e))) return rc;
@@ -779,8 +794,9 @@ int task(swsph_engine *e, LoopKind k, bool subset, int ci, int cj, bool both) {
     for (int k2 = 0; k2 < 3; k2++) ta.t[3 + k2] = -t[k2];
   }
   if (ta.ntiles[0] + ta.ntiles[1] == 0) return 0;
-  HIPCHK(sphk::launch_task(k, subset, e->d, loop_args(e), chunk_args(tb), ta,
-                           e->st));
+  LoopArgs la = loop_args(e);
+  if (flag) la.flag = flag;
+  HIPCHK(sphk::launch_task(k, subset, e->d, la, chunk_args(tb), ta, e->st));
   e->launches++;
   return 0;
 }
@@ -826,24 +842,74 @@ int read_redo(swsph_engine *e, int *count, int *relist = nullptr,
   return 0;
 }
 
-int ghost_cell(swsph_engine *e, int ci, int *rounds) {
+// Device buffers of the caller-subset entry points: slot flags and the
+// slot of every caller index for the current binning, and room for
+// `count` uploaded indices.
+int ensure_subset(swsph_engine *e, int count) {
+  if (!e->d_subflag || e->sub_cap < e->cap_n) {
+    for (void *p : {(void *)e->d_subflag, (void *)e->d_inv, (void *)e->d_suberr})
+      if (p) (void)hipFree(p);
+    e->d_subflag = e->d_inv = nullptr;
+    e->d_suberr = nullptr;
+    e->sub_cap = 0;
+    e->inv_gen = 0;
+    const size_t n = std::max<size_t>(1, e->cap_n);
+    HIPCHK(dalloc(&e->d_subflag, n));
+    HIPCHK(dalloc(&e->d_inv, n));
+    HIPCHK(dalloc(&e->d_suberr, 1));
+    e->sub_cap = n;
+  }
+  if ((size_t)count > e->subind_cap) {
+    if (e->d_subind) (void)hipFree(e->d_subind);
+    e->d_subind = nullptr;
+    e->subind_cap = 0;
+    HIPCHK(dalloc(&e->d_subind, (size_t)count));
+    e->subind_cap = (size_t)count;
+  }
+  if (e->inv_gen != e->perm_gen) {
+    HIPCHK(sphk::launch_invert_perm(e->d.perm, e->d_inv, (int)e->n, e->st));
+    e->launches++;
+    e->inv_gen = e->perm_gen;
+  }
+  return 0;
+}
+
+// runner_do_ghost (runner_ghost.c:1113) for share `off` of `nt` of cell ci's
+// particles: rounds of the ghost kernel over the share, each followed by
+// the subset re-runs of the share's flagged particles over every density
+// interaction of the cell (runner_ghost.c:1548-1580).  The last share to
+// finish leaves the cell's gradient records and chunk bounds current.
+int ghost_cell(swsph_engine *e, int ci, int *rounds, int off = 0, int nt = 1) {
   const int lo = e->cell_start[ci], hi = e->cell_start[ci + 1];
   const GhostArgs ga = ghost_args(e);
   std::vector<int> nb;
   all_neighbours(e, ci, nb);
   int redo = 0;
   int it = 0;
+  int rc = 0;
+  // with shares, the re-runs take the flags of this share only (the other
+  // shares' particles keep the flags hydro_init_part set until their own
+  // ghost runs)
+  const int *flag = nullptr;
+  if (nt > 1) {
+    if ((rc = ensure_subset(e, 0))) return rc;
+    flag = e->d_subflag;
+  }
   for (; it < e->props.max_iter; it++) {
-    HIPCHK(sphk::launch_ghost(e->d, ga, lo, hi, e->d_redo, e->st));
+    HIPCHK(sphk::launch_ghost(e->d, ga, lo, hi, e->d_redo, e->st, nullptr,
+                              nullptr, off, nt));
     e->launches += 2;
-    int rc = read_redo(e, &redo);
-    if (rc) return rc;
+    if ((rc = read_redo(e, &redo))) return rc;
     if (redo == 0) break;
-    // Subset re-runs over every density interaction of the cell
-    // (runner_ghost.c:1548-1580).
-    if ((rc = task(e, LoopKind::Density, true, ci, ci, false))) return rc;
+    if (flag) {
+      HIPCHK(sphk::launch_share_flags(e->d_subflag, e->d.redo, lo, hi, off,
+                                      nt, e->st));
+      e->launches++;
+    }
+    if ((rc = task(e, LoopKind::Density, true, ci, ci, false, flag))) return rc;
     for (int cj : nb)
-      if ((rc = task(e, LoopKind::Density, true, ci, cj, false))) return rc;
+      if ((rc = task(e, LoopKind::Density, true, ci, cj, false, flag)))
+        return rc;
   }
   if (redo > 0)
     return fail(3, "Smoothing length failed to converge on %d particles", redo);
@@ -960,6 +1026,7 @@ int finish_binning(swsph_engine *e) {
   e->recs_ok = true;
   e->stale_mab = swsph_engine::kNoMab;
   e->fields_epoch++;
+  e->perm_gen++;
   for (auto &t : e->tables) t.valid = false;
   for (int k = 0; k < 3; k++)
     if (!chunk_table(e, table_c(e, k), &rc)) return rc;
@@ -1350,6 +1417,67 @@ int pass_batched_locked(swsph_engine *e, swsph_pass_stats *stats) {
   if ((rc = phase_finish(e))) return rc;
   return finish_stats(e, stats, it + 1, l0);
 }
+// Device arrays of a pass output or state field: up to three components
+// of one element type, in slot order.
+struct FieldView {
+  enum Kind { F64, F32, I32, I8 } kind = F32;
+  const void *c[3] = {nullptr, nullptr, nullptr};
+  int ncomp = 1;
+  int esize() const { return kind == F64 ? 8 : (kind == I8 ? 1 : 4); }
+};
+
+bool field_view(const DevParts &d, int field, FieldView &v) {
+  auto one = [&](FieldView::Kind k, const void *p) {
+    v.kind = k;
+    v.c[0] = p;
+    v.ncomp = 1;
+    return true;
+  };
+  auto three = [&](FieldView::Kind k, const void *a, const void *b,
+                   const void *c) {
+    v.kind = k;
+    v.c[0] = a;
+    v.c[1] = b;
+    v.c[2] = c;
+    v.ncomp = 3;
+    return true;
+  };
+  const auto F = FieldView::F32;
+  switch (field) {
+    case SWSPH_H: return one(F, d.h);
+    case SWSPH_RHO: return one(F, d.rho);
+    case SWSPH_RHO_DH: return one(F, d.rho_dh);
+    case SWSPH_WCOUNT: return one(F, d.wc);
+    case SWSPH_WCOUNT_DH: return one(F, d.wc_dh);
+    case SWSPH_DIV_V: return one(F, d.divv);
+    case SWSPH_ROT_V: return three(F, d.rotx, d.roty, d.rotz);
+    case SWSPH_PRESSURE: return one(F, d.P);
+    case SWSPH_SOUNDSPEED: return one(F, d.c);
+    case SWSPH_F: return one(F, d.f);
+    case SWSPH_BALSARA: return one(F, d.bal);
+    case SWSPH_V_SIG: return one(F, d.vsig);
+    case SWSPH_LAPLACE_U: return one(F, d.lapu);
+    case SWSPH_ALPHA_VISC_MAX_NGB: return one(F, d.avmax);
+    case SWSPH_ALPHA_VISC: return one(F, d.alpha);
+    case SWSPH_ALPHA_DIFF: return one(F, d.adiff);
+    case SWSPH_DIV_V_DT: return one(F, d.divv_dt);
+    case SWSPH_DIV_V_PREV: return one(F, d.divv_prev);
+    case SWSPH_A_HYDRO: return three(F, d.ax, d.ay, d.az);
+    case SWSPH_U_DT: return one(F, d.udt);
+    case SWSPH_H_DT: return one(F, d.hdt);
+    case SWSPH_N_DENSITY: return one(FieldView::I32, d.nd);
+    case SWSPH_N_GRADIENT: return one(FieldView::I32, d.ng);
+    case SWSPH_N_FORCE: return one(FieldView::I32, d.nf);
+    case SWSPH_MIN_NGB_TIME_BIN: return one(FieldView::I8, d.mintb);
+    case SWSPH_TIME_BIN: return one(FieldView::I8, d.tb);
+    case SWSPH_X: return three(FieldView::F64, d.px, d.py, d.pz);
+    case SWSPH_V: return three(F, d.vx, d.vy, d.vz);
+    case SWSPH_V_FULL: return three(F, d.vfx, d.vfy, d.vfz);
+    case SWSPH_U: return one(F, d.u);
+    case SWSPH_U_FULL: return one(F, d.ufull);
+    default: return false;
+  }
+}
 }  // namespace
 
 extern "C" {
@@ -1563,77 +1691,72 @@ int swsph_get_field(swsph_engine *e, int field, void *out) {
   if (!e || !out) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
   const int n = (int)e->n;
-  const DevParts &d = e->d;
+  FieldView v;
+  if (!field_view(e->d, field, v)) return fail(1, "unknown field %d", field);
   // Scattered into the caller's order on the device (through the staging
   // buffer, 24 B per particle), then one contiguous copy to the host.
-  auto fetch = [&](auto *const *src, int ncomp) -> int {
-    using T = typename std::remove_const<
-        typename std::remove_pointer<typename std::remove_pointer<
-            decltype(src)>::type>::type>::type;
+  auto fetch = [&](auto tag) -> int {
+    using T = decltype(tag);
     T *stage = (T *)e->d_stage;
-    for (int k = 0; k < ncomp; k++)
-      HIPCHK(sphk::launch_scatter(stage, (const T *)src[k], ncomp, k, d.perm,
-                                  n, e->st));
+    for (int k = 0; k < v.ncomp; k++)
+      HIPCHK(sphk::launch_scatter(stage, (const T *)v.c[k], v.ncomp, k,
+                                  e->d.perm, n, e->st));
     if (n)
-      HIPCHK(hipMemcpyAsync(out, stage, sizeof(T) * (size_t)ncomp * n,
+      HIPCHK(hipMemcpyAsync(out, stage, sizeof(T) * (size_t)v.ncomp * n,
                             hipMemcpyDeviceToHost, e->st));
     HIPCHK(hipStreamSynchronize(e->st));
     return 0;
   };
-  auto f1 = [&](float *p) {
-    float *const v[1] = {p};
-    return fetch(v, 1);
-  };
-  auto f3 = [&](float *a, float *b, float *c) {
-    float *const v[3] = {a, b, c};
-    return fetch(v, 3);
-  };
-  auto i1 = [&](int *p) {
-    int *const v[1] = {p};
-    return fetch(v, 1);
-  };
-  auto b1 = [&](int8_t *p) {
-    int8_t *const v[1] = {p};
-    return fetch(v, 1);
-  };
-  switch (field) {
-    case SWSPH_H: return f1(d.h);
-    case SWSPH_RHO: return f1(d.rho);
-    case SWSPH_RHO_DH: return f1(d.rho_dh);
-    case SWSPH_WCOUNT: return f1(d.wc);
-    case SWSPH_WCOUNT_DH: return f1(d.wc_dh);
-    case SWSPH_DIV_V: return f1(d.divv);
-    case SWSPH_ROT_V: return f3(d.rotx, d.roty, d.rotz);
-    case SWSPH_PRESSURE: return f1(d.P);
-    case SWSPH_SOUNDSPEED: return f1(d.c);
-    case SWSPH_F: return f1(d.f);
-    case SWSPH_BALSARA: return f1(d.bal);
-    case SWSPH_V_SIG: return f1(d.vsig);
-    case SWSPH_LAPLACE_U: return f1(d.lapu);
-    case SWSPH_ALPHA_VISC_MAX_NGB: return f1(d.avmax);
-    case SWSPH_ALPHA_VISC: return f1(d.alpha);
-    case SWSPH_ALPHA_DIFF: return f1(d.adiff);
-    case SWSPH_DIV_V_DT: return f1(d.divv_dt);
-    case SWSPH_DIV_V_PREV: return f1(d.divv_prev);
-    case SWSPH_A_HYDRO: return f3(d.ax, d.ay, d.az);
-    case SWSPH_U_DT: return f1(d.udt);
-    case SWSPH_H_DT: return f1(d.hdt);
-    case SWSPH_N_DENSITY: return i1(d.nd);
-    case SWSPH_N_GRADIENT: return i1(d.ng);
-    case SWSPH_N_FORCE: return i1(d.nf);
-    case SWSPH_MIN_NGB_TIME_BIN: return b1(d.mintb);
-    case SWSPH_TIME_BIN: return b1(d.tb);
-    case SWSPH_X: {
-      double *const v[3] = {d.px, d.py, d.pz};
-      return fetch(v, 3);
-    }
-    case SWSPH_V: return f3(d.vx, d.vy, d.vz);
-    case SWSPH_V_FULL: return f3(d.vfx, d.vfy, d.vfz);
-    case SWSPH_U: return f1(d.u);
-    case SWSPH_U_FULL: return f1(d.ufull);
-    default:
-      return fail(1, "unknown field %d", field);
+  switch (v.kind) {
+    case FieldView::F64: return fetch(double{});
+    case FieldView::F32: return fetch(float{});
+    case FieldView::I32: return fetch(int{});
+    default: return fetch(int8_t{});
   }
+}
+
+int swsph_cell_write_back(swsph_engine *e, int ci, const swsph_field_dst *dst,
+                          int nfields) {
+  if (!e || (nfields > 0 && !dst) || nfields < 0)
+    return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  int lo = 0, hi = (int)e->n;
+  if (ci != -1) {
+    if (int rc = cell_ok(e, ci)) return rc;
+    lo = e->cell_start[ci];
+    hi = e->cell_start[ci + 1];
+  }
+  std::vector<FieldView> views((size_t)nfields);
+  for (int f = 0; f < nfields; f++) {
+    if (!field_view(e->d, dst[f].field, views[f]))
+      return fail(1, "unknown field %d", dst[f].field);
+    const int64_t need = (int64_t)views[f].ncomp * views[f].esize();
+    if (!dst[f].base || std::llabs((long long)dst[f].stride) < need)
+      return fail(1, "destination of field %d: null base or stride %lld "
+                  "below its %lld bytes", dst[f].field,
+                  (long long)dst[f].stride, (long long)need);
+  }
+  // completion: everything enqueued so far has finished
+  HIPCHK(hipStreamSynchronize(e->st));
+  const size_t cnt = (size_t)(hi - lo);
+  if (cnt == 0 || nfields == 0) return 0;
+  std::vector<int> perm(cnt);
+  HIPCHK(hipMemcpy(perm.data(), e->d.perm + lo, sizeof(int) * cnt,
+                   hipMemcpyDeviceToHost));
+  std::vector<unsigned char> stage(cnt * 8);
+  for (int f = 0; f < nfields; f++) {
+    const FieldView &v = views[f];
+    const size_t es = (size_t)v.esize();
+    unsigned char *base = (unsigned char *)dst[f].base;
+    for (int k = 0; k < v.ncomp; k++) {
+      HIPCHK(hipMemcpy(stage.data(), (const unsigned char *)v.c[k] + es * lo,
+                       es * cnt, hipMemcpyDeviceToHost));
+      for (size_t s2 = 0; s2 < cnt; s2++)
+        std::memcpy(base + (int64_t)perm[s2] * dst[f].stride + es * k,
+                    stage.data() + es * s2, es);
+    }
+  }
+  return 0;
 }
 
 int swsph_num_cells(swsph_engine *e, int *ncells) {
@@ -1786,20 +1909,66 @@ int swsph_run_tasks(swsph_engine *e, int loop, const swsph_task *tasks,
 
 SELF_ENTRY(swsph_doself1_density, LoopKind::Density, false)
 TASK_ENTRY(swsph_dopair1_density, LoopKind::Density, false, true)
-SELF_ENTRY(swsph_doself_subset_density, LoopKind::Density, true)
-TASK_ENTRY(swsph_dopair_subset_density, LoopKind::Density, true, false)
 SELF_ENTRY(swsph_doself1_gradient, LoopKind::Gradient, false)
 TASK_ENTRY(swsph_dopair1_gradient, LoopKind::Gradient, false, true)
 SELF_ENTRY(swsph_doself2_force, LoopKind::Force, false)
 TASK_ENTRY(swsph_dopair2_force, LoopKind::Force, false, true)
 
-int swsph_do_ghost(swsph_engine *e, int ci) {
+// The density of a subset of cell ci's particles (caller indices
+// first + ind[k]) against cell cj (ci itself for the self form).
+int subset_task(swsph_engine *e, int ci, int cj, int64_t first,
+                const int *ind, int count) {
+  int rc;
+  if ((rc = cell_ok(e, ci)) || (rc = cell_ok(e, cj))) return rc;
+  if (count < 0 || (count > 0 && !ind) || first < 0)
+    return fail(1, "bad subset (first %lld, count %d)", (long long)first,
+                count);
+  for (int k = 0; k < count; k++)
+    if (ind[k] < 0 || first + ind[k] >= e->n)
+      return fail(1, "subset entry %d: particle %lld out of range [0, %lld)",
+                  k, (long long)(first + ind[k]), (long long)e->n);
+  if ((rc = ensure_subset(e, count))) return rc;
+  const int lo = e->cell_start[ci], hi = e->cell_start[ci + 1];
+  if (count > 0)
+    HIPCHK(hipMemcpyAsync(e->d_subind, ind, sizeof(int) * (size_t)count,
+                          hipMemcpyHostToDevice, e->st));
+  HIPCHK(sphk::launch_mark_subset(e->d_subflag, e->d_inv, (long long)first,
+                                  e->d_subind, count, lo, hi, e->d_suberr,
+                                  e->st));
+  e->launches += 3;
+  unsigned int bad = 0;
+  HIPCHK(hipMemcpyAsync(&bad, e->d_suberr, sizeof(bad), hipMemcpyDeviceToHost,
+                        e->st));
+  // (also the end of the host read of ind)
+  HIPCHK(hipStreamSynchronize(e->st));
+  if (bad)
+    return fail(1, "%u particles of the subset are not in cell %d", bad, ci);
+  return task(e, LoopKind::Density, true, ci, cj, false, e->d_subflag);
+}
+
+int swsph_doself_subset_density(swsph_engine *e, int ci, int64_t first,
+                                const int *ind, int count) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  return subset_task(e, ci, ci, first, ind, count);
+}
+
+int swsph_dopair_subset_density(swsph_engine *e, int ci, int64_t first,
+                                const int *ind, int count, int cj) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  return subset_task(e, ci, cj, first, ind, count);
+}
+
+int swsph_do_ghost(swsph_engine *e, int ci, int offset, int ntasks) {
   if (!e) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
   int rc = cell_ok(e, ci);
   if (rc) return rc;
+  if (ntasks < 1 || offset < 0 || offset >= ntasks)
+    return fail(1, "ghost share %d of %d", offset, ntasks);
   int rounds = 0;
-  return ghost_cell(e, ci, &rounds);
+  return ghost_cell(e, ci, &rounds, offset, ntasks);
 }
 
 int swsph_do_extra_ghost(swsph_engine *e, int ci) {

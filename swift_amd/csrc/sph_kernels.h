@@ -266,6 +266,18 @@ hipError_t launch_bundle(LoopKind kind, const DevParts &d, const LoopArgs &la,
 hipError_t launch_build_pos(const DevParts &d, const GridDesc &g, float pos_eps,
                             int lo, int hi, hipStream_t st);
 hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t st);
+// inv[perm[s]] = s over the n slots.
+hipError_t launch_invert_perm(const int *perm, int *inv, int n,
+                              hipStream_t st);
+// Subset flags of a task on the slots [lo, hi) of its cell: cleared, then
+// set for the caller indices first + ind[k] (k < count, on the device);
+// *err counts those whose slot is not in [lo, hi).
+hipError_t launch_mark_subset(int *flag, const int *inv, long long first,
+                              const int *ind, int count, int lo, int hi,
+                              unsigned int *err, hipStream_t st);
+// dst[i] = src[i] for i in [lo, hi) with (i - lo) % nt == off, else 0.
+hipError_t launch_share_flags(int *dst, const int *src, int lo, int hi,
+                              int off, int nt, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st);
 // Bounding boxes and largest search radius of chunks [c0, c1); the largest
 // search radius over them is max-reduced into *hg2_all (float bits).
@@ -280,10 +292,13 @@ hipError_t launch_init_density(const DevParts &d, int lo, int hi,
 // longer covers the new radius; [2]: converged particles whose list does
 // not cover their final radius (flagged in d.lfb); [3]: density
 // interactions evaluated for the processed particles, accumulated over the
-// rounds (the pass zeroes it).  [0..2] are zeroed by each launch.
+// rounds (the pass zeroes it).  [0..2] are zeroed by each launch.  Over a
+// range, only the slots i with (i - lo) % nt == off are processed (one of
+// nt ghost tasks of a cell, runner_do_ghost's offset / ntasks).
 hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,
                         unsigned long long *counters, hipStream_t st,
-                        const int *list = nullptr, const int *cnt = nullptr);
+                        const int *list = nullptr, const int *cnt = nullptr,
+                        int off = 0, int nt = 1);
 // The per-particle kernels below take either the particle range [lo, hi)
 // or (list != nullptr) the particles list[0 .. *cnt) of a compact list whose
 // capacity is hi - lo.

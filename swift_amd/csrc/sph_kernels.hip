@@ -360,8 +360,8 @@ __device__ bool ghost_one(const DevParts &d, const GhostArgs &ga, int i,
 // order.
 constexpr int kGhostBlock = 256;
 __global__ __launch_bounds__(kGhostBlock) void k_ghost(
-    DevParts d, GhostArgs ga, int lo, int hi, unsigned long long *counters,
-    const int *list, const int *cnt) {
+    DevParts d, GhostArgs ga, int lo, int hi, int off, int nt,
+    unsigned long long *counters, const int *list, const int *cnt) {
   constexpr int W = kGhostBlock / 64;
   __shared__ int wcnt[3][W];
   __shared__ unsigned long long wwork[W];
@@ -378,8 +378,9 @@ __global__ __launch_bounds__(kGhostBlock) void k_ghost(
       const int k = k0 + u * kGhostBlock;
       idx[u] = k < nl ? list[k] : -1;
     } else {
+      // share off of nt of the slots (runner_do_ghost's offset / ntasks)
       const int i = i0 + u * kGhostBlock;
-      idx[u] = i < hi ? i : -1;
+      idx[u] = (i < hi && (nt == 1 || (i - lo) % nt == off)) ? i : -1;
     }
   }
 #pragma unroll
@@ -528,6 +529,36 @@ __global__ void k_zero_int(int *p, int n) {
 __global__ void k_zero_u64(unsigned long long *p, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = 0ull;
+}
+
+// inv[perm[s]] = s: device slot of every caller index.
+__global__ void k_invert_perm(const int *perm, int *inv, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) inv[perm[s]] = s;
+}
+
+// flag[slot of caller index first + ind[k]] = 1 for k < count; a particle
+// whose slot lies outside [lo, hi) (not in the task's cell) is counted in
+// *err instead.  The caller checked first + ind[k] against the particle
+// count.
+__global__ void k_mark_subset(int *flag, const int *inv, long long first,
+                              const int *ind, int count, int lo, int hi,
+                              unsigned int *err) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= count) return;
+  const int s = inv[first + ind[k]];
+  if (s >= lo && s < hi)
+    flag[s] = 1;
+  else
+    atomicAdd(err, 1u);
+}
+
+// dst[i] = src[i] for the particles i of [lo, hi) with (i - lo) % nt == off,
+// 0 for the others: the share of a cell's ghost task `off` of `nt`.
+__global__ void k_share_flags(int *dst, const int *src, int lo, int hi,
+                              int off, int nt) {
+  const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < hi) dst[i] = ((i - lo) % nt == off) ? src[i] : 0;
 }
 
 // Sums the per-particle interaction counters of active particles:
@@ -1036,6 +1067,35 @@ hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st) {
   return hipGetLastError();
 }
 
+hipError_t launch_invert_perm(const int *perm, int *inv, int n,
+                              hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_invert_perm, dim3(nblk(n, 256)), dim3(256), 0, st, perm,
+                     inv, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_mark_subset(int *flag, const int *inv, long long first,
+                              const int *ind, int count, int lo, int hi,
+                              unsigned int *err, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, (int *)err, 1);
+  if (hi > lo)
+    hipLaunchKernelGGL(k_zero_int, dim3(nblk(hi - lo, 256)), dim3(256), 0, st,
+                       flag + lo, hi - lo);
+  if (count > 0)
+    hipLaunchKernelGGL(k_mark_subset, dim3(nblk(count, 256)), dim3(256), 0, st,
+                       flag, inv, first, ind, count, lo, hi, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_share_flags(int *dst, const int *src, int lo, int hi,
+                              int off, int nt, hipStream_t st) {
+  if (hi <= lo) return hipSuccess;
+  hipLaunchKernelGGL(k_share_flags, dim3(nblk(hi - lo, 256)), dim3(256), 0, st,
+                     dst, src, lo, hi, off, nt);
+  return hipGetLastError();
+}
+
 hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t st) {
   hipLaunchKernelGGL(k_zero_int, dim3(nblk(n, 64)), dim3(64), 0, st, (int *)p,
                      n);
@@ -1096,13 +1156,13 @@ hipError_t launch_init_density(const DevParts &d, int lo, int hi,
 
 hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,
                         unsigned long long *counters, hipStream_t st,
-                        const int *list, const int *cnt) {
+                        const int *list, const int *cnt, int off, int nt) {
   // counters[3] accumulates over the rounds of a pass (zeroed by the pass)
   hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, st, counters, 3);
   if (hi > lo)
     hipLaunchKernelGGL(k_ghost, dim3(run_blocks(hi - lo, kGhostBlock)),
-                       dim3(kGhostBlock), 0, st, d, ga, lo, hi, counters,
-                       list, cnt);
+                       dim3(kGhostBlock), 0, st, d, ga, lo, hi, off, nt,
+                       counters, list, cnt);
   return hipGetLastError();
 }
 

@@ -37,6 +37,10 @@ FIELDS = {
 PASS_FIELDS = [k for k, v in FIELDS.items() if v[0] < 25]
 
 
+class _FieldDst(C.Structure):
+    _fields_ = [("field", C.c_int), ("base", C.c_void_p), ("stride", C.c_int64)]
+
+
 class _TuneResult(C.Structure):
     _fields_ = [("schedule", C.c_int), ("chunk", C.c_int * 3),
                 ("loop_ms", C.c_double * 3), ("pass_ms", C.c_double),
@@ -174,6 +178,23 @@ class Engine:
               f"swsph_get_field({name})")
         return out
 
+    def write_back(self, ci: int, dst: dict) -> None:
+        """swsph_cell_write_back: waits for the engine and writes the fields
+        of cell ci's particles (ci = -1: all) into the caller's arrays,
+        dst[name] = an array indexed by the caller's particle index (rows of
+        a structured array work too: its field views carry the stride)."""
+        recs = (_FieldDst * max(1, len(dst)))()
+        for k, (name, arr) in enumerate(dst.items()):
+            fid, dt, w = FIELDS[name]
+            if arr.dtype != dt:
+                raise SwsphError(f"write_back({name}): dtype {arr.dtype}, "
+                                 f"want {np.dtype(dt)}")
+            recs[k].field = fid
+            recs[k].base = arr.ctypes.data
+            recs[k].stride = arr.strides[0]
+        check(lib().swsph_cell_write_back(self._h, ci, recs, len(dst)),
+              "swsph_cell_write_back")
+
     def results(self, names=None) -> dict:
         return {k: self.get(k) for k in (names or PASS_FIELDS)}
 
@@ -201,16 +222,26 @@ class Engine:
         check(lib().swsph_dopair1_density(self._h, ci, cj),
               "runner_dopair1_branch_density")
 
-    def runner_doself_subset_branch_density(self, ci: int) -> None:
-        check(lib().swsph_doself_subset_density(self._h, ci),
-              "runner_doself_subset_branch_density")
+    def runner_doself_subset_branch_density(self, ci: int, ind,
+                                            first: int = 0) -> None:
+        """Density of the caller's particles first + ind[k] (all in cell ci)
+        against ci (runner_doself_subset_branch_density(r, ci, parts, ind,
+        count) with parts = the caller's array + first)."""
+        ind = np.ascontiguousarray(ind, dtype=np.int32)
+        check(lib().swsph_doself_subset_density(
+            self._h, ci, C.c_int64(first), ind.ctypes.data_as(C.c_void_p),
+            int(ind.size)), "runner_doself_subset_branch_density")
 
-    def runner_dopair_subset_branch_density(self, ci: int, cj: int) -> None:
-        check(lib().swsph_dopair_subset_density(self._h, ci, cj),
-              "runner_dopair_subset_branch_density")
+    def runner_dopair_subset_branch_density(self, ci: int, ind, cj: int,
+                                            first: int = 0) -> None:
+        ind = np.ascontiguousarray(ind, dtype=np.int32)
+        check(lib().swsph_dopair_subset_density(
+            self._h, ci, C.c_int64(first), ind.ctypes.data_as(C.c_void_p),
+            int(ind.size), cj), "runner_dopair_subset_branch_density")
 
-    def runner_do_ghost(self, ci: int) -> None:
-        check(lib().swsph_do_ghost(self._h, ci), "runner_do_ghost")
+    def runner_do_ghost(self, ci: int, offset: int = 0, ntasks: int = 1) -> None:
+        check(lib().swsph_do_ghost(self._h, ci, offset, ntasks),
+              "runner_do_ghost")
 
     def runner_doself1_branch_gradient(self, ci: int) -> None:
         check(lib().swsph_doself1_gradient(self._h, ci),

@@ -49,13 +49,15 @@ def test_structs_match_header(tmp_path):
     import shutil
     import subprocess
     from swift_amd._lib import HEADER_PATH
-    from swift_amd.engine import _Props, _HostParts, _TuneResult, PassStats
+    from swift_amd.engine import (_FieldDst, _HostParts, _Props, _TuneResult,
+                                  PassStats)
     from swift_amd.server import _Info, _Opts
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
     structs = {"swsph_props": _Props, "swsph_host_parts": _HostParts,
                "swsph_pass_stats": PassStats, "swsph_tune_result": _TuneResult,
-               "swsph_server_opts": _Opts, "swsph_request_info": _Info}
+               "swsph_server_opts": _Opts, "swsph_request_info": _Info,
+               "swsph_field_dst": _FieldDst}
     lines = ['#include <stdio.h>', '#include <stddef.h>',
              f'#include "{HEADER_PATH}"', 'int main(void) {']
     for cname, py in structs.items():
@@ -74,3 +76,22 @@ def test_structs_match_header(tmp_path):
         assert int(out[cname]) == C.sizeof(py), cname
         for f, _ in py._fields_:
             assert int(out[f"{cname}.{f}"]) == getattr(py, f).offset, (cname, f)
+
+
+def test_c_abi_program_builds(tmp_path):
+    """The C-ABI test program (tests/c/abi_27cells.c) compiles with gcc
+    against include/swsph.h and links with the library and the oracle."""
+    import os
+    import shutil
+    import subprocess
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = tmp_path / "abi_27cells"
+    subprocess.run(["gcc", "-O2", "-std=c11", "-Wall", "-Wextra", "-Werror",
+                    "-o", str(exe), os.path.join(root, "tests", "c",
+                                                 "abi_27cells.c"),
+                    "-L" + os.path.join(root, "swift_amd"),
+                    "-L" + os.path.join(root, "oracle"), "-lswsph", "-lorc",
+                    "-lm"], check=True)
+    assert exe.exists()
