@@ -224,6 +224,22 @@ int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats);
  * or settings.  Results are identical to swsph_pass_batched.  Needs the
  * neighbour lists disabled.  No counterpart in the reference. */
 int swsph_pass_graph(swsph_engine *e, swsph_pass_stats *stats);
+/* swsph_pass_batched without any device allocation: the pass's scratch
+ * (the work queue: particle flags, scans, compact particle list, tile
+ * table and queue heads; the ghost and interaction counters) lives in one
+ * fixed device array of the library (about 30 MB per GPU, held from the
+ * first engine creation on that GPU until the process ends) instead of the
+ * engine's buffers.  Same kernels and launches, so the results are
+ * bit-identical to swsph_pass_batched.  Limits: at most 2^21 particles on
+ * at most 2^16 top-level cells (status 1 beyond them); neighbour lists and
+ * kernel timing must be off (status 1); a chunk table that would need
+ * device memory (first pass after swsph_set_particles grew the set, or a
+ * new tile shape) is status 1 -- run swsph_pass_batched once first.  All
+ * calls on one GPU share the array: calls from different engines that
+ * overlap in time on the same GPU race and must be serialised by the
+ * caller.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (serving loops, harnesses that forbid hipMalloc). */
+int swsph_pass_batched_noalloc(swsph_engine *e, swsph_pass_stats *stats);
 
 /* Device-to-device save / restore of the state a pass reads and rewrites
  * (h and the (io) fields), on the engine stream: the device-resident

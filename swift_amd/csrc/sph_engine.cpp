@@ -182,6 +182,11 @@ struct swsph_engine {
   static constexpr int kNoMab = -1000;
   int stale_mab = kNoMab;
   int last_pass_mab = kNoMab;
+  // Device the engine was created on, and set while a pass runs on the
+  // fixed scratch of swsph_pass_batched_noalloc: any path that would
+  // allocate device memory fails instead.
+  int device = 0;
+  bool no_alloc = false;
   // Subsets of the caller's particles (swsph_do*_subset_density): flags per
   // slot, the device slot of every caller index (valid for the binning
   // generation inv_gen), the uploaded indices, and an error counter;
@@ -374,6 +379,13 @@ swsph_engine::ChunkTable *chunk_table(swsph_engine *e, int C, int *rc) {
   start.push_back((int)e->n);
   // (re)allocate only when the capacity does not suffice
   if (t.nchunks > t.cap_chunks || e->ncells > t.cap_cells) {
+    if (e->no_alloc) {
+      t.valid = false;
+      *rc = fail(1, "the chunk table of capacity %d needs device memory: run "
+                 "swsph_pass_batched once with these particles and shapes "
+                 "before the allocation-free pass", C);
+      return nullptr;
+    }
     void *tp2[] = {t.d_start, t.d_cell_chunk, t.d_chunk_cell, t.d_bbox,
                    t.d_hg2max};
     for (void *q : tp2)
@@ -1498,6 +1510,16 @@ int swsph_engine_create(const swsph_props *props, swsph_engine **out) {
     e->grid.dim[k] = props->dim[k];
   }
   e->ncells = props->ncell[0] * props->ncell[1] * props->ncell[2];
+  if (hipGetDevice(&e->device) != hipSuccess) {
+    delete e;
+    return fail(2, "hipGetDevice failed");
+  }
+  // the fixed scratch of the allocation-free pass on this device
+  if (hipError_t err = sphk::noalloc_setup(e->device)) {
+    delete e;
+    return fail(2, "device address of the scratch array: %s",
+                hipGetErrorString(err));
+  }
   if (hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking) != hipSuccess) {
     delete e;
     return fail(2, "hipStreamCreate failed");
@@ -2050,6 +2072,44 @@ int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats) {
   if (!e) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
   return pass_batched_locked(e, stats);
+}
+
+int swsph_pass_batched_noalloc(swsph_engine *e, swsph_pass_stats *stats) {
+  if (!e) return fail(1, "null engine");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->n > kNoAllocMaxParts || e->ncells > kNoAllocMaxCells)
+    return fail(1, "the allocation-free pass takes at most %lld particles on "
+                "%d cells (%lld particles on %d cells given)",
+                (long long)kNoAllocMaxParts, kNoAllocMaxCells,
+                (long long)e->n, e->ncells);
+  if (e->use_lists)
+    return fail(1, "neighbour lists allocate their storage: disable them "
+                "(swsph_set_lists) for the allocation-free pass");
+  if (e->timing)
+    return fail(1, "kernel timing creates events: disable it "
+                "(swsph_set_timing) for the allocation-free pass");
+  TileScratch ts{};
+  size_t tile_cap = 0;
+  unsigned long long *redo = nullptr, *cnt = nullptr;
+  HIPCHK(sphk::noalloc_scratch(e->device, e->n, e->ncells, &ts, &tile_cap,
+                               &redo, &cnt));
+  // The pass runs on the scratch in place of the engine's own work queue
+  // and counters; the all-active tile table it leaves there is not the
+  // engine's (tiles_c = 0 before and after).
+  std::swap(e->ts, ts);
+  std::swap(e->tile_cap, tile_cap);
+  std::swap(e->d_redo, redo);
+  std::swap(e->d_cnt, cnt);
+  e->tiles_c = 0;
+  e->no_alloc = true;
+  const int rc = pass_batched_locked(e, stats);
+  e->no_alloc = false;
+  e->tiles_c = 0;
+  std::swap(e->ts, ts);
+  std::swap(e->tile_cap, tile_cap);
+  std::swap(e->d_redo, redo);
+  std::swap(e->d_cnt, cnt);
+  return rc;
 }
 
 int swsph_pass_graph(swsph_engine *e, swsph_pass_stats *stats) {

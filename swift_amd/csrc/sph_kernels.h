@@ -123,6 +123,10 @@ constexpr int kListChunk = 64 / kListStreams;
 // Smallest tile (i-particles per wave) of the work-queue schedules; the
 // tile tables are sized for it.
 constexpr int kMinTile = 4;
+// Largest particle and cell counts of the allocation-free batched pass,
+// whose per-pass scratch is a fixed device array (sph_scratch.hip).
+constexpr int64_t kNoAllocMaxParts = (int64_t)1 << 21;
+constexpr int kNoAllocMaxCells = 1 << 16;
 struct NbrList {
   unsigned int *idx;
   int *cnt;
@@ -363,4 +367,14 @@ hipError_t launch_kick(const DevParts &d, const KickArgs &a, hipStream_t st);
 hipError_t launch_timestep(const DevParts &d, const StepArgs &a,
                            unsigned long long *ti_end_min, unsigned int *err,
                            hipStream_t st);
+// The fixed scratch of the allocation-free pass (sph_scratch.hip): its size,
+// the one-time lookup of its address on `device` (which must be current),
+// and the work queue and counters of a pass over n particles on ncells
+// cells carved from it (hipErrorInvalidValue beyond the limits above,
+// hipErrorNotInitialized before noalloc_setup on that device).
+size_t noalloc_scratch_bytes();
+hipError_t noalloc_setup(int device);
+hipError_t noalloc_scratch(int device, int64_t n, int ncells, TileScratch *ts,
+                           size_t *tile_cap, unsigned long long **redo,
+                           unsigned long long **cnt);
 }  // namespace sphk

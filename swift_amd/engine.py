@@ -288,6 +288,14 @@ class Engine:
         check(lib().swsph_pass_batched(self._h, C.byref(s)), "swsph_pass_batched")
         return s.as_dict()
 
+    def pass_batched_noalloc(self) -> dict:
+        """The batched pass on the library's fixed device scratch: no device
+        allocation (at most 2^21 particles on 2^16 cells)."""
+        s = PassStats()
+        check(lib().swsph_pass_batched_noalloc(self._h, C.byref(s)),
+              "swsph_pass_batched_noalloc")
+        return s.as_dict()
+
     def autotune(self, repeats: int = 2) -> dict:
         """Time the schedules and chunk capacities on the current particles
         and apply the fastest (the pass state is restored afterwards)."""
