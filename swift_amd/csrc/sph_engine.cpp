@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -187,6 +188,9 @@ struct swsph_engine {
   // allocate device memory fails instead.
   int device = 0;
   bool no_alloc = false;
+  // Tiles of the all-active work queue per tile capacity, counted before a
+  // graph capture (the captured grids are sized by them).
+  std::map<int, int> cap_tiles;
   // Subsets of the caller's particles (swsph_do*_subset_density): flags per
   // slot, the device slot of every caller index (valid for the binning
   // generation inv_gen), the uploaded indices, and an error counter;
@@ -701,6 +705,7 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
     HIPCHK(sphk::launch_zero_u32((unsigned int *)e->ts.heads, 8 * 64, e->st));
   }
   int max_tiles = (int)(e->n / C) + e->ncells + 1;
+  bool persistent = e->schedule == 2, rest = false;
   if (!e->capturing && e->schedule == 1) {
     // Eager: one wave per tile of the device-side count (a grid for the
     // largest possible count would mostly be waves that exit at once).
@@ -708,14 +713,28 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
                           hipMemcpyDeviceToHost, e->st));
     HIPCHK(hipStreamSynchronize(e->st));
     max_tiles = *e->h_ntiles;
+  } else if (e->capturing && e->schedule == 1) {
+    // Captured: the all-active tiles are counted before the capture (the
+    // count holds for every replay of this configuration), with a small
+    // grid behind for any tiles beyond it; the re-run rounds, whose counts
+    // vary, are drained by the persistent grid.
+    if (mode == 0) {
+      const auto it = e->cap_tiles.find(C);
+      if (it != e->cap_tiles.end()) {
+        max_tiles = std::min(max_tiles, it->second);
+        rest = true;
+      }
+    } else {
+      persistent = true;
+    }
   }
   // the loop kernel alone is timed (the tile build is not)
   hipEvent_t a;
   if ((rc = time_begin(e, &a))) return rc;
-  if (max_tiles > 0)
+  if (max_tiles > 0 || rest)
     HIPCHK(sphk::launch_queue(k, build, e->d, la, chunk_args(t), tile_args(e),
-                              e->nl, C, CJ, e->schedule == 2, e->queue_waves,
-                              max_tiles, e->st));
+                              e->nl, C, CJ, persistent, e->queue_waves,
+                              max_tiles, rest, e->st));
   e->launches += 13;
   return time_end(e, k, a);
 }
@@ -2134,6 +2153,22 @@ int swsph_pass_graph(swsph_engine *e, swsph_pass_stats *stats) {
     // chunk tables must exist before capture (they allocate)
     for (int k = 0; k < 3; k++)
       if (!chunk_table(e, table_c(e, k), &rc)) return rc;
+    // the all-active tile count of each tile capacity in use
+    e->cap_tiles.clear();
+    if (e->schedule == 1) {
+      const LoopArgs la = loop_args(e);
+      for (int k = 0; k < 3; k++) {
+        const int C = e->loop_c[k];
+        if (e->cap_tiles.count(C)) continue;
+        HIPCHK(sphk::launch_build_tiles(e->d, la, (int)e->n, e->ncells, C, 0,
+                                        nullptr, e->ts, e->st));
+        HIPCHK(hipMemcpyAsync(e->h_ntiles, e->ts.ntiles, sizeof(int),
+                              hipMemcpyDeviceToHost, e->st));
+        HIPCHK(hipStreamSynchronize(e->st));
+        e->cap_tiles[C] = *e->h_ntiles;
+      }
+      e->tiles_c = 0;
+    }
     if ((rc = capture(e, &e->g_density, [&]() -> int {
            int r = phase_start(e);
            for (int k = 0; k < reruns && !r; k++) {
@@ -2432,17 +2467,16 @@ int swsph_autotune(swsph_engine *e, int repeats, swsph_tune_result *out) {
     }
   }
   if (!rc && best_s >= 1) {
-    // Under the work queue: candidate chunks of half the tile size, each
-    // loop keeping its faster shape.  (The narrow tiles are not tried: the
-    // trials time eager passes, which launch one wave per tile, while a
-    // captured graph launches a grid for the largest possible tile count,
-    // four times larger for tiles of 4 -- on the clustered box the narrow
-    // shapes won 11 % of loop time eagerly and lost 6 % per replayed step.
-    // engine_step, which runs eager passes, takes them by its own rule.)
+    // Under the work queue: candidate chunks of half the tile size, then
+    // the narrow tiles (4, 32) and (8, 16) (a few active particles per
+    // cell), each loop keeping its fastest shape.  The trials time eager
+    // passes (one wave per tile); a captured graph sizes its grids by the
+    // tile count of its configuration, so the narrow tiles cost no idle
+    // waves there either.
     e->schedule = best_s;
     e->queue_waves = 0;
     int best_ci[3] = {best_c[0], best_c[1], best_c[2]};
-    for (int trial = 0; trial < 1 && !rc; trial++) {
+    for (int trial = 0; trial < 3 && !rc; trial++) {
       bool any = false;
       for (int k = 0; k < 3; k++) {
         int ci = best_c[k], cj = best_c[k];

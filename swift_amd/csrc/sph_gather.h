@@ -165,10 +165,9 @@ struct GatherLds {
 
 // Interaction phase over the rows recorded in the lanes' masks.  Called from
 // converged code; the per-lane loop below is divergent, so everything it
-// shares is in LDS.  The density walk is software pipelined (the next
-// hit's record is loaded before the current hit is evaluated, so its latency
-// overlaps the interaction body); measured slower for the gradient loop's
-// larger records and register budget, so the others walk one hit at a time.
+// shares is in LDS.  Each lane walks its own hits one at a time (a
+// software-pipelined walk that holds the next record in a second register
+// set spilled registers in the density loop and was slower).
 template <class L, int CI, int CJ, bool BUILD>
 __device__ __forceinline__ void interact_rows(
     const typename L::Rec *__restrict__ rec, const GatherLds<CI, CJ> &sh,
@@ -194,11 +193,32 @@ __device__ __forceinline__ void interact_rows(
     return (int)(info & kRowMask) + (row % RPC) * S + s;
   };
   using Rec = typename L::Rec;
+  // The pair contract in the frame of the lane's current neighbour image
+  // cq: dx = a - (float)(x_j - b), with b = x_i and a = 0 in the i-cell
+  // itself (the FP64 difference rounded once; (float)(x_i - x_j) equals
+  // 0 - (float)(x_j - x_i) bit for bit) and b = oj, a = (float)(x_i - oi)
+  // across cells (pair_frame).  Hits come in neighbour order, so the
+  // frame changes a few times per walk and is kept in registers.
+  int cq = -1;
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  double bx = 0.0, by = 0.0, bz = 0.0;
   // Evaluates hit (j, q) with record r.
   auto eval = [&](int j, int q, const Rec &r) {
-    float dx, dy, dz;
-    const float r2 = pair_r2((selfm >> q) & 1u, st.x, st.y, st.z, r.x, r.y,
-                             r.z, &sh.fo[6 * q], dx, dy, dz);
+    if (q != cq) {
+      cq = q;
+      const bool self = (selfm >> q) & 1u;
+      const double *fo = &sh.fo[6 * q];
+      bx = self ? st.x : fo[3];
+      by = self ? st.y : fo[4];
+      bz = self ? st.z : fo[5];
+      ax = self ? 0.f : (float)(st.x - fo[0]);
+      ay = self ? 0.f : (float)(st.y - fo[1]);
+      az = self ? 0.f : (float)(st.z - fo[2]);
+    }
+    const float dx = ax - (float)(r.x - bx);
+    const float dy = ay - (float)(r.y - by);
+    const float dz = az - (float)(r.z - bz);
+    const float r2 = __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
     if (BUILD) {
       if (r2 < rt2) {
         if (ncnt < nl.Ks)
@@ -211,36 +231,10 @@ __device__ __forceinline__ void interact_rows(
       L::interact(r2, dx, dy, dz, st, r, acc);
     }
   };
-  if (!L::kPrefetchWalk) {
-    while (any()) {
-      int q;
-      const int j = locate(pop(), q);
-      eval(j, q, rec[j]);
-    }
-    return;
-  }
-  if (!any()) return;
-  // Pipelined: each step first takes the next row (if any) and loads its
-  // record into the other register set, then evaluates the current one.
-  // The loads are issued unconditionally (the current record again when no
-  // row is left), so the wait for the current record leaves the next one
-  // in flight; the loop is unrolled by two, so no copy waits for a load.
-  auto step = [&](int j, int q, const Rec &r, int &jn, int &qn,
-                  Rec &rn) -> bool {
-    const bool more = any();
-    const int row = pop();
-    int ql;
-    const int jl = locate(more ? row : 0, ql);
-    jn = more ? jl : j;
-    qn = more ? ql : q;
-    rn = rec[jn];
-    eval(j, q, r);
-    return more;
-  };
-  int ja, qa, jb = 0, qb = 0;
-  ja = locate(pop(), qa);
-  Rec ra = rec[ja], rb;
-  while (step(ja, qa, ra, jb, qb, rb) && step(jb, qb, rb, ja, qa, ra)) {
+  while (any()) {
+    int q;
+    const int j = locate(pop(), q);
+    eval(j, q, rec[j]);
   }
 }
 
@@ -330,18 +324,10 @@ __device__ __forceinline__ void run_chunk(
                          la.grid.dim[1] / la.grid.nc[1],
                          la.grid.dim[2] / la.grid.nc[2]};
     const double t[3] = {tx, ty, tz};
-    const int ci3[3] = {cx, cy, cz}, cj3[3] = {jx, jy, jz};
-    // stencil offset of j's cell (adjacent cells, wrapped)
-    int o[3];
+    const int cj3[3] = {jx, jy, jz};
     for (int k = 0; k < 3; k++) {
-      int v = cj3[k] - ci3[k];
-      v = v > 1 ? v - la.grid.nc[k] : (v < -1 ? v + la.grid.nc[k] : v);
-      o[k] = v;
       sh.qorg[3 * lane + k] = (double)cj3[k] * w[k] + t[k];
     }
-    double fo[6];
-    qself = pair_frame(o, ci3, cj3, t, w, fo);
-    for (int k = 0; k < 6; k++) sh.fo[6 * lane + k] = fo[k];
     const int k0 = ca.cell_chunk[cj], k1 = ca.cell_chunk[cj + 1];
     sh.qk0[lane] = k0;
     sh.qj0[lane] = la.cell_start[cj];
@@ -353,8 +339,6 @@ __device__ __forceinline__ void run_chunk(
     sh.ocode[lane] = stencil ? (unsigned int)kStencilOrder[lane] : 13u;
     nchq = k1 - k0;
   }
-  // neighbours that are the i-cell itself (self-pair contract), as bits
-  const unsigned int selfm = (unsigned int)uniform((int)(unsigned int)__ballot(qself));
   int incl = nchq;  // inclusive prefix over lanes 0..31
 #pragma unroll
   for (int off = 1; off < 32; off <<= 1) {
@@ -433,6 +417,30 @@ __device__ __forceinline__ void run_chunk(
     const float b = r * r * (1.f + 8.f * kU16) * (1.f + 1.f / 1024.f);
     rt2h = act ? (_Float16)fminf(b, 65504.f) : (_Float16)(-1.f);
   }
+  wave_sync();
+  // pair-contract frames of the neighbours (from the table in LDS)
+  if (lane < nq) {
+    int cx, cy, cz, jx, jy, jz;
+    cell_coords(cell_i, la.grid, cx, cy, cz);
+    cell_coords(sh.qcell[lane], la.grid, jx, jy, jz);
+    const int ci3[3] = {cx, cy, cz}, cj3[3] = {jx, jy, jz};
+    const double w[3] = {la.grid.dim[0] / la.grid.nc[0],
+                         la.grid.dim[1] / la.grid.nc[1],
+                         la.grid.dim[2] / la.grid.nc[2]};
+    const double t[3] = {sh.ttab[3 * lane], sh.ttab[3 * lane + 1],
+                         sh.ttab[3 * lane + 2]};
+    int o[3];
+    for (int k = 0; k < 3; k++) {
+      int v = cj3[k] - ci3[k];
+      v = v > 1 ? v - la.grid.nc[k] : (v < -1 ? v + la.grid.nc[k] : v);
+      o[k] = v;
+    }
+    double fo[6];
+    qself = pair_frame(o, ci3, cj3, t, w, fo);
+    for (int k = 0; k < 6; k++) sh.fo[6 * lane + k] = fo[k];
+  }
+  const unsigned int selfm =
+      (unsigned int)uniform((int)(unsigned int)__ballot(qself));
   wave_sync();
 
   unsigned long long m0 = 0ull, m1 = 0ull;  // row hits of this lane
@@ -665,6 +673,28 @@ __global__ SWSPH_TILE_BOUNDS(L) void k_loop_tiles(
   if (t >= *tq.ntiles) return;  // uniform per wave
   run_chunk<L, CI, CJ, false, BUILD, true>(d, la, ca, rec, pos, 0, true, 0, 0.0,
                                            0.0, 0.0, nl, &tq, t);
+}
+
+// The tiles that a grid of b0 blocks of k_loop_tiles does not reach (a
+// grid sized for an earlier tile count, as in a captured graph): blocks
+// b0, b0 + 1, ... of the same block -> tile mapping, visited in strides of
+// this kernel's grid.  Normally there are none and every wave exits at
+// once; the loop around the tile costs registers, so the main grid runs
+// one tile per wave.
+template <class L, int CI, int CJ, bool BUILD>
+__global__ __launch_bounds__(64) void k_loop_tiles_rest(
+    DevParts d, LoopArgs la, ChunkArgs ca,
+    const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
+    NbrList nl, TileArgs tq, int b0) {
+  const int n = *tq.ntiles;
+  const int nb = xcd_blocks(n);
+  for (int b = b0 + (int)blockIdx.x; b < nb; b += (int)gridDim.x) {
+    const int t = xcd_item(b, threadIdx.x >> 6, n);
+    if (t >= n) continue;  // uniform per wave
+    wave_sync();
+    run_chunk<L, CI, CJ, false, BUILD, true>(d, la, ca, rec, pos, 0, true, 0,
+                                             0.0, 0.0, 0.0, nl, &tq, t);
+  }
 }
 
 // List loop: lane g + kListChunk * s walks stream s of the neighbour list of

@@ -252,12 +252,14 @@ hipError_t launch_build_tiles(const DevParts &d, const LoopArgs &la, int n,
 // A neighbour loop over the tiles (of <= CI particles) against candidate
 // chunks of CJ particles (the table `ca`): by a persistent grid of `waves`
 // waves (0: full occupancy on every CU) when `persistent`, else by one wave
-// per tile over a grid for `max_tiles` tiles.  Shapes: CJ = CI, or
-// CJ = CI / 2 for CI = 16, 32.
+// per tile over a grid for `max_tiles` tiles, and with `rest` a small grid
+// that takes the tiles beyond the first max_tiles (a tile count that grew
+// past the grid a captured graph was sized for).  Shapes: CJ = CI, or
+// CJ = CI / 2 for CI = 16, 32, and the narrow tiles.
 hipError_t launch_queue(LoopKind kind, bool build_lists, const DevParts &d,
                         const LoopArgs &la, const ChunkArgs &ca,
                         const TileArgs &ta, const NbrList &nl, int CI, int CJ,
-                        bool persistent, int waves, int max_tiles,
+                        bool persistent, int waves, int max_tiles, bool rest,
                         hipStream_t st);
 hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
                        const LoopArgs &la, const ChunkArgs &ca,

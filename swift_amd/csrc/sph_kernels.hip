@@ -773,6 +773,9 @@ __global__ void k_tile_fill(const int *cell_start, const int *S, const int *T,
 namespace sphk {
 
 static inline int nblk(int n, int b) { return (n + b - 1) / b; }
+// Blocks (one wave each) of the grid that takes the tiles beyond a captured
+// grid's reach.
+constexpr int kRestBlocks = 2048;
 
 template <class L, bool SUB, int C>
 void stencil_c(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
@@ -889,11 +892,17 @@ template <class L, int CI, int CJ, bool B>
 hipError_t queue_c(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
                    const TileArgs &ta, const NbrList &nl,
                    const typename L::Rec *rec, bool persistent, int waves,
-                   int max_tiles, hipStream_t st) {
+                   int max_tiles, bool rest, hipStream_t st) {
   if (!persistent) {
-    hipLaunchKernelGGL((k_loop_tiles<L, CI, CJ, B>),
-                       dim3(xcd_blocks(max_tiles)), dim3(64 * kWaves), 0, st,
-                       d, la, ca, rec, (const PosRec *)d.pos, nl, ta);
+    const int nb = max_tiles > 0 ? xcd_blocks(max_tiles) : 0;
+    if (nb > 0)
+      hipLaunchKernelGGL((k_loop_tiles<L, CI, CJ, B>), dim3(nb),
+                         dim3(64 * kWaves), 0, st, d, la, ca, rec,
+                         (const PosRec *)d.pos, nl, ta);
+    if (rest)
+      hipLaunchKernelGGL((k_loop_tiles_rest<L, CI, CJ, B>), dim3(kRestBlocks),
+                         dim3(64), 0, st, d, la, ca, rec,
+                         (const PosRec *)d.pos, nl, ta, nb);
     return hipGetLastError();
   }
   auto kern = k_loop_queue<L, CI, CJ, B>;
@@ -914,18 +923,18 @@ template <class L>
 hipError_t queue_any(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
                      const TileArgs &ta, const NbrList &nl,
                      const typename L::Rec *rec, int CI, int CJ, bool p,
-                     int waves, int mt, hipStream_t st) {
+                     int waves, int mt, bool rest, hipStream_t st) {
   switch (CI * 100 + CJ) {
-    case 808: return queue_c<L, 8, 8, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
-    case 1616: return queue_c<L, 16, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
-    case 1608: return queue_c<L, 16, 8, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
-    case 3232: return queue_c<L, 32, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
-    case 3216: return queue_c<L, 32, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
-    case 6464: return queue_c<L, 64, 64, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
-    case 416: return queue_c<L, 4, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
-    case 432: return queue_c<L, 4, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
-    case 816: return queue_c<L, 8, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
-    case 1632: return queue_c<L, 16, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, st);
+    case 808: return queue_c<L, 8, 8, false>(d, la, ca, ta, nl, rec, p, waves, mt, rest, st);
+    case 1616: return queue_c<L, 16, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, rest, st);
+    case 1608: return queue_c<L, 16, 8, false>(d, la, ca, ta, nl, rec, p, waves, mt, rest, st);
+    case 3232: return queue_c<L, 32, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, rest, st);
+    case 3216: return queue_c<L, 32, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, rest, st);
+    case 6464: return queue_c<L, 64, 64, false>(d, la, ca, ta, nl, rec, p, waves, mt, rest, st);
+    case 416: return queue_c<L, 4, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, rest, st);
+    case 432: return queue_c<L, 4, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, rest, st);
+    case 816: return queue_c<L, 8, 16, false>(d, la, ca, ta, nl, rec, p, waves, mt, rest, st);
+    case 1632: return queue_c<L, 16, 32, false>(d, la, ca, ta, nl, rec, p, waves, mt, rest, st);
   }
   return hipErrorInvalidValue;
 }
@@ -933,7 +942,7 @@ hipError_t queue_any(const DevParts &d, const LoopArgs &la, const ChunkArgs &ca,
 hipError_t launch_queue(LoopKind kind, bool build_lists, const DevParts &d,
                         const LoopArgs &la, const ChunkArgs &ca,
                         const TileArgs &ta, const NbrList &nl, int CI, int CJ,
-                        bool persistent, int waves, int max_tiles,
+                        bool persistent, int waves, int max_tiles, bool rest,
                         hipStream_t st) {
   if (ca.C != CJ) return hipErrorInvalidValue;
   switch (kind) {
@@ -941,16 +950,19 @@ hipError_t launch_queue(LoopKind kind, bool build_lists, const DevParts &d,
       if (build_lists) {
         if (CI != kListChunk || CJ != kListChunk) return hipErrorInvalidValue;
         return queue_c<DensityLoop, kListChunk, kListChunk, true>(
-            d, la, ca, ta, nl, d.rd, persistent, waves, max_tiles, st);
+            d, la, ca, ta, nl, d.rd, persistent, waves, max_tiles, rest, st);
       }
       return queue_any<DensityLoop>(d, la, ca, ta, nl, d.rd, CI, CJ,
-                                    persistent, waves, max_tiles, st);
+                                    persistent, waves, max_tiles, rest,
+                                    st);
     case LoopKind::Gradient:
       return queue_any<GradientLoop>(d, la, ca, ta, nl, d.rg, CI, CJ,
-                                     persistent, waves, max_tiles, st);
+                                     persistent, waves, max_tiles, rest,
+                                    st);
     case LoopKind::Force:
       return queue_any<ForceLoop>(d, la, ca, ta, nl, d.rf, CI, CJ,
-                                  persistent, waves, max_tiles, st);
+                                  persistent, waves, max_tiles, rest,
+                                    st);
   }
   return hipErrorInvalidValue;
 }

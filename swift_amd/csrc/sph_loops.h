@@ -156,7 +156,6 @@ struct DensityLoop {
     d.nd[i] = a.n;
   }
   static constexpr bool kSymmetric = false;  // search radius is H_i only
-  static constexpr bool kPrefetchWalk = true;  // pipelined hit walk
   static __device__ __forceinline__ void identity(Acc &a) {
     a.rho = a.rho_dh = a.wc = a.wc_dh = a.divv = a.rx = a.ry = a.rz = 0.f;
     a.n = 0;
@@ -250,7 +249,6 @@ struct GradientLoop {
     d.ng[i] = a.n;
   }
   static constexpr bool kSymmetric = false;
-  static constexpr bool kPrefetchWalk = false;
   // v_sig >= 0 and alpha >= 0, so 0 is the identity of both maxima.
   static __device__ __forceinline__ void identity(Acc &a) {
     a.vsig = 0.f;
@@ -349,7 +347,6 @@ struct ForceLoop {
     d.mintb[i] = (int8_t)a.mintb;
   }
   static constexpr bool kSymmetric = true;  // r < max(H_i, H_j)
-  static constexpr bool kPrefetchWalk = false;
   static __device__ __forceinline__ void identity(Acc &a) {
     a.ax = a.ay = a.az = a.udt = a.hdt = 0.f;
     a.n = 0;
