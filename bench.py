@@ -66,16 +66,21 @@ def _pct(a, q):
     return float(np.percentile(np.asarray(a), q))
 
 
-def _profile_values(path, key):
-    """`key: name = value` lines of a committed rocprofv3 summary."""
+def _profile_values(key):
+    """`key: name = value` lines of the committed rocprofv3 summaries in
+    profiles/ (the figures a summary reports for a kernel and workload)."""
     vals = {}
-    if not os.path.exists(path):
+    pdir = os.path.join(ROOT, "profiles")
+    if not os.path.isdir(pdir):
         return vals
-    for line in open(path):
-        line = line.strip()
-        if line.startswith(key + ":"):
-            k, v = line[len(key) + 1:].split("=")
-            vals[k.strip()] = float(v)
+    for fn in sorted(os.listdir(pdir)):
+        if not fn.endswith((".md", ".txt")):
+            continue
+        for line in open(os.path.join(pdir, fn)):
+            line = line.strip()
+            if line.startswith(key + ":"):
+                k, v = line[len(key) + 1:].split("=")
+                vals[k.strip()] = float(v)
     return vals
 
 
@@ -116,7 +121,7 @@ def modulated_state(n_side: int, ncell: int):
 
 
 def measure(props, st, workload: str, steps: int, warmup: int, tune: bool,
-            graph: bool, plain: bool, traffic_key: str) -> dict:
+            graph: bool, plain: bool, traffic_key: str, shape=None) -> dict:
     """Times `steps` passes (after `warmup`), then the per-loop kernel times
     of as many eager passes with HIP events on the engine stream, then one
     plain per-task pass.  Returns the measurement and the roofline of the
@@ -126,7 +131,11 @@ def measure(props, st, workload: str, steps: int, warmup: int, tune: bool,
     eng.set_particles(st)
     eng.save_state()
     tuning = None
-    if tune:
+    if shape:
+        # a fixed tile shape for all three loops (profiling runs)
+        for loop in ("density", "gradient", "force"):
+            eng.set_tile_shape(loop, *shape)
+    elif tune:
         tuning = eng.autotune(repeats=2)
         print(f"bench: autotuned {tuning}", file=sys.stderr, flush=True)
     run = eng.pass_graph if graph else eng.pass_batched
@@ -189,11 +198,10 @@ def measure(props, st, workload: str, steps: int, warmup: int, tune: bool,
     name = LOOPS[dom]
     avg_ms = kernels[name]["avg_ms"]
     achieved = kernels[name]["tflops"]
-    prof = _profile_values(os.path.join(ROOT, "profiles", "pmc_traffic.txt"),
-                           f"{name}_{traffic_key}")
+    shape = "x".join(str(c) for c in eng.get_tile_shape(name.split("_")[0]))
+    prof = _profile_values(f"{name}_{shape}_{traffic_key}")
     traffic = prof.get("hbm_bytes_per_launch")
-    valu = _profile_values(os.path.join(ROOT, "profiles", "pmc_valu.md"),
-                           f"{name}_{traffic_key}")
+    valu = _profile_values(f"{name}_{shape}_{traffic_key}")
     roofline = {"bound": "valu", "kernel": name, "achieved": achieved,
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
@@ -500,6 +508,9 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0,
                     help="host threads of the CPU baseline (0: OMP_NUM_THREADS "
                          "if set, else every core this process may run on)")
+    ap.add_argument("--shape", default="",
+                    help="CI:CJ tile shape of all loops instead of the "
+                         "autotuner's choice (e.g. 4:32)")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default schedule and tile shapes")
     ap.add_argument("--graph", type=int, default=1,
@@ -524,10 +535,11 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
+    shape = tuple(int(c) for c in args.shape.split(":")) if args.shape else None
     props, st, workload = clustered_state(args.n, args.ncell)
     head = measure(props, st, workload, args.steps, args.warmup,
                    not args.no_tune, bool(args.graph), not args.no_plain,
-                   f"clustered{args.n}")
+                   f"clustered{args.n}", shape)
     out = {
         "metric": "SPH pair interactions/s (density+force) batched vs plain on GPU; p50/p99 step ms",
         "value": head["value"],
@@ -569,7 +581,7 @@ def main() -> int:
         out["modulated"] = measure(mp, ms_, mw, args.steps, args.warmup,
                                    not args.no_tune, bool(args.graph),
                                    not args.no_plain,
-                                   f"modulated{args.modulated_n}")
+                                   f"modulated{args.modulated_n}", shape)
         del ms_
         print(f"bench: modulated {out['modulated']['ms_per_step']:.2f} ms/step",
               file=sys.stderr, flush=True)
