@@ -414,13 +414,14 @@ def _hbm_used_bytes() -> int:
 
 
 def evolve_bench(n_side: int, ncell: int, nsteps: int, seed: int = 0,
-                 chunk: int = 0) -> dict:
+                 chunk: int = 0, sched: bool = False) -> dict:
     """Multi-step run of the clustered construction: forces at t = 0 for all
     particles (the initial h estimate converged by the ghost), then `nsteps`
     drift-kick steps on a fixed 4-level time-bin hierarchy (bins 1..4 hold
     1.5 / 4.5 / 19 / 75 % of the particles, so 1.5, 6, 25 or 100 % are
     active), state resident in HBM, re-binned on the device when particles
-    leave the stencil."""
+    leave the stencil (a bulk flow of (150, 80, -40) carries them across
+    cell faces)."""
     from swift_amd import Engine, HydroProps, ics
     from swift_amd.engine import integration
     N = n_side ** 3
@@ -429,10 +430,14 @@ def evolve_bench(n_side: int, ncell: int, nsteps: int, seed: int = 0,
                            a_max=0.02)
     ics.set_bin_hierarchy(st, {1: 0.015, 2: 0.045, 3: 0.19, 4: 0.75},
                           seed=seed + 3)
+    # a bulk flow carries the particles out of their cells' stencils within
+    # the run, so the device re-binning runs at this size too
+    st["v"] += np.asarray((150.0, 80.0, -40.0), np.float32)
     t_ics = time.perf_counter() - t0
     print(f"bench: evolve box built ({t_ics:.1f} s)", file=sys.stderr, flush=True)
-    # time line: the longest step (bin 4, 32 units) is 3.2e-8
-    props = HydroProps(ncell=(ncell,) * 3, time_base=1e-9)
+    # time line: bin 1 steps are 4 units of 3e-7; over 20 steps the bulk
+    # flow moves every particle by ~0.004, a third of a 1/96 cell
+    props = HydroProps(ncell=(ncell,) * 3, time_base=3e-7)
     eng = Engine(props)
     ts = time.perf_counter()
     eng.set_particles(st)
@@ -440,6 +445,9 @@ def evolve_bench(n_side: int, ncell: int, nsteps: int, seed: int = 0,
     t_up = time.perf_counter() - ts
     del st
     eng.set_integration(integration(fixed_bins=True))
+    # passes through the reference's task graph, released to the device
+    # as bundles, instead of the device work queue
+    eng.set_step_sched(sched)
     if chunk:
         for loop in ("density", "gradient", "force"):
             eng.set_chunk(loop, chunk)
@@ -475,7 +483,10 @@ def evolve_bench(n_side: int, ncell: int, nsteps: int, seed: int = 0,
                     "spheres + 30 % uniform, u log-uniform [0.1, 100], "
                     f"{ncell}^3 top-level cells, {nsteps} drift-kick steps on "
                     "a fixed 4-level time-bin hierarchy (active 100/25/6/1.5 %),"
-                    " state resident in HBM",
+                    " state resident in HBM" + (
+                        ", passes through the hydro task graph released as "
+                        "bundles" if sched else ", passes through the device "
+                        "work queue"),
         "particles": N, "steps": nsteps,
         "value": inter / wall, "unit": "interactions/s",
         "ms_per_step": 1e3 * wall / nsteps,
@@ -518,9 +529,15 @@ def main() -> int:
     ap.add_argument("--serving-s", type=float, default=5.0,
                     help="seconds of the serving stream (0: skip)")
     ap.add_argument("--serving-rate", type=float, default=200.0)
-    ap.add_argument("--evolve-n", type=int, default=0,
+    ap.add_argument("--serving-sweep", default="100,300,400",
+                    help="offered loads (req/s) of the latency sweep")
+    ap.add_argument("--serving-sweep-s", type=float, default=3.0)
+    ap.add_argument("--evolve-n", type=int, default=512,
                     help="particles per axis of the multi-step clustered run "
                          "(0: skip; 512 is the full-size case)")
+    ap.add_argument("--evolve-sched-steps", type=int, default=0,
+                    help="steps of the same run with the passes driven by the "
+                         "task graph (0: skip)")
     ap.add_argument("--evolve-ncell", type=int, default=96)
     ap.add_argument("--evolve-steps", type=int, default=20)
     ap.add_argument("--evolve-chunk", type=int, default=0,
@@ -591,11 +608,28 @@ def main() -> int:
                                      args.evolve_steps, chunk=args.evolve_chunk)
         print(f"bench: evolve {out['evolve']['ms_per_step']:.1f} ms/step",
               file=sys.stderr, flush=True)
+        if args.evolve_sched_steps > 0:
+            out["evolve_sched"] = evolve_bench(
+                args.evolve_n, args.evolve_ncell, args.evolve_sched_steps,
+                chunk=args.evolve_chunk, sched=True)
+            print(f"bench: evolve (task graph) "
+                  f"{out['evolve_sched']['ms_per_step']:.1f} ms/step",
+                  file=sys.stderr, flush=True)
 
     if args.serving_s > 0:
         out["serving"] = serving_bench(args.serving_s, args.serving_rate)
         print(f"bench: serving {out['serving']['requests']} requests",
               file=sys.stderr, flush=True)
+        # latency against offered load
+        sweep = []
+        for rate in [float(r) for r in args.serving_sweep.split(",") if r]:
+            sv = serving_bench(args.serving_sweep_s, rate, seed=1)
+            sweep.append({k: sv[k] for k in (
+                "requests", "p50_latency_ms", "p99_latency_ms",
+                "requests_per_s", "interactions_per_s")} | {"offered_per_s": rate})
+            print(f"bench: serving at {rate:g} req/s: p99 "
+                  f"{sv['p99_latency_ms']:.1f} ms", file=sys.stderr, flush=True)
+        out["serving"]["load_sweep"] = sweep
     print(json.dumps(out))
     return 0
 

@@ -18,7 +18,8 @@
  *
  * Scope: SPHENIX, Wendland C2, 3D, gamma = 5/3, no cosmology (a = 1, H = 0),
  * periodic box with >= 3 top-level cells per axis and kernel support
- * H = gamma_K h <= top-level cell width.
+ * H = gamma_K h <= top-level cell width (the whole-pass forms regrid when
+ * it grows past it, see swsph_set_regrid).
  */
 #ifndef SWSPH_API_H_
 #define SWSPH_API_H_
@@ -102,6 +103,8 @@ typedef struct swsph_pass_stats {
   int64_t list_fallback;         /* batched: gradient via the chunk loop */
   int64_t n_density_all;         /* batched: density interactions evaluated
                                     over all ghost rounds (work done) */
+  int64_t regrids;               /* whole passes: top-level regrids the pass
+                                    needed (kernel support outgrew a cell) */
 } swsph_pass_stats;
 
 const char *swsph_last_error(void);
@@ -117,6 +120,11 @@ int swsph_set_props(swsph_engine *e, const swsph_props *props);
  * sort for this path.  Starts a new time line for swsph_engine_step. */
 int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p);
 int swsph_get_field(swsph_engine *e, int field, void *host_out);
+/* nf fields at once (as swsph_get_field each), with one wait at the end:
+ * with page-locked outputs (hipHostMalloc) the copies are asynchronous
+ * DMA behind each other on the engine's stream. */
+int swsph_get_fields(swsph_engine *e, int nf, const int *fields,
+                     void *const *outs);
 int swsph_num_cells(swsph_engine *e, int *ncells);
 int swsph_cell_count(swsph_engine *e, int cid, int *count);
 int swsph_sync(swsph_engine *e);
@@ -241,6 +249,20 @@ int swsph_pass_graph(swsph_engine *e, swsph_pass_stats *stats);
  * allocate device memory (serving loops, harnesses that forbid hipMalloc). */
 int swsph_pass_batched_noalloc(swsph_engine *e, swsph_pass_stats *stats);
 
+/* Kernel support larger than a top-level cell.  The whole-pass forms
+ * (swsph_pass_batched, swsph_pass_graph, swsph_sched_pass,
+ * swsph_engine_step) then regrid as space_regrid does
+ * (src/space_regrid.c:118-126): cdim = floor(dim / (H_max * 1.10)) per
+ * axis (space_stretch, src/space.h:74), never finer than before, the
+ * particles re-binned on the device and the pass repeated from the
+ * smoothing lengths found so far; fewer than 3 cells per axis is status 4.
+ * Cell indices then refer to the new grid (swsph_num_cells).  The per-task
+ * entry points, task bundles and the allocation-free pass do not regrid:
+ * their caller owns the cell indices, and they return status 4.  on = 0
+ * turns regridding off (status 4 everywhere); default on. */
+int swsph_set_regrid(swsph_engine *e, int on);
+int swsph_get_grid(swsph_engine *e, int ncell[3]);
+
 /* Device-to-device save / restore of the state a pass reads and rewrites
  * (h and the (io) fields), on the engine stream: the device-resident
  * equivalent of restart_write_blocks / restart_read for this path
@@ -286,6 +308,48 @@ typedef struct swsph_task {
  * swsph_do_ghost -> gradient bundle -> ...).  Status 0 on success. */
 int swsph_run_tasks(swsph_engine *e, int loop, const swsph_task *tasks,
                     int ntasks);
+/* Bundles of cell tasks: runner_do_ghost, runner_do_extra_ghost,
+ * runner_do_end_hydro_force and the hydro_init_part of the drift/init
+ * (src/runner_ghost.c:1113, 1016; src/runner_others.c:815;
+ * src/hydro/SPHENIX/hydro.h:564) on each of the n listed cells, released
+ * together by the scheduler: one set of device launches over the cells'
+ * particles (the ghost in device-wide rounds with re-runs over each
+ * particle's 27-cell stencil).  *rounds (if not null) receives the largest
+ * number of ghost rounds so far.  Status 3 if a smoothing length does not
+ * converge, as swsph_do_ghost. */
+int swsph_init_cells(swsph_engine *e, const int *cells, int n);
+int swsph_do_ghost_cells(swsph_engine *e, const int *cells, int n,
+                         int *rounds);
+int swsph_do_extra_ghost_cells(swsph_engine *e, const int *cells, int n);
+int swsph_do_end_hydro_force_cells(swsph_engine *e, const int *cells, int n);
+/* act[c] = 1 for the cells holding an active particle (cell_is_active_-
+ * hydro, src/active.h), else 0; act has swsph_num_cells entries. */
+int swsph_cell_active(swsph_engine *e, uint8_t *act);
+
+/* One density -> ghost -> gradient -> extra ghost -> force -> end force
+ * pass driven by the reference's task graph instead of the device work
+ * queue: the hydro tasks of the cells with active particles and their
+ * dependencies (engine_make_hydroloop_tasks_mapper and engine_make_hydro_-
+ * loops_dependencies, src/engine_maketasks.c:3501, 2546-2573; pair tasks
+ * active when either cell is, as engine_marktasks), run by releasing each
+ * set of ready tasks as one bundle per task type (scheduler_done,
+ * src/scheduler.c:1203-1222) through swsph_run_tasks and the bundles of
+ * cell tasks above.  Same interactions as swsph_pass_batched (FP32 sums in
+ * another order). */
+typedef struct swsph_sched_stats {
+  int64_t active_cells;  /* cells with active particles */
+  int64_t tasks;         /* tasks in the graph */
+  int64_t bundles;       /* bundles submitted */
+  int levels;            /* release rounds */
+  double wall_ms;        /* wall time of the pass (host and device) */
+} swsph_sched_stats;
+int swsph_sched_pass(swsph_engine *e, swsph_sched_stats *sched,
+                     swsph_pass_stats *stats);
+/* 1: swsph_engine_step runs its pass through swsph_sched_pass's task graph
+ * (tasks streamed from the scheduler); 0 (default): through the device
+ * work queue. */
+int swsph_set_step_sched(swsph_engine *e, int on);
+
 /* Tile shape of the passes of swsph_engine_step when its active particles
  * number fewer than per_cell per top-level cell on average (work-queue
  * schedules only): all three loops then use tiles of CI against chunks of

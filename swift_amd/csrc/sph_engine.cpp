@@ -22,11 +22,11 @@
 #include <mutex>
 #include <string>
 #include <type_traits>
-#include <unordered_map>
 #include <vector>
 
 #include "../../include/swsph.h"
 #include "sph_kernels.h"
+#include "sph_sched.h"
 
 namespace {
 
@@ -191,6 +191,20 @@ struct swsph_engine {
   // Tiles of the all-active work queue per tile capacity, counted before a
   // graph capture (the captured grids are sized by them).
   std::map<int, int> cap_tiles;
+  // Bundles of cell tasks (swsph_*_cells): the cells' mask on the device,
+  // the uploaded cell list, and whether the mask is in force; the cells
+  // holding active particles (swsph_sched_pass).
+  unsigned char *d_cmask = nullptr, *d_cact = nullptr;
+  int *d_clist = nullptr;
+  int cmask_cap = 0;
+  bool cmask_on = false;
+  // swsph_engine_step runs its pass through the task graph (1) or the work
+  // queue (0)
+  int step_sched = 0;
+  // Whole passes regrid the top level when the kernel support outgrows a
+  // cell (swsph_set_regrid); regrids done since creation.
+  bool auto_regrid = true;
+  int64_t regrids = 0;
   // Subsets of the caller's particles (swsph_do*_subset_density): flags per
   // slot, the device slot of every caller index (valid for the binning
   // generation inv_gen), the uploaded indices, and an error counter;
@@ -262,6 +276,7 @@ LoopArgs loop_args(const swsph_engine *e) {
   la.flag = e->d.redo;
   la.max_active_bin = e->props.max_active_bin;
   la.pos_eps = pos_eps(e);
+  la.cmask = e->cmask_on ? e->d_cmask : nullptr;
   return la;
 }
 
@@ -299,13 +314,18 @@ void free_dev(swsph_engine *e) {
                   e->d_cell_start, e->d_tiles, e->d.vfx, e->d.vfy, e->d.vfz,
                   e->d.ufull, e->d.perm, e->d_key[0], e->d_key[1],
                   e->d_val[0], e->d_val[1], e->d_sort_tmp, e->d_stage,
-                  e->d_subflag, e->d_inv, e->d_subind, e->d_suberr};
+                  e->d_subflag, e->d_inv, e->d_subind, e->d_suberr,
+                  e->d_cmask, e->d_cact, e->d_clist};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   e->d_subflag = e->d_inv = e->d_subind = nullptr;
   e->d_suberr = nullptr;
   e->sub_cap = e->subind_cap = 0;
   e->inv_gen = 0;
+  e->d_cmask = e->d_cact = nullptr;
+  e->d_clist = nullptr;
+  e->cmask_cap = 0;
+  e->cmask_on = false;
   e->d = DevParts{};
   e->ts = TileScratch{};
   e->nl.idx = nullptr;
@@ -1017,6 +1037,67 @@ int check_fits(swsph_engine *e) {
 // drift since the last binning) must fit the stencil.
 int check_support(swsph_engine *e) { return check_fits(e); }
 
+int rebin(swsph_engine *e);
+
+// A coarser top-level grid for the kernel support H, as space_regrid
+// chooses it (src/space_regrid.c:118-126): floor(dim / (H space_stretch))
+// cells per axis, space_stretch = 1.10 (src/space.h:74), never finer than
+// the current grid; the particles are re-binned into it on the device.
+int regrid(swsph_engine *e, double H) {
+  int nc[3];
+  bool same = true;
+  for (int k = 0; k < 3; k++) {
+    nc[k] = std::min(e->grid.nc[k],
+                     (int)std::floor(e->grid.dim[k] / (H * 1.10)));
+    if (nc[k] < 3)
+      return fail(4, "kernel support %g needs fewer than 3 top-level cells "
+                  "per axis of the box (%g)", H, e->grid.dim[k]);
+    same = same && nc[k] == e->grid.nc[k];
+  }
+  if (same)
+    return fail(4, "kernel support %g exceeds the top-level cell width %g",
+                H, e->grid.dim[0] / e->grid.nc[0]);
+  for (int k = 0; k < 3; k++) {
+    e->grid.nc[k] = nc[k];
+    e->props.ncell[k] = nc[k];
+  }
+  e->ncells = nc[0] * nc[1] * nc[2];
+  for (auto &t : e->tables) t.valid = false;
+  e->cap_tiles.clear();
+  e->regrids++;
+  return rebin(e);
+}
+
+// After a whole pass that ended with status 4 (kernel support beyond the
+// 27-cell stencil): re-bin when the drift's slack alone broke it, else
+// regrid (when allowed); the caller repeats the pass from the smoothing
+// lengths found so far.
+int make_fit(swsph_engine *e) {
+  double H, dm;
+  int rc = read_fit(e, &H, &dm);
+  if (rc) return rc;
+  if (dm > 0.0 && fits(e, H, 0.0)) return rebin(e);
+  if (!e->auto_regrid || e->no_alloc)
+    return fail(4, "kernel support %g plus twice the drift %g exceeds the "
+                "top-level cell width %g", H, dm,
+                e->grid.dim[0] / e->grid.nc[0]);
+  return regrid(e, H);
+}
+
+// Runs a whole pass, re-binning or regridding and repeating it while the
+// kernel support does not fit (at most a few times).
+template <class F>
+int fitting(swsph_engine *e, swsph_pass_stats *stats, F pass) {
+  const int64_t r0 = e->regrids;
+  int rc = pass();
+  for (int k = 0; rc == 4 && k < 4; k++) {
+    if ((rc = make_fit(e))) break;
+    rc = pass();
+  }
+  if (stats) stats->regrids = e->regrids - r0;
+  return rc;
+}
+
 // Keys, stable radix sort and cell bounds of the current n positions
 // (component k of particle i at p_k[stride * i]); the sort's values are the
 // positions' indices, written to vals_out in the new order.
@@ -1509,6 +1590,315 @@ bool field_view(const DevParts &d, int field, FieldView &v) {
     default: return false;
   }
 }
+// ---- bundles of cell tasks (ghost, extra ghost, end force, init) --------
+// The cells of a bundle as the device mask that the work-queue selections
+// read, until clear_cells.
+int select_cells(swsph_engine *e, const int *cells, int n) {
+  if (n < 0 || (n > 0 && !cells)) return fail(1, "bad cell list");
+  if (n > e->ncells) return fail(1, "%d cells listed, %d exist", n, e->ncells);
+  for (int k = 0; k < n; k++)
+    if (int rc = cell_ok(e, cells[k])) return rc;
+  if (!e->d_cmask || e->cmask_cap < e->ncells) {
+    for (void *p : {(void *)e->d_cmask, (void *)e->d_cact, (void *)e->d_clist})
+      if (p) (void)hipFree(p);
+    e->d_cmask = e->d_cact = nullptr;
+    e->d_clist = nullptr;
+    e->cmask_cap = 0;
+    HIPCHK(dalloc(&e->d_cmask, (size_t)e->ncells));
+    HIPCHK(dalloc(&e->d_cact, (size_t)e->ncells));
+    HIPCHK(dalloc(&e->d_clist, (size_t)e->ncells));
+    e->cmask_cap = e->ncells;
+  }
+  if (n > 0)
+    HIPCHK(hipMemcpyAsync(e->d_clist, cells, sizeof(int) * (size_t)n,
+                          hipMemcpyHostToDevice, e->st));
+  HIPCHK(sphk::launch_cell_mask(e->d_cmask, e->ncells, e->d_clist, n, e->st));
+  e->launches += 2;
+  // (the host list must outlive its copy)
+  HIPCHK(hipStreamSynchronize(e->st));
+  e->cmask_on = true;
+  e->tiles_c = 0;
+  return 0;
+}
+
+void clear_cells(swsph_engine *e) {
+  e->cmask_on = false;
+  e->tiles_c = 0;
+}
+
+// The active particles of the selected cells (mode 0), or those of them
+// flagged in `flag` (mode 1), as the work queue's compact list and its
+// device-side length.
+int select_particles(swsph_engine *e, int mode, const int *flag,
+                     const int **list, const int **cnt) {
+  HIPCHK(sphk::launch_build_tiles(e->d, loop_args(e), (int)e->n, e->ncells,
+                                  16, mode, flag, e->ts, e->st));
+  e->launches += 12;
+  e->tiles_c = 0;
+  *list = e->ts.alist;
+  *cnt = e->ts.scan + e->n;
+  return 0;
+}
+
+// hydro_init_part on the active particles of the cells.
+int init_cells_locked(swsph_engine *e, const int *cells, int n) {
+  int rc;
+  if ((rc = ensure_records(e)) || (rc = select_cells(e, cells, n))) return rc;
+  const int *list, *cnt;
+  if (!(rc = select_particles(e, 0, nullptr, &list, &cnt))) {
+    const hipError_t err = sphk::launch_init_density(
+        e->d, 0, (int)e->n, e->props.max_active_bin, e->props.h_max, e->st,
+        list, cnt);
+    e->launches++;
+    if (err != hipSuccess) rc = fail(2, "init: %s", hipGetErrorString(err));
+  }
+  clear_cells(e);
+  return rc;
+}
+
+// runner_do_ghost on each of the cells, as device-wide rounds over their
+// particles: end of density, Newton-Raphson / bisection step, and the
+// density re-run of the particles that go again over their 27-cell
+// stencil; then their gradient records and the chunk bounds.
+int ghost_cells_locked(swsph_engine *e, const int *cells, int n, int *rounds) {
+  int rc;
+  if ((rc = select_cells(e, cells, n))) return rc;
+  const GhostArgs ga = ghost_args(e);
+  const int *list, *cnt;
+  int redo = 0, it = 0;
+  for (; it < e->props.max_iter; it++) {
+    if ((rc = select_particles(e, 1, e->d.redo, &list, &cnt))) break;
+    const hipError_t err = sphk::launch_ghost(e->d, ga, 0, (int)e->n,
+                                              e->d_redo, e->st, list, cnt);
+    e->launches += 2;
+    if (err != hipSuccess) {
+      rc = fail(2, "ghost: %s", hipGetErrorString(err));
+      break;
+    }
+    if ((rc = read_redo(e, &redo)) || redo == 0) break;
+    if ((rc = queue_loop(e, LoopKind::Density, 1, e->d.redo, false))) break;
+  }
+  if (!rc && redo > 0)
+    rc = fail(3, "Smoothing length failed to converge on %d particles", redo);
+  if (!rc && !(rc = select_particles(e, 0, nullptr, &list, &cnt))) {
+    const hipError_t err = sphk::launch_build_records(
+        LoopKind::Gradient, e->d, 0, (int)e->n, e->st,
+        e->props.max_active_bin, list, cnt);
+    e->launches++;
+    if (err != hipSuccess) rc = fail(2, "records: %s", hipGetErrorString(err));
+  }
+  clear_cells(e);
+  if (!rc) rc = refresh_bounds(e, 0, e->ncells);
+  if (rounds) *rounds = std::max(*rounds, it + 1);
+  return rc;
+}
+
+// runner_do_extra_ghost (then the force records) or runner_do_end_hydro_-
+// force on the active particles of the cells.
+int cell_op_locked(swsph_engine *e, const int *cells, int n, bool extra) {
+  int rc;
+  if ((rc = select_cells(e, cells, n))) return rc;
+  const int *list, *cnt;
+  if (!(rc = select_particles(e, 0, nullptr, &list, &cnt))) {
+    hipError_t err;
+    if (extra) {
+      err = sphk::launch_extra_ghost(e->d, ghost_args(e), 0, (int)e->n,
+                                     e->st, list, cnt);
+      if (err == hipSuccess)
+        err = sphk::launch_build_records(LoopKind::Force, e->d, 0, (int)e->n,
+                                         e->st, e->props.max_active_bin, list,
+                                         cnt);
+      e->launches += 2;
+    } else {
+      err = sphk::launch_end_force(e->d, 0, (int)e->n,
+                                   e->props.max_active_bin, e->st, list, cnt);
+      e->launches++;
+    }
+    if (err != hipSuccess) rc = fail(2, "cell tasks: %s", hipGetErrorString(err));
+  }
+  clear_cells(e);
+  return rc;
+}
+
+// A bundle of released self / pair tasks of one loop (swsph_run_tasks).
+int run_tasks_locked(swsph_engine *e, int loop, const swsph_task *tasks,
+                     int ntasks) {
+  if (!tasks && ntasks > 0) return fail(1, "null argument");
+  if (loop < 0 || loop > 2) return fail(1, "loop index %d out of range", loop);
+  if (ntasks < 0) return fail(1, "negative task count");
+  if (ntasks == 0) return 0;
+  if (e->need_particles) return fail(1, "no particles for the current grid");
+  const LoopKind k = (LoopKind)loop;
+  // Per i-cell, bit o of the stencil images it meets (o = 9 (ox + 1) +
+  // 3 (oy + 1) + (oz + 1) for the offset of the image's cell), and the
+  // i-cells in the order the tasks first touch them.
+  std::vector<uint32_t> mask((size_t)e->ncells, 0u);
+  std::vector<int> cells;
+  auto add = [&](int ci, int o, int cj) -> int {
+    if (!mask[ci]) cells.push_back(ci);
+    if (mask[ci] >> o & 1u)
+      return fail(1, "cell %d meets cell %d twice in one bundle", ci, cj);
+    mask[ci] |= 1u << o;
+    return 0;
+  };
+  const int *nc = e->grid.nc;
+  auto coords = [&](int c, int v[3]) {
+    v[0] = c / (nc[1] * nc[2]);
+    v[1] = (c / nc[2]) % nc[1];
+    v[2] = c % nc[2];
+  };
+  int rc = 0;
+  for (int n = 0; n < ntasks; n++) {
+    const swsph_task &tk = tasks[n];
+    if ((rc = cell_ok(e, tk.ci))) return rc;
+    if (tk.type == SWSPH_TASK_SELF) {
+      if ((rc = add(tk.ci, 13, tk.ci))) return rc;
+    } else if (tk.type == SWSPH_TASK_PAIR) {
+      if ((rc = cell_ok(e, tk.cj))) return rc;
+      double t[3];
+      if (tk.ci == tk.cj || !adjacent(e, tk.ci, tk.cj, t))
+        return fail(1, "cells %d and %d are not neighbours", tk.ci, tk.cj);
+      int a[3], b[3], o = 0;
+      coords(tk.ci, a);
+      coords(tk.cj, b);
+      for (int q = 0; q < 3; q++) {
+        int dd = b[q] - a[q];
+        dd = dd > 1 ? dd - nc[q] : (dd < -1 ? dd + nc[q] : dd);
+        o = 3 * o + dd + 1;
+      }
+      if ((rc = add(tk.ci, o, tk.cj)) || (rc = add(tk.cj, 26 - o, tk.ci)))
+        return rc;
+    } else {
+      return fail(1, "task %d: unknown type %d", n, tk.type);
+    }
+  }
+  if ((rc = ensure_records(e))) return rc;
+  const swsph_engine::ChunkTable *tb = chunk_table(e, chunk_c(e, loop), &rc);
+  if (!tb) return rc;
+  std::vector<BundleItem> items;
+  std::vector<BundleNbr> nbrs;
+  for (const int c : cells) {
+    // all 27 images: the whole stencil, no list
+    const bool whole = mask[c] == (1u << 27) - 1u;
+    const int off = (int)nbrs.size();
+    int cnt = 0;
+    if (!whole) {
+      int a[3];
+      coords(c, a);
+      for (int o = 0; o < 27; o++) {
+        if (!(mask[c] >> o & 1u)) continue;
+        const int d[3] = {o / 9 - 1, (o / 3) % 3 - 1, o % 3 - 1};
+        BundleNbr bn{};
+        int v[3];
+        for (int q = 0; q < 3; q++) {
+          v[q] = a[q] + d[q];
+          bn.t[q] = 0.0;
+          if (v[q] < 0) {
+            v[q] += nc[q];
+            bn.t[q] = -e->grid.dim[q];
+          } else if (v[q] >= nc[q]) {
+            v[q] -= nc[q];
+            bn.t[q] = e->grid.dim[q];
+          }
+        }
+        bn.cell = (v[0] * nc[1] + v[1]) * nc[2] + v[2];
+        nbrs.push_back(bn);
+        cnt++;
+      }
+    }
+    for (int ch = tb->cell_chunk_h[c]; ch < tb->cell_chunk_h[c + 1]; ch++)
+      items.push_back(BundleItem{ch, off, whole ? -1 : cnt, 0});
+  }
+  if (items.empty()) return 0;
+  if (items.size() > e->cap_bitems) {
+    if (e->d_bitems) (void)hipFree(e->d_bitems);
+    e->d_bitems = nullptr;
+    HIPCHK(hipMalloc((void **)&e->d_bitems, items.size() * sizeof(BundleItem)));
+    e->cap_bitems = items.size();
+  }
+  if (nbrs.empty()) nbrs.push_back(BundleNbr{});
+  if (nbrs.size() > e->cap_bnbrs) {
+    if (e->d_bnbrs) (void)hipFree(e->d_bnbrs);
+    e->d_bnbrs = nullptr;
+    HIPCHK(hipMalloc((void **)&e->d_bnbrs, nbrs.size() * sizeof(BundleNbr)));
+    e->cap_bnbrs = nbrs.size();
+  }
+  // the host vectors must outlive the copies: wait for them
+  HIPCHK(hipMemcpyAsync(e->d_bitems, items.data(),
+                        items.size() * sizeof(BundleItem),
+                        hipMemcpyHostToDevice, e->st));
+  HIPCHK(hipMemcpyAsync(e->d_bnbrs, nbrs.data(), nbrs.size() * sizeof(BundleNbr),
+                        hipMemcpyHostToDevice, e->st));
+  HIPCHK(hipStreamSynchronize(e->st));
+  BundleArgs ba{e->d_bitems, e->d_bnbrs, (int)items.size()};
+  HIPCHK(sphk::launch_bundle(k, e->d, loop_args(e), chunk_args(tb), ba, e->st));
+  e->launches++;
+  return 0;
+}
+
+
+// The engine as the device side of the task scheduler (sph_sched.cpp); the
+// engine mutex is held by the caller.
+struct EngineTasks final : sphs::DeviceTasks {
+  swsph_engine *e;
+  explicit EngineTasks(swsph_engine *eng) : e(eng) {}
+  int cell_active(std::vector<unsigned char> &act) override {
+    int rc = select_cells(e, nullptr, 0);  // allocates the masks
+    clear_cells(e);
+    if (rc) return rc;
+    HIPCHK(sphk::launch_cell_active(e->d, (int)e->n, e->ncells,
+                                    e->props.max_active_bin, e->d_cact,
+                                    e->st));
+    e->launches += 2;
+    act.assign((size_t)e->ncells, 0);
+    HIPCHK(hipMemcpyAsync(act.data(), e->d_cact, (size_t)e->ncells,
+                          hipMemcpyDeviceToHost, e->st));
+    HIPCHK(hipStreamSynchronize(e->st));
+    return 0;
+  }
+  int init_cells(const std::vector<int> &c) override {
+    return init_cells_locked(e, c.data(), (int)c.size());
+  }
+  int loop_tasks(int loop, const std::vector<swsph_task> &t) override {
+    return run_tasks_locked(e, loop, t.data(), (int)t.size());
+  }
+  int ghost_cells(const std::vector<int> &c, int *rounds) override {
+    return ghost_cells_locked(e, c.data(), (int)c.size(), rounds);
+  }
+  int extra_ghost_cells(const std::vector<int> &c) override {
+    return cell_op_locked(e, c.data(), (int)c.size(), true);
+  }
+  int end_force_cells(const std::vector<int> &c) override {
+    return cell_op_locked(e, c.data(), (int)c.size(), false);
+  }
+};
+
+// One pass through the task graph (swsph_sched_pass).
+int sched_pass_locked(swsph_engine *e, swsph_sched_stats *ss,
+                      swsph_pass_stats *stats) {
+  int rc;
+  if (e->need_particles)
+    return fail(1, "the grid changed: call swsph_set_particles first");
+  if (e->n == 0) {
+    if (ss) *ss = swsph_sched_stats{};
+    return empty_pass(stats);
+  }
+  const int64_t l0 = e->launches;
+  e->list_rebuilt = e->list_fallback = 0;
+  e->density_work = 0;
+  if ((rc = ensure_records(e))) return rc;
+  e->last_pass_mab = e->props.max_active_bin;
+  HIPCHK(sphk::launch_zero_u64(e->d_redo + 3, 1, e->st));
+  e->launches++;
+  EngineTasks dev(e);
+  int rounds = 0;
+  swsph_sched_stats local{};
+  rc = sphs::run_hydro_graph(dev, e->grid.nc, ss ? ss : &local, &rounds);
+  clear_cells(e);
+  if (rc) return rc == 2 ? fail(2, "task graph left tasks unreleased") : rc;
+  e->last_rounds = std::max(1, rounds);
+  return finish_stats(e, stats, std::max(1, rounds), l0);
+}
 }  // namespace
 
 extern "C" {
@@ -1756,6 +2146,43 @@ int swsph_get_field(swsph_engine *e, int field, void *out) {
   }
 }
 
+int swsph_get_fields(swsph_engine *e, int nf, const int *fields,
+                     void *const *outs) {
+  if (!e || nf < 0 || (nf > 0 && (!fields || !outs)))
+    return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  const int n = (int)e->n;
+  for (int q = 0; q < nf; q++) {
+    FieldView v;
+    if (!field_view(e->d, fields[q], v))
+      return fail(1, "unknown field %d", fields[q]);
+    if (!outs[q]) return fail(1, "null output for field %d", fields[q]);
+    // scattered into the caller's order in the staging buffer, copied out;
+    // the stream keeps the next field's scatter behind this copy
+    auto fetch = [&](auto tag) -> int {
+      using T = decltype(tag);
+      T *stage = (T *)e->d_stage;
+      for (int k = 0; k < v.ncomp; k++)
+        HIPCHK(sphk::launch_scatter(stage, (const T *)v.c[k], v.ncomp, k,
+                                    e->d.perm, n, e->st));
+      if (n)
+        HIPCHK(hipMemcpyAsync(outs[q], stage, sizeof(T) * (size_t)v.ncomp * n,
+                              hipMemcpyDeviceToHost, e->st));
+      return 0;
+    };
+    int rc;
+    switch (v.kind) {
+      case FieldView::F64: rc = fetch(double{}); break;
+      case FieldView::F32: rc = fetch(float{}); break;
+      case FieldView::I32: rc = fetch(int{}); break;
+      default: rc = fetch(int8_t{});
+    }
+    if (rc) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(e->st));
+  return 0;
+}
+
 int swsph_cell_write_back(swsph_engine *e, int ci, const swsph_field_dst *dst,
                           int nfields) {
   if (!e || (nfields > 0 && !dst) || nfields < 0)
@@ -1859,93 +2286,62 @@ int swsph_init_cell(swsph_engine *e, int ci) {
 // a list of neighbour images, and every chunk of such an i-cell becomes one
 // work item.  Equivalent to running the tasks one by one (the same
 // interactions; FP32 sums in another order).
+int swsph_init_cells(swsph_engine *e, const int *cells, int n) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  return init_cells_locked(e, cells, n);
+}
+
+int swsph_do_ghost_cells(swsph_engine *e, const int *cells, int n,
+                         int *rounds) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  return ghost_cells_locked(e, cells, n, rounds);
+}
+
+int swsph_do_extra_ghost_cells(swsph_engine *e, const int *cells, int n) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  return cell_op_locked(e, cells, n, true);
+}
+
+int swsph_do_end_hydro_force_cells(swsph_engine *e, const int *cells, int n) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  return cell_op_locked(e, cells, n, false);
+}
+
+int swsph_cell_active(swsph_engine *e, uint8_t *act) {
+  if (!e || !act) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->need_particles) return fail(1, "no particles for the current grid");
+  EngineTasks dev(e);
+  std::vector<unsigned char> a;
+  if (int rc = dev.cell_active(a)) return rc;
+  std::memcpy(act, a.data(), a.size());
+  return 0;
+}
+
+int swsph_sched_pass(swsph_engine *e, swsph_sched_stats *sched,
+                     swsph_pass_stats *stats) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  return fitting(e, stats,
+                 [&]() { return sched_pass_locked(e, sched, stats); });
+}
+
+int swsph_set_step_sched(swsph_engine *e, int on) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  e->step_sched = on ? 1 : 0;
+  return 0;
+}
+
 int swsph_run_tasks(swsph_engine *e, int loop, const swsph_task *tasks,
                     int ntasks) {
-  if (!e || (!tasks && ntasks > 0)) return fail(1, "null argument");
+  if (!e) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
-  if (loop < 0 || loop > 2) return fail(1, "loop index %d out of range", loop);
-  if (ntasks < 0) return fail(1, "negative task count");
-  if (ntasks == 0) return 0;
-  if (e->need_particles) return fail(1, "no particles for the current grid");
-  const LoopKind k = (LoopKind)loop;
-  // neighbour images per i-cell, in task order
-  std::vector<int> cells;                       // i-cells with a list
-  std::vector<std::vector<BundleNbr>> lists;
-  std::unordered_map<int, int> slot;            // i-cell -> index in lists
-  auto add = [&](int ci, int cj, const double t[3]) -> int {
-    auto it = slot.find(ci);
-    int q;
-    if (it == slot.end()) {
-      q = (int)lists.size();
-      slot.emplace(ci, q);
-      cells.push_back(ci);
-      lists.emplace_back();
-    } else {
-      q = it->second;
-    }
-    for (const BundleNbr &b : lists[q])
-      if (b.cell == cj && b.t[0] == t[0] && b.t[1] == t[1] && b.t[2] == t[2])
-        return fail(1, "cell %d meets cell %d twice in one bundle", ci, cj);
-    BundleNbr b{};
-    b.cell = cj;
-    for (int a = 0; a < 3; a++) b.t[a] = t[a];
-    lists[q].push_back(b);
-    return 0;
-  };
-  int rc = 0;
-  for (int n = 0; n < ntasks; n++) {
-    const swsph_task &tk = tasks[n];
-    if ((rc = cell_ok(e, tk.ci))) return rc;
-    const double zero[3] = {0.0, 0.0, 0.0};
-    if (tk.type == SWSPH_TASK_SELF) {
-      if ((rc = add(tk.ci, tk.ci, zero))) return rc;
-    } else if (tk.type == SWSPH_TASK_PAIR) {
-      if ((rc = cell_ok(e, tk.cj))) return rc;
-      double t[3];
-      if (tk.ci == tk.cj || !adjacent(e, tk.ci, tk.cj, t))
-        return fail(1, "cells %d and %d are not neighbours", tk.ci, tk.cj);
-      const double mt[3] = {-t[0], -t[1], -t[2]};
-      if ((rc = add(tk.ci, tk.cj, t)) || (rc = add(tk.cj, tk.ci, mt))) return rc;
-    } else {
-      return fail(1, "task %d: unknown type %d", n, tk.type);
-    }
-  }
-  if ((rc = ensure_records(e))) return rc;
-  const swsph_engine::ChunkTable *tb = chunk_table(e, chunk_c(e, loop), &rc);
-  if (!tb) return rc;
-  std::vector<BundleItem> items;
-  std::vector<BundleNbr> nbrs;
-  for (size_t q = 0; q < cells.size(); q++) {
-    const int c = cells[q];
-    const int off = (int)nbrs.size();
-    nbrs.insert(nbrs.end(), lists[q].begin(), lists[q].end());
-    for (int ch = tb->cell_chunk_h[c]; ch < tb->cell_chunk_h[c + 1]; ch++)
-      items.push_back(BundleItem{ch, off, (int)lists[q].size(), 0});
-  }
-  if (items.empty()) return 0;
-  if (items.size() > e->cap_bitems) {
-    if (e->d_bitems) (void)hipFree(e->d_bitems);
-    e->d_bitems = nullptr;
-    HIPCHK(hipMalloc((void **)&e->d_bitems, items.size() * sizeof(BundleItem)));
-    e->cap_bitems = items.size();
-  }
-  if (nbrs.size() > e->cap_bnbrs) {
-    if (e->d_bnbrs) (void)hipFree(e->d_bnbrs);
-    e->d_bnbrs = nullptr;
-    HIPCHK(hipMalloc((void **)&e->d_bnbrs, nbrs.size() * sizeof(BundleNbr)));
-    e->cap_bnbrs = nbrs.size();
-  }
-  // the host vectors must outlive the copies: wait for them
-  HIPCHK(hipMemcpyAsync(e->d_bitems, items.data(),
-                        items.size() * sizeof(BundleItem),
-                        hipMemcpyHostToDevice, e->st));
-  HIPCHK(hipMemcpyAsync(e->d_bnbrs, nbrs.data(), nbrs.size() * sizeof(BundleNbr),
-                        hipMemcpyHostToDevice, e->st));
-  HIPCHK(hipStreamSynchronize(e->st));
-  BundleArgs ba{e->d_bitems, e->d_bnbrs, (int)items.size()};
-  HIPCHK(sphk::launch_bundle(k, e->d, loop_args(e), chunk_args(tb), ba, e->st));
-  e->launches++;
-  return 0;
+  return run_tasks_locked(e, loop, tasks, ntasks);
 }
 
 SELF_ENTRY(swsph_doself1_density, LoopKind::Density, false)
@@ -2009,7 +2405,10 @@ int swsph_do_ghost(swsph_engine *e, int ci, int offset, int ntasks) {
   if (ntasks < 1 || offset < 0 || offset >= ntasks)
     return fail(1, "ghost share %d of %d", offset, ntasks);
   int rounds = 0;
-  return ghost_cell(e, ci, &rounds, offset, ntasks);
+  if ((rc = ghost_cell(e, ci, &rounds, offset, ntasks))) return rc;
+  // a support past the cell width would truncate the neighbour sets of the
+  // tasks that follow: status 4 (the caller rebuilds its cells)
+  return check_fits(e);
 }
 
 int swsph_do_extra_ghost(swsph_engine *e, int ci) {
@@ -2090,7 +2489,21 @@ int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
 int swsph_pass_batched(swsph_engine *e, swsph_pass_stats *stats) {
   if (!e) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
-  return pass_batched_locked(e, stats);
+  return fitting(e, stats, [&]() { return pass_batched_locked(e, stats); });
+}
+
+int swsph_set_regrid(swsph_engine *e, int on) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  e->auto_regrid = on != 0;
+  return 0;
+}
+
+int swsph_get_grid(swsph_engine *e, int ncell[3]) {
+  if (!e || !ncell) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  for (int k = 0; k < 3; k++) ncell[k] = e->grid.nc[k];
+  return 0;
 }
 
 int swsph_pass_batched_noalloc(swsph_engine *e, swsph_pass_stats *stats) {
@@ -2131,9 +2544,15 @@ int swsph_pass_batched_noalloc(swsph_engine *e, swsph_pass_stats *stats) {
   return rc;
 }
 
+static int pass_graph_locked(swsph_engine *e, swsph_pass_stats *stats);
+
 int swsph_pass_graph(swsph_engine *e, swsph_pass_stats *stats) {
   if (!e) return fail(1, "null argument");
   std::lock_guard<std::mutex> lk(e->mtx);
+  return fitting(e, stats, [&]() { return pass_graph_locked(e, stats); });
+}
+
+static int pass_graph_locked(swsph_engine *e, swsph_pass_stats *stats) {
   int rc = 0;
   if (e->use_lists)
     return fail(1, "graph replay needs the neighbour lists disabled");
@@ -2635,15 +3054,15 @@ int swsph_engine_step(swsph_engine *e, swsph_step_info *info) {
     e->tiles_c = 0;
   }
   swsph_pass_stats ps{};
-  rc = pass_batched_locked(e, &ps);
-  if (rc == 4 && e->dmax > 0.f) {
-    // the smoothing lengths outgrew the slack the drift left: re-bin and
-    // repeat the pass from the lengths found so far
-    if (!(rc = rebin(e))) {
-      rebuilt = 1;
-      rc = pass_batched_locked(e, &ps);
-    }
-  }
+  auto pass = [&]() {
+    return e->step_sched ? sched_pass_locked(e, nullptr, &ps)
+                         : pass_batched_locked(e, &ps);
+  };
+  // the smoothing lengths may outgrow the slack the drift left (re-bin) or
+  // the cells themselves (regrid): the pass is then repeated
+  const uint64_t pg0 = e->perm_gen;
+  rc = fitting(e, &ps, pass);
+  if (e->perm_gen != pg0) rebuilt = 1;
   e->sweep_list = false;
   if (sparse) {
     for (int k = 0; k < 3; k++) {

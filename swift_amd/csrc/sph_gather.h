@@ -594,7 +594,8 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_stencil(
                                     0.0, 0.0, nl);
 }
 
-// Task bundle: one wave per item (an i-chunk and its neighbour images).
+// Task bundle: one wave per item (an i-chunk and its neighbour images, or
+// with n < 0 its cell's whole 27-cell stencil).
 template <class L, int C>
 __global__ __launch_bounds__(64) void k_loop_bundle(
     DevParts d, LoopArgs la, ChunkArgs ca,
@@ -602,9 +603,11 @@ __global__ __launch_bounds__(64) void k_loop_bundle(
     BundleArgs ba) {
   const BundleItem it = ba.items[blockIdx.x];
   const NbrList none{};
+  const int nn = uniform(it.n);
   run_chunk<L, C, C, false, false, false>(
-      d, la, ca, rec, pos, uniform(it.ich), false, 0, 0.0, 0.0, 0.0, none,
-      nullptr, 0, ba.nbrs + uniform(it.off), uniform(it.n));
+      d, la, ca, rec, pos, uniform(it.ich), nn < 0, 0, 0.0, 0.0, 0.0, none,
+      nullptr, 0, nn < 0 ? nullptr : ba.nbrs + uniform(it.off),
+      nn < 0 ? 0 : nn);
 }
 
 // Per task: up to two directed (i-cell <- j-cell) interactions; blocks

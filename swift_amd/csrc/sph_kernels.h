@@ -148,6 +148,9 @@ struct LoopArgs {
   // Bound on the per-component difference between the FP32 pre-test
   // distance and the pair contract's distance (see pretest_radius2).
   float pos_eps;
+  // Work-queue selections take only particles of cells with cmask[c] != 0
+  // (null: every cell) -- the cells of a bundle of cell tasks.
+  const unsigned char *cmask;
 };
 
 // Up to two directed cell interactions (i-cell gathers from j-cell, the
@@ -166,6 +169,7 @@ struct BundleNbr {
   int cell, pad;
   double t[3];
 };
+// n < 0: the i-chunk's cell meets all 27 images of its stencil (no list).
 struct BundleItem {
   int ich, off, n, pad;
 };
@@ -272,6 +276,13 @@ hipError_t launch_bundle(LoopKind kind, const DevParts &d, const LoopArgs &la,
 hipError_t launch_build_pos(const DevParts &d, const GridDesc &g, float pos_eps,
                             int lo, int hi, hipStream_t st);
 hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t st);
+// mask[c] = 1 for the n listed cells, 0 for the other ncells - n.
+hipError_t launch_cell_mask(unsigned char *mask, int ncells, const int *cells,
+                            int n, hipStream_t st);
+// act[c] = 1 for the cells that hold an active particle, else 0.
+hipError_t launch_cell_active(const DevParts &d, int n, int ncells,
+                              int max_active_bin, unsigned char *act,
+                              hipStream_t st);
 // inv[perm[s]] = s over the n slots.
 hipError_t launch_invert_perm(const int *perm, int *inv, int n,
                               hipStream_t st);
@@ -292,7 +303,9 @@ hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
                                float *hg2max, unsigned int *hg2_all,
                                hipStream_t st);
 hipError_t launch_init_density(const DevParts &d, int lo, int hi,
-                               int max_active_bin, float h_max, hipStream_t st);
+                               int max_active_bin, float h_max, hipStream_t st,
+                               const int *list = nullptr,
+                               const int *cnt = nullptr);
 // counters[0]: particles sent back for another density round (their
 // indices are appended to d.sel); [1]: of those, the ones whose list no
 // longer covers the new radius; [2]: converged particles whose list does

@@ -190,8 +190,8 @@ __global__ __launch_bounds__(kRecBlock) void k_build_force_rec(
 // hydro_init_part (hydro.h:564-606) for active particles in [lo, hi); also
 // resets the ghost bracket.
 __global__ void k_init_density(DevParts d, int lo, int hi, int max_active_bin,
-                               float h_max) {
-  for_runs(lo, hi, d.tb, [&](int i, int8_t tb) {
+                               float h_max, const int *list, const int *cnt) {
+  auto body = [&](int i, int8_t tb) {
     const bool act = is_active(tb, max_active_bin);
     d.redo[i] = act ? 1 : 0;
     d.lfb[i] = 0;
@@ -208,7 +208,11 @@ __global__ void k_init_density(DevParts d, int lo, int hi, int max_active_bin,
     d.lapu[i] = 0.f;
     d.left[i] = 0.f;
     d.right[i] = h_max;
-  });
+  };
+  if (list)
+    for_runs_list(list, cnt, d.tb, body);
+  else
+    for_runs(lo, hi, d.tb, body);
 }
 
 // hydro_prepare_gradient + hydro_reset_gradient (hydro.h:671-750) on a
@@ -531,6 +535,25 @@ __global__ void k_zero_u64(unsigned long long *p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
+// mask[c] = 1 for the n cells of `cells`, 0 for the others (two launches:
+// clear, then set).
+__global__ void k_zero_u8(unsigned char *p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+__global__ void k_set_mask(unsigned char *mask, const int *cells, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) mask[cells[k]] = 1;
+}
+
+// act[c] = 1 for the cells holding an active particle (cleared before).
+__global__ void k_cell_active(DevParts d, int n, int max_active_bin,
+                              unsigned char *act) {
+  for_runs(0, n, d.tb, [&](int i, int8_t tb) {
+    if (is_active(tb, max_active_bin)) act[d.cell[i]] = 1;
+  });
+}
+
 // inv[perm[s]] = s: device slot of every caller index.
 __global__ void k_invert_perm(const int *perm, int *inv, int n) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -684,6 +707,7 @@ __device__ __forceinline__ bool tile_pred(const DevParts &d, const LoopArgs &la,
                                           int mode, const int *flag, int i,
                                           int8_t tb) {
   if (!is_active(tb, la.max_active_bin)) return false;
+  if (la.cmask && !la.cmask[d.cell[i]]) return false;
   if (mode >= 1 && !flag[i]) return false;
   // mode 2: only particles whose neighbour list does not cover their radius
   if (mode == 2 && search_radius2(d.h[i]) <= d.r2b[i]) return false;
@@ -1079,6 +1103,29 @@ hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st) {
   return hipGetLastError();
 }
 
+hipError_t launch_cell_mask(unsigned char *mask, int ncells, const int *cells,
+                            int n, hipStream_t st) {
+  if (ncells > 0)
+    hipLaunchKernelGGL(k_zero_u8, dim3(nblk(ncells, 256)), dim3(256), 0, st,
+                       mask, ncells);
+  if (n > 0)
+    hipLaunchKernelGGL(k_set_mask, dim3(nblk(n, 256)), dim3(256), 0, st, mask,
+                       cells, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_cell_active(const DevParts &d, int n, int ncells,
+                              int max_active_bin, unsigned char *act,
+                              hipStream_t st) {
+  if (ncells > 0)
+    hipLaunchKernelGGL(k_zero_u8, dim3(nblk(ncells, 256)), dim3(256), 0, st,
+                       act, ncells);
+  if (n > 0)
+    hipLaunchKernelGGL(k_cell_active, dim3(run_blocks(n, 256)), dim3(256), 0,
+                       st, d, n, max_active_bin, act);
+  return hipGetLastError();
+}
+
 hipError_t launch_invert_perm(const int *perm, int *inv, int n,
                               hipStream_t st) {
   if (n <= 0) return hipSuccess;
@@ -1159,10 +1206,11 @@ hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
 }
 
 hipError_t launch_init_density(const DevParts &d, int lo, int hi,
-                               int max_active_bin, float h_max, hipStream_t st) {
+                               int max_active_bin, float h_max, hipStream_t st,
+                               const int *list, const int *cnt) {
   if (hi <= lo) return hipSuccess;
   hipLaunchKernelGGL(k_init_density, dim3(run_blocks(hi - lo, 256)), dim3(256), 0, st,
-                     d, lo, hi, max_active_bin, h_max);
+                     d, lo, hi, max_active_bin, h_max, list, cnt);
   return hipGetLastError();
 }
 

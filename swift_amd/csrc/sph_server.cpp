@@ -46,19 +46,57 @@ struct Box {
   bool busy = false;
 };
 
+// Page-locked host blocks, reused from request to request: the host copies
+// of submitted particles and the results live in them, so that uploads and
+// result copies are DMA transfers that do not wait on the host.  A block is
+// allocated (hipHostMalloc, rounded up to a power of two of at least 1 MiB)
+// only when no free one is large enough.
+struct PinnedPool {
+  std::mutex m;
+  std::multimap<size_t, unsigned char *> free;
+  std::vector<unsigned char *> all;
+  unsigned char *get(size_t bytes, size_t *cap) {
+    std::lock_guard<std::mutex> lk(m);
+    auto it = free.lower_bound(bytes);
+    if (it != free.end()) {
+      unsigned char *p = it->second;
+      *cap = it->first;
+      free.erase(it);
+      return p;
+    }
+    size_t c = (size_t)1 << 20;
+    while (c < bytes) c <<= 1;
+    void *p = nullptr;
+    if (hipHostMalloc(&p, c, 0) != hipSuccess) return nullptr;
+    all.push_back((unsigned char *)p);
+    *cap = c;
+    return (unsigned char *)p;
+  }
+  void put(unsigned char *p, size_t cap) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(m);
+    free.emplace(cap, p);
+  }
+  ~PinnedPool() {
+    for (unsigned char *p : all) (void)hipHostFree(p);
+  }
+};
+
 struct Request {
   int64_t id = 0;
   int64_t box = 0;        // resident box to step (0: upload from `parts`)
   bool keep = false;      // keep the box resident after the step
   swsph_props props{};
   int64_t n = 0;
-  // host copies of the submitted particles
-  std::vector<double> x;
-  std::vector<float> v, m, u, h, rho, P, c, f, bal, av, ad, dvp;
-  std::vector<int8_t> tb;
+  // host copy of the submitted particles: arrays in one pinned block
+  unsigned char *in = nullptr;
+  size_t in_cap = 0;
+  swsph_host_parts parts{};
   size_t bytes = 0;
-  // results
-  std::vector<std::vector<unsigned char>> fields;
+  // results: the fetched fields in one pinned block, at off[field]
+  unsigned char *out = nullptr;
+  size_t out_cap = 0;
+  int64_t off[SWSPH_NUM_FIELDS];
   swsph_request_info info{};
   bool done = false;
   std::string err;
@@ -85,6 +123,7 @@ size_t field_bytes(int f) {
 
 struct swsph_server {
   swsph_server_opts opts{};
+  PinnedPool pool;
   std::mutex m;
   std::condition_variable cv_work, cv_done;
   std::deque<std::shared_ptr<Request>> queue;
@@ -127,23 +166,12 @@ void serve(swsph_server *s, const std::shared_ptr<Request> &r,
     } else if (swsph_engine_create(&r->props, &e)) {
       return fail_req("engine_create");
     }
-    swsph_host_parts hp{};
-    hp.n = r->n;
-    hp.x = r->x.data();
-    hp.v = r->v.data();
-    hp.m = r->m.data();
-    hp.u = r->u.data();
-    hp.time_bin = r->tb.data();
-    hp.h = r->h.data();
-    hp.rho = r->rho.data();
-    hp.pressure = r->P.data();
-    hp.soundspeed = r->c.data();
-    hp.f = r->f.data();
-    hp.balsara = r->bal.data();
-    hp.alpha_visc = r->av.data();
-    hp.alpha_diff = r->ad.data();
-    hp.div_v_prev = r->dvp.data();
-    if (swsph_set_particles(e, &hp)) {
+    const swsph_host_parts hp = r->parts;
+    const int up = swsph_set_particles(e, &hp);
+    // the upload has completed: the input block goes back to the pool
+    s->pool.put(r->in, r->in_cap);
+    r->in = nullptr;
+    if (up) {
       fail_req("set_particles");
       swsph_engine_destroy(e);
       return;
@@ -163,13 +191,25 @@ void serve(swsph_server *s, const std::shared_ptr<Request> &r,
   if (rc) {
     fail_req("pass");
   } else {
-    r->fields.resize(SWSPH_NUM_FIELDS);
+    // every requested field in one pinned block, fetched with one wait
     const unsigned int mask = s->opts.fields ? s->opts.fields : ~0u;
-    for (int f = 0; f < SWSPH_NUM_FIELDS && r->err.empty(); f++) {
+    std::vector<int> fl;
+    std::vector<void *> outs;
+    size_t total = 0;
+    for (int f = 0; f < SWSPH_NUM_FIELDS; f++) {
+      r->off[f] = -1;
       if (!(mask >> f & 1u)) continue;
-      r->fields[f].resize((size_t)r->n * field_bytes(f));
-      if (r->n && swsph_get_field(e, f, r->fields[f].data()))
-        fail_req("get_field");
+      r->off[f] = (int64_t)total;
+      total += ((size_t)r->n * field_bytes(f) + 255) & ~(size_t)255;
+      fl.push_back(f);
+    }
+    r->out = s->pool.get(std::max<size_t>(total, 1), &r->out_cap);
+    if (!r->out) {
+      r->err = "page-locked result buffer: allocation failed";
+    } else {
+      for (int f : fl) outs.push_back(r->out + r->off[f]);
+      if (r->n && swsph_get_fields(e, (int)fl.size(), fl.data(), outs.data()))
+        fail_req("get_fields");
     }
     r->info.ghost_rounds = st.ghost_rounds;
     r->info.n_density = st.n_density;
@@ -249,10 +289,8 @@ void worker(swsph_server *s) {
       r->info.t_done_ms = now_ms();
       r->done = true;
       // the request's host copy of the particles is no longer needed
-      r->x = {};
-      r->v = r->m = r->u = r->h = r->rho = r->P = r->c = r->f = r->bal =
-          r->av = r->ad = r->dvp = {};
-      r->tb = {};
+      s->pool.put(r->in, r->in_cap);
+      r->in = nullptr;
     }
     s->cv_done.notify_all();
     s->cv_work.notify_all();
@@ -316,26 +354,48 @@ int swsph_server_submit(swsph_server *s, const swsph_props *props,
   r->keep = keep != 0;
   const int64_t n = p->n;
   r->n = n;
-  r->x.assign(p->x, p->x + 3 * n);
-  r->v.assign(p->v, p->v + 3 * n);
-  r->m.assign(p->m, p->m + n);
-  r->u.assign(p->u, p->u + n);
-  r->tb.assign(p->time_bin, p->time_bin + n);
-  r->h.assign(p->h, p->h + n);
-  auto opt = [&](std::vector<float> &dst, const float *src, float dflt) {
-    if (src)
-      dst.assign(src, src + n);
-    else
-      dst.assign(n, dflt);
+  // the arrays, 256-byte aligned, in one pinned block
+  const size_t un = (size_t)n;
+  const size_t sizes[14] = {24 * un, 12 * un, 4 * un, 4 * un, un, 4 * un,
+                            4 * un, 4 * un, 4 * un, 4 * un, 4 * un, 4 * un,
+                            4 * un, 4 * un};
+  size_t offs[14], total = 0;
+  for (int k = 0; k < 14; k++) {
+    offs[k] = total;
+    total += (sizes[k] + 255) & ~(size_t)255;
+  }
+  r->in = s->pool.get(std::max<size_t>(total, 1), &r->in_cap);
+  if (!r->in) return sfail(2, "page-locked request buffer: allocation failed");
+  unsigned char *b = r->in;
+  auto put = [&](int k, const void *src) {
+    if (sizes[k]) std::memcpy(b + offs[k], src, sizes[k]);
+    return (const void *)(b + offs[k]);
   };
-  opt(r->rho, p->rho, 0.f);
-  opt(r->P, p->pressure, 0.f);
-  opt(r->c, p->soundspeed, 0.f);
-  opt(r->f, p->f, 0.f);
-  opt(r->bal, p->balsara, 0.f);
-  opt(r->av, p->alpha_visc, 0.1f);
-  opt(r->ad, p->alpha_diff, 0.f);
-  opt(r->dvp, p->div_v_prev, 0.f);
+  auto putf = [&](int k, const float *src, float dflt) {
+    float *dst = (float *)(b + offs[k]);
+    if (src) {
+      if (sizes[k]) std::memcpy(dst, src, sizes[k]);
+    } else {
+      for (int64_t i = 0; i < n; i++) dst[i] = dflt;
+    }
+    return (const float *)dst;
+  };
+  swsph_host_parts &hp = r->parts;
+  hp.n = n;
+  hp.x = (const double *)put(0, p->x);
+  hp.v = (const float *)put(1, p->v);
+  hp.m = (const float *)put(2, p->m);
+  hp.u = (const float *)put(3, p->u);
+  hp.time_bin = (const int8_t *)put(4, p->time_bin);
+  hp.h = (const float *)put(5, p->h);
+  hp.rho = putf(6, p->rho, 0.f);
+  hp.pressure = putf(7, p->pressure, 0.f);
+  hp.soundspeed = putf(8, p->soundspeed, 0.f);
+  hp.f = putf(9, p->f, 0.f);
+  hp.balsara = putf(10, p->balsara, 0.f);
+  hp.alpha_visc = putf(11, p->alpha_visc, 0.1f);
+  hp.alpha_diff = putf(12, p->alpha_diff, 0.f);
+  hp.div_v_prev = putf(13, p->div_v_prev, 0.f);
   const int64_t ncells =
       (int64_t)props->ncell[0] * props->ncell[1] * props->ncell[2];
   r->bytes = engine_bytes(n, ncells);
@@ -398,9 +458,10 @@ int swsph_server_get(swsph_server *s, int64_t id, int field, void *host_out) {
     if (!r->done) return sfail(1, "request not finished");
   }
   if (!r->err.empty()) return sfail(3, r->err);
-  if (r->fields[field].size() != (size_t)r->n * field_bytes(field))
+  if (!r->out || r->off[field] < 0)
     return sfail(1, "field not fetched (server option `fields`)");
-  std::memcpy(host_out, r->fields[field].data(), r->fields[field].size());
+  std::memcpy(host_out, r->out + r->off[field],
+              (size_t)r->n * field_bytes(field));
   return 0;
 }
 
@@ -410,6 +471,8 @@ int swsph_server_release(swsph_server *s, int64_t id) {
   auto it = s->reqs.find(id);
   if (it == s->reqs.end()) return sfail(1, "unknown request id");
   if (!it->second->done) return sfail(1, "request not finished");
+  s->pool.put(it->second->out, it->second->out_cap);
+  it->second->out = nullptr;
   s->reqs.erase(it);
   return 0;
 }

@@ -41,6 +41,15 @@ class _FieldDst(C.Structure):
     _fields_ = [("field", C.c_int), ("base", C.c_void_p), ("stride", C.c_int64)]
 
 
+class SchedStats(C.Structure):
+    _fields_ = [("active_cells", C.c_int64), ("tasks", C.c_int64),
+                ("bundles", C.c_int64), ("levels", C.c_int),
+                ("wall_ms", C.c_double)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class _TuneResult(C.Structure):
     _fields_ = [("schedule", C.c_int), ("chunk", C.c_int * 3),
                 ("loop_ms", C.c_double * 3), ("pass_ms", C.c_double),
@@ -68,7 +77,8 @@ class PassStats(C.Structure):
     _fields_ = [("ghost_rounds", C.c_int), ("n_density", C.c_int64),
                 ("n_gradient", C.c_int64), ("n_force", C.c_int64),
                 ("launches", C.c_int64), ("list_rebuilt", C.c_int64),
-                ("list_fallback", C.c_int64), ("n_density_all", C.c_int64)]
+                ("list_fallback", C.c_int64), ("n_density_all", C.c_int64),
+                ("regrids", C.c_int64)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -264,6 +274,62 @@ class Engine:
     def runner_do_end_hydro_force(self, ci: int) -> None:
         check(lib().swsph_do_end_hydro_force(self._h, ci),
               "runner_do_end_hydro_force")
+
+    # -- bundles of cell tasks and the scheduler-driven pass -----------------
+    @staticmethod
+    def _cells(cells):
+        return np.ascontiguousarray(cells, dtype=np.int32)
+
+    def init_cells(self, cells) -> None:
+        c = self._cells(cells)
+        check(lib().swsph_init_cells(self._h, c.ctypes.data_as(C.c_void_p),
+                                     int(c.size)), "swsph_init_cells")
+
+    def runner_do_ghost_cells(self, cells) -> int:
+        c = self._cells(cells)
+        r = C.c_int(0)
+        check(lib().swsph_do_ghost_cells(self._h, c.ctypes.data_as(C.c_void_p),
+                                         int(c.size), C.byref(r)),
+              "swsph_do_ghost_cells")
+        return r.value
+
+    def runner_do_extra_ghost_cells(self, cells) -> None:
+        c = self._cells(cells)
+        check(lib().swsph_do_extra_ghost_cells(
+            self._h, c.ctypes.data_as(C.c_void_p), int(c.size)),
+            "swsph_do_extra_ghost_cells")
+
+    def runner_do_end_hydro_force_cells(self, cells) -> None:
+        c = self._cells(cells)
+        check(lib().swsph_do_end_hydro_force_cells(
+            self._h, c.ctypes.data_as(C.c_void_p), int(c.size)),
+            "swsph_do_end_hydro_force_cells")
+
+    def cell_active(self) -> np.ndarray:
+        out = np.zeros(self.num_cells(), np.uint8)
+        check(lib().swsph_cell_active(self._h, out.ctypes.data_as(C.c_void_p)),
+              "swsph_cell_active")
+        return out
+
+    def sched_pass(self) -> tuple:
+        """One pass through the task graph: (scheduler stats, pass stats)."""
+        ss, ps = SchedStats(), PassStats()
+        check(lib().swsph_sched_pass(self._h, C.byref(ss), C.byref(ps)),
+              "swsph_sched_pass")
+        return ss.as_dict(), ps.as_dict()
+
+    def set_regrid(self, on: bool) -> None:
+        check(lib().swsph_set_regrid(self._h, 1 if on else 0),
+              "swsph_set_regrid")
+
+    def get_grid(self) -> tuple:
+        nc = (C.c_int * 3)()
+        check(lib().swsph_get_grid(self._h, nc), "swsph_get_grid")
+        return tuple(nc)
+
+    def set_step_sched(self, on: bool) -> None:
+        check(lib().swsph_set_step_sched(self._h, 1 if on else 0),
+              "swsph_set_step_sched")
 
     # -- task bundles (scheduler -> device export) ----------------------------
     TASK_SELF, TASK_PAIR = 0, 1

@@ -8,12 +8,15 @@ dopair2 force; src/runner_doiact_hydro.h:177-195): the same neighbour counts
 and the same fields to FP32 summation order, whether the tasks come as one
 bundle or as several bundles in any order.  The h tolerance is loose so
 that the ghost keeps the initial smoothing lengths and every path meets the
-same neighbour sets.
+same neighbour sets, which are then checked against the oracle's FP64
+rerun at that h.
 """
 from __future__ import annotations
 
 import numpy as np
 import pytest
+
+from parity import DENSITY_FIELDS, FORCE_FIELDS, assert_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -76,7 +79,7 @@ def _pass(eng, tasks, bundles):
 
 
 @pytest.mark.parametrize("box", ["modulated", "clustered"])
-def test_bundles_match_per_task_entry_points(box):
+def test_bundles_match_per_task_entry_points(orc, box):
     from swift_amd import Engine, HydroProps, ics
     props = HydroProps(ncell=(4, 4, 4), h_tolerance=1.0, max_iter=30)
     if box == "modulated":
@@ -118,6 +121,13 @@ def test_bundles_match_per_task_entry_points(box):
             a, b = r[k][act].astype(np.float64), ref[k][act].astype(np.float64)
             scale = np.abs(b).max()
             assert np.abs(a - b).max() <= 2e-5 * scale, (mode, k)
+    # and every form against the oracle's FP64 rerun at the same h
+    chk = {k: np.array(v, copy=True) for k, v in st.items()}
+    oref = orc.pass_fp64(props, chk)
+    fields = {k: v for k, v in {**DENSITY_FIELDS, **FORCE_FIELDS}.items()
+              if k in FIELDS}
+    for mode in ("tasks", "one", "split"):
+        assert_parity(res[mode], oref, act, fields=fields)
 
 
 def test_bundle_errors():

@@ -50,14 +50,14 @@ def test_structs_match_header(tmp_path):
     import subprocess
     from swift_amd._lib import HEADER_PATH
     from swift_amd.engine import (_FieldDst, _HostParts, _Props, _TuneResult,
-                                  PassStats)
+                                  PassStats, SchedStats)
     from swift_amd.server import _Info, _Opts
     if shutil.which("gcc") is None:
         pytest.skip("gcc not available")
     structs = {"swsph_props": _Props, "swsph_host_parts": _HostParts,
                "swsph_pass_stats": PassStats, "swsph_tune_result": _TuneResult,
                "swsph_server_opts": _Opts, "swsph_request_info": _Info,
-               "swsph_field_dst": _FieldDst}
+               "swsph_field_dst": _FieldDst, "swsph_sched_stats": SchedStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>',
              f'#include "{HEADER_PATH}"', 'int main(void) {']
     for cname, py in structs.items():
