@@ -277,6 +277,16 @@ LoopArgs loop_args(const swsph_engine *e) {
   la.max_active_bin = e->props.max_active_bin;
   la.pos_eps = pos_eps(e);
   la.cmask = e->cmask_on ? e->d_cmask : nullptr;
+  la.fresh = 0;
+  return la;
+}
+
+// The batched loops (tiles, stencil grid, list walks) give each particle
+// its whole stencil in one wave: density and force sums start from the
+// identity (the gradient sums start from the ghost's v_sig and alpha).
+LoopArgs batched_args(const swsph_engine *e, LoopKind k) {
+  LoopArgs la = loop_args(e);
+  la.fresh = k != LoopKind::Gradient ? 1 : 0;
   return la;
 }
 
@@ -295,6 +305,7 @@ GhostArgs ghost_args(const swsph_engine *e) {
   g.diff_beta = e->props.diff_beta;
   g.diff_alpha_max = e->props.diff_alpha_max;
   g.diff_alpha_min = e->props.diff_alpha_min;
+  g.fresh_force = 0;
   return g;
 }
 
@@ -474,13 +485,32 @@ int table_c(const swsph_engine *e, int k) {
   return e->schedule >= 1 ? e->loop_cj[k] : chunk_c(e, k);
 }
 
+// The sparse shape (tiles of 4, sub-chunks of 8) of the work queue: its
+// loops also read the chunks of 32 of the same cells.
+bool sparse_loop(const swsph_engine *e, int k) {
+  return e->schedule >= 1 && e->loop_c[k] == kMinTile &&
+         e->loop_cj[k] == kSparseSub;
+}
+// Capacity of the second table a loop reads (0: none).
+int table2_c(const swsph_engine *e, int k) {
+  return sparse_loop(e, k) ? kSparseChunk : 0;
+}
+// Builds (or refreshes) the tables loop k reads.
+int ensure_tables(swsph_engine *e, int k) {
+  int rc = 0;
+  if (!chunk_table(e, table_c(e, k), &rc)) return rc;
+  if (table2_c(e, k) && !chunk_table(e, table2_c(e, k), &rc)) return rc;
+  return 0;
+}
+
 // Tables no loop reads any more are dropped from the per-pass refresh
 // (rebuilt on their next use).
 void prune_tables(swsph_engine *e) {
   for (auto &t : e->tables) {
     bool used = false;
     for (int k = 0; k < 3; k++)
-      used = used || t.C == table_c(e, k) || t.C == chunk_c(e, k);
+      used = used || t.C == table_c(e, k) || t.C == chunk_c(e, k) ||
+             t.C == table2_c(e, k);
     if (!used) t.valid = false;
   }
 }
@@ -500,7 +530,8 @@ int refresh_bounds(swsph_engine *e, int c_lo, int c_hi, bool pos,
   }
   auto used = [&](const swsph_engine::ChunkTable &t) {
     bool u = &t == extra;
-    for (int k = 0; k < 3; k++) u = u || t.C == table_c(e, k);
+    for (int k = 0; k < 3; k++)
+      u = u || t.C == table_c(e, k) || t.C == table2_c(e, k);
     return u;
   };
   auto refresh = [&](swsph_engine::ChunkTable &t) -> int {
@@ -685,7 +716,7 @@ int stencil(swsph_engine *e, LoopKind k, bool subset,
   if (!t) return rc;
   hipEvent_t a;
   if ((rc = time_begin(e, &a))) return rc;
-  LoopArgs la = loop_args(e);
+  LoopArgs la = batched_args(e, k);
   if (flag) la.flag = flag;
   HIPCHK(sphk::launch_stencil(k, subset, e->d, la, chunk_args(t), e->st));
   e->launches++;
@@ -699,6 +730,8 @@ TileArgs tile_args(const swsph_engine *e) {
   ta.tile_cell = e->ts.tile_cell;
   ta.ntiles = e->ts.ntiles;
   ta.heads = e->ts.heads;
+  ta.ids = nullptr;
+  ta.filt = nullptr;
   return ta;
 }
 
@@ -715,9 +748,27 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
     return fail(1, "tiles of %d particles exceed the tile table", C);
   const swsph_engine::ChunkTable *t = chunk_table(e, CJ, &rc);
   if (!t) return rc;
-  const LoopArgs la = loop_args(e);
-  // The all-active table is kept for the later loops of the pass.
-  if (!(mode == 0 && e->tiles_c == C)) {
+  const bool sparse = !build && sparse_loop(e, (int)k);
+  const swsph_engine::ChunkTable *t1 =
+      sparse ? chunk_table(e, kSparseChunk, &rc) : nullptr;
+  if (sparse && !t1) return rc;
+  const LoopArgs la = batched_args(e, k);
+  TileArgs ta = tile_args(e);
+  // The all-active table is kept for the later loops of the pass, and the
+  // flagged rounds (mode 1) take the subset of its tiles that hold a
+  // flagged particle, with the others masked out, instead of a new table
+  // (a build sweeps and scans every slot; this sweeps the tiles).
+  const bool subset = mode == 1 && e->tiles_c == C && !build && !la.cmask;
+  if (subset) {
+    ta.ids = e->ts.flag;    // free until the next build (tiles <= particles)
+    ta.ntiles = e->ts.blk;  // the subset's count
+    ta.filt = flag;
+    HIPCHK(sphk::launch_select_tiles(e->ts, flag, e->ts.flag, e->ts.blk,
+                                     e->st));
+    if (e->schedule == 2 || e->capturing)
+      HIPCHK(sphk::launch_zero_u32((unsigned int *)e->ts.heads, 8 * 64,
+                                   e->st));
+  } else if (!(mode == 0 && e->tiles_c == C)) {
     HIPCHK(sphk::launch_build_tiles(e->d, la, (int)e->n, e->ncells, C, mode,
                                     flag, e->ts, e->st));
     e->tiles_c = mode == 0 ? C : 0;
@@ -729,7 +780,7 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
   if (!e->capturing && e->schedule == 1) {
     // Eager: one wave per tile of the device-side count (a grid for the
     // largest possible count would mostly be waves that exit at once).
-    HIPCHK(hipMemcpyAsync(e->h_ntiles, e->ts.ntiles, sizeof(int),
+    HIPCHK(hipMemcpyAsync(e->h_ntiles, ta.ntiles, sizeof(int),
                           hipMemcpyDeviceToHost, e->st));
     HIPCHK(hipStreamSynchronize(e->st));
     max_tiles = *e->h_ntiles;
@@ -751,10 +802,14 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
   // the loop kernel alone is timed (the tile build is not)
   hipEvent_t a;
   if ((rc = time_begin(e, &a))) return rc;
-  if (max_tiles > 0 || rest)
-    HIPCHK(sphk::launch_queue(k, build, e->d, la, chunk_args(t), tile_args(e),
-                              e->nl, C, CJ, persistent, e->queue_waves,
-                              max_tiles, rest, e->st));
+  if ((max_tiles > 0 || rest) && sparse)
+    HIPCHK(sphk::launch_sparse(k, e->d, la, chunk_args(t1), chunk_args(t), ta,
+                               persistent, e->queue_waves, max_tiles, rest,
+                               e->st));
+  else if (max_tiles > 0 || rest)
+    HIPCHK(sphk::launch_queue(k, build, e->d, la, chunk_args(t), ta, e->nl, C,
+                              CJ, persistent, e->queue_waves, max_tiles, rest,
+                              e->st));
   e->launches += 13;
   return time_end(e, k, a);
 }
@@ -788,7 +843,9 @@ int build_lists(swsph_engine *e, bool subset) {
   if (!t) return rc;
   hipEvent_t a;
   if ((rc = time_begin(e, &a))) return rc;
-  HIPCHK(sphk::launch_build_lists(subset, e->d, loop_args(e), chunk_args(t),
+  HIPCHK(sphk::launch_build_lists(subset, e->d,
+                                  batched_args(e, LoopKind::Density),
+                                  chunk_args(t),
                                   e->nl, e->st));
   e->launches++;
   return time_end(e, LoopKind::Density, a);
@@ -802,7 +859,8 @@ int list_loop(swsph_engine *e, LoopKind k, bool compact, int nsel) {
   if (!t) return rc;
   hipEvent_t a;
   if ((rc = time_begin(e, &a))) return rc;
-  HIPCHK(sphk::launch_list_loop(k, compact, e->d, loop_args(e), chunk_args(t),
+  HIPCHK(sphk::launch_list_loop(k, compact, e->d, batched_args(e, k),
+                                chunk_args(t),
                                 e->nl, nsel, e->st));
   e->launches++;
   return time_end(e, k, a);
@@ -1141,7 +1199,7 @@ int finish_binning(swsph_engine *e) {
   e->perm_gen++;
   for (auto &t : e->tables) t.valid = false;
   for (int k = 0; k < 3; k++)
-    if (!chunk_table(e, table_c(e, k), &rc)) return rc;
+    if ((rc = ensure_tables(e, k))) return rc;
   HIPCHK(hipStreamSynchronize(e->st));
   return 0;
 }
@@ -1319,7 +1377,8 @@ namespace {
 // lists: a fresh neighbour list for every active particle).
 int phase_start(swsph_engine *e) {
   HIPCHK(sphk::launch_init_density(e->d, 0, (int)e->n, e->props.max_active_bin,
-                                   e->props.h_max, e->st));
+                                   e->props.h_max, e->st, nullptr, nullptr,
+                                   true));
   HIPCHK(sphk::launch_zero_u64(e->d_redo + 3, 1, e->st));
   e->launches += 2;
   e->tiles_c = 0;  // activity may have changed since the last pass
@@ -1400,7 +1459,9 @@ int phase_finish(swsph_engine *e) {
   // after the gradient loop the queue's list holds all active particles
   const int *cnt;
   const int *list = queue_list(e, &cnt);
-  HIPCHK(sphk::launch_extra_ghost(e->d, ghost_args(e), 0, n, e->st, list, cnt));
+  GhostArgs gx = ghost_args(e);
+  gx.fresh_force = 1;
+  HIPCHK(sphk::launch_extra_ghost(e->d, gx, 0, n, e->st, list, cnt));
   HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, 0, n, e->st, mab,
                                     list, cnt));
   e->launches += 2;
@@ -2571,7 +2632,7 @@ static int pass_graph_locked(swsph_engine *e, swsph_pass_stats *stats) {
     drop_graphs(e);
     // chunk tables must exist before capture (they allocate)
     for (int k = 0; k < 3; k++)
-      if (!chunk_table(e, table_c(e, k), &rc)) return rc;
+      if ((rc = ensure_tables(e, k))) return rc;
     // the all-active tile count of each tile capacity in use
     e->cap_tiles.clear();
     if (e->schedule == 1) {
@@ -2692,16 +2753,13 @@ int swsph_set_tile_shape(swsph_engine *e, int which, int CI, int CJ) {
   if (which < 0 || which > 2) return fail(1, "loop index %d out of range", which);
   const bool ok = (CJ == CI && (CI == 8 || CI == 16 || CI == 32 || CI == 64)) ||
                   (CJ * 2 == CI && (CI == 16 || CI == 32)) ||
-                  (CI == 4 && (CJ == 16 || CJ == 32)) || (CI == 8 && CJ == 16) ||
-                  (CI == 16 && CJ == 32);
+                  (CI == 4 && (CJ == 8 || CJ == 16 || CJ == 32)) ||
+                  (CI == 8 && CJ == 16) || (CI == 16 && CJ == 32);
   if (!ok) return fail(1, "tile shape (%d, %d) not supported", CI, CJ);
   e->loop_c[which] = CI;
   e->loop_cj[which] = CJ;
   prune_tables(e);
-  if (!e->cell_start.empty()) {
-    int rc = 0;
-    if (!chunk_table(e, table_c(e, which), &rc)) return rc;
-  }
+  if (!e->cell_start.empty()) return ensure_tables(e, which);
   return 0;
 }
 
@@ -2712,7 +2770,7 @@ int swsph_set_sparse_shape(swsph_engine *e, int CI, int CJ, int per_cell) {
   if (per_cell > 0) {
     const bool ok = (CJ == CI && (CI == 8 || CI == 16 || CI == 32 || CI == 64)) ||
                     (CJ * 2 == CI && (CI == 16 || CI == 32)) ||
-                    (CI == 4 && (CJ == 16 || CJ == 32)) ||
+                    (CI == 4 && (CJ == 8 || CJ == 16 || CJ == 32)) ||
                     (CI == 8 && CJ == 16);
     if (!ok) return fail(1, "tile shape (%d, %d) not supported", CI, CJ);
     e->sparse_ci = CI;
@@ -2887,15 +2945,15 @@ int swsph_autotune(swsph_engine *e, int repeats, swsph_tune_result *out) {
   }
   if (!rc && best_s >= 1) {
     // Under the work queue: candidate chunks of half the tile size, then
-    // the narrow tiles (4, 32) and (8, 16) (a few active particles per
-    // cell), each loop keeping its fastest shape.  The trials time eager
-    // passes (one wave per tile); a captured graph sizes its grids by the
-    // tile count of its configuration, so the narrow tiles cost no idle
-    // waves there either.
+    // the narrow tiles (4, 32) and (8, 16) and the sparse tiles (4, 8) (a
+    // few active particles per cell), each loop keeping its fastest shape.
+    // The trials time eager passes (one wave per tile); a captured graph
+    // sizes its grids by the tile count of its configuration, so the narrow
+    // tiles cost no idle waves there either.
     e->schedule = best_s;
     e->queue_waves = 0;
     int best_ci[3] = {best_c[0], best_c[1], best_c[2]};
-    for (int trial = 0; trial < 3 && !rc; trial++) {
+    for (int trial = 0; trial < 4 && !rc; trial++) {
       bool any = false;
       for (int k = 0; k < 3; k++) {
         int ci = best_c[k], cj = best_c[k];
@@ -2903,6 +2961,7 @@ int swsph_autotune(swsph_engine *e, int repeats, swsph_tune_result *out) {
           if (trial == 0 && (best_c[k] == 16 || best_c[k] == 32)) cj = ci / 2;
           if (trial == 1) ci = 4, cj = 32;
           if (trial == 2) ci = 8, cj = 16;
+          if (trial == 3) ci = kMinTile, cj = kSparseSub;
         }
         e->loop_c[k] = ci;
         e->loop_cj[k] = cj;

@@ -264,11 +264,13 @@ __device__ __forceinline__ void run_chunk(
   int i, cell_i;
   bool act;
   if (TILE) {
-    // the tile builder applied the loop's predicate
-    const int ts = uniform(tq->tile_start[tile]);
-    act = g < uniform(tq->tile_start[tile + 1]) - ts;
+    // the tile builder applied the loop's predicate (and filt narrows it)
+    const int tl = tq->ids ? uniform(tq->ids[tile]) : tile;
+    const int ts = uniform(tq->tile_start[tl]);
+    act = g < uniform(tq->tile_start[tl + 1]) - ts;
     i = act ? tq->alist[ts + g] : 0;
-    cell_i = uniform(tq->tile_cell[tile]);
+    if (tq->filt && act) act = tq->filt[i] != 0;
+    cell_i = uniform(tq->tile_cell[tl]);
   } else {
     i = ca.chunk_start[ich] + g;
     act = i < ca.chunk_start[ich + 1];
@@ -289,7 +291,7 @@ __device__ __forceinline__ void run_chunk(
     st = typename L::I{};
     st.x = st.y = st.z = kFar;
   }
-  if (act && s == 0)
+  if (act && s == 0 && !la.fresh)
     L::load_acc(d, i, acc);
   else
     L::identity(acc);
@@ -646,11 +648,17 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_queue(
   const int per = (nt + 7) >> 3;
   int q = blockIdx.x & 7;
   for (int tries = 0; tries < 8;) {
+    const int hi = min(nt, (q + 1) * per);
     int base = 0;
-    if ((threadIdx.x & 63) == 0) base = atomicAdd(&tq.heads[64 * q], kGrab);
+    if ((threadIdx.x & 63) == 0) {
+      // a plain read first: an exhausted queue costs no atomic (the tail
+      // of a drain, and every wave of a grid for a nearly empty queue)
+      base = __hip_atomic_load(&tq.heads[64 * q], __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+      if (q * per + base < hi) base = atomicAdd(&tq.heads[64 * q], kGrab);
+    }
     base = uniform(__shfl(base, 0));
     const int lo = q * per + base;
-    const int hi = min(nt, (q + 1) * per);
     if (lo >= hi) {  // uniform: this queue is exhausted
       q = (q + 1) & 7;
       tries++;
@@ -734,7 +742,7 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_list(
   }
   if (!__any(act)) return;  // uniform
   typename L::Acc acc;
-  if (act && s == 0)
+  if (act && s == 0 && !la.fresh)
     L::load_acc(d, i, acc);
   else
     L::identity(acc);

@@ -66,8 +66,12 @@ struct TileArgs {
   const int *alist;
   const int *tile_start;  // [ntiles + 1]
   const int *tile_cell;   // [ntiles]
-  const int *ntiles;      // device-side count
+  const int *ntiles;      // device-side count (of ids, when ids is set)
   int *heads;             // [8 * 64]
+  // A subset of the tiles: work item t is tile ids[t], and only the
+  // particles with filt[i] != 0 take part (null: every tile, particle).
+  const int *ids;
+  const int *filt;
 };
 
 // Structure-of-arrays particle store in HBM, sorted by top-level cell.
@@ -123,6 +127,11 @@ constexpr int kListChunk = 64 / kListStreams;
 // Smallest tile (i-particles per wave) of the work-queue schedules; the
 // tile tables are sized for it.
 constexpr int kMinTile = 4;
+// Sparse tiles (sph_sparse.h): tiles of kMinTile particles, candidates
+// culled by the chunks of kSparseChunk, then by their sub-chunks of
+// kSparseSub against each particle.
+constexpr int kSparseChunk = 32;
+constexpr int kSparseSub = 8;
 // Largest particle and cell counts of the allocation-free batched pass,
 // whose per-pass scratch is a fixed device array (sph_scratch.hip).
 constexpr int64_t kNoAllocMaxParts = (int64_t)1 << 21;
@@ -151,6 +160,10 @@ struct LoopArgs {
   // Work-queue selections take only particles of cells with cmask[c] != 0
   // (null: every cell) -- the cells of a bundle of cell tasks.
   const unsigned char *cmask;
+  // 1: every particle a launch takes meets its whole 27-cell stencil in one
+  // wave, so its accumulators start from the identity and are stored, not
+  // loaded (the batched density and force loops; their resets are skipped).
+  int fresh;
 };
 
 // Up to two directed cell interactions (i-cell gathers from j-cell, the
@@ -185,6 +198,9 @@ struct GhostArgs {
   double time_base;
   float alpha_max, alpha_min, visc_length;
   float diff_beta, diff_alpha_max, diff_alpha_min;
+  // 1: the force loop that follows starts its sums from the identity, so
+  // the extra ghost leaves the force accumulators alone.
+  int fresh_force;
 };
 
 enum class LoopKind { Density, Gradient, Force };
@@ -265,6 +281,18 @@ hipError_t launch_queue(LoopKind kind, bool build_lists, const DevParts &d,
                         const TileArgs &ta, const NbrList &nl, int CI, int CJ,
                         bool persistent, int waves, int max_tiles, bool rest,
                         hipStream_t st);
+// The tiles of the table in `ts` that hold a particle with filt[i] != 0,
+// appended to ids (in no fixed order: each tile's sums are its own) with
+// their count in *nids.
+hipError_t launch_select_tiles(const TileScratch &ts, const int *filt,
+                               int *ids, int *nids, hipStream_t st);
+// The sparse form of the same loop over tiles of 4 particles: candidates
+// culled by the chunks of 32 of `c1` against the tile, then by the chunks
+// of 8 of `c2` (the same cells) against each particle (sph_sparse.h).
+hipError_t launch_sparse(LoopKind kind, const DevParts &d, const LoopArgs &la,
+                         const ChunkArgs &c1, const ChunkArgs &c2,
+                         const TileArgs &ta, bool persistent, int waves,
+                         int max_tiles, bool rest, hipStream_t st);
 hipError_t launch_task(LoopKind kind, bool subset, const DevParts &d,
                        const LoopArgs &la, const ChunkArgs &ca,
                        const TaskArgs &ta, hipStream_t st);
@@ -302,10 +330,12 @@ hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
                                int c0, int c1, int C, double *bbox,
                                float *hg2max, unsigned int *hg2_all,
                                hipStream_t st);
+// light: only the ghost's flags and brackets (the density loop that follows
+// starts its sums from the identity).
 hipError_t launch_init_density(const DevParts &d, int lo, int hi,
                                int max_active_bin, float h_max, hipStream_t st,
                                const int *list = nullptr,
-                               const int *cnt = nullptr);
+                               const int *cnt = nullptr, bool light = false);
 // counters[0]: particles sent back for another density round (their
 // indices are appended to d.sel); [1]: of those, the ones whose list no
 // longer covers the new radius; [2]: converged particles whose list does

@@ -15,6 +15,7 @@
 #include "sph_kernels.h"
 #include "sph_loops.h"
 #include "sph_gather.h"
+#include "sph_sparse.h"
 
 using namespace sph;
 using namespace sphl;
@@ -190,12 +191,16 @@ __global__ __launch_bounds__(kRecBlock) void k_build_force_rec(
 // hydro_init_part (hydro.h:564-606) for active particles in [lo, hi); also
 // resets the ghost bracket.
 __global__ void k_init_density(DevParts d, int lo, int hi, int max_active_bin,
-                               float h_max, const int *list, const int *cnt) {
+                               float h_max, const int *list, const int *cnt,
+                               int light) {
   auto body = [&](int i, int8_t tb) {
     const bool act = is_active(tb, max_active_bin);
     d.redo[i] = act ? 1 : 0;
     d.lfb[i] = 0;
     if (!act) return;
+    d.left[i] = 0.f;
+    d.right[i] = h_max;
+    if (light) return;
     d.rho[i] = 0.f;
     d.rho_dh[i] = 0.f;
     d.wc[i] = 0.f;
@@ -206,8 +211,6 @@ __global__ void k_init_density(DevParts d, int lo, int hi, int max_active_bin,
     d.rotz[i] = 0.f;
     d.nd[i] = 0;
     d.lapu[i] = 0.f;
-    d.left[i] = 0.f;
-    d.right[i] = h_max;
   };
   if (list)
     for_runs_list(list, cnt, d.tb, body);
@@ -499,6 +502,7 @@ __global__ void k_extra_ghost(DevParts d, GhostArgs ga, int lo, int hi,
     na = fmaxf(na, ga.diff_alpha_min);
     const float lim = ga.diff_alpha_max * (1.f - d.avmax[i] / ga.alpha_max);
     d.adiff[i] = fminf(na, lim);
+    if (ga.fresh_force) return;
     d.ax[i] = 0.f;
     d.ay[i] = 0.f;
     d.az[i] = 0.f;
@@ -791,6 +795,29 @@ __global__ void k_tile_fill(const int *cell_start, const int *S, const int *T,
   }
 }
 
+// Tiles with a particle of filt[i] != 0, appended to ids with one atomic
+// per wave (grid-stride over the device-side tile count).
+__global__ void k_select_tiles(const int *tile_start, const int *alist,
+                               const int *ntiles, const int *filt, int *ids,
+                               int *nids) {
+  const int nt = *ntiles;
+  const int lane = threadIdx.x & 63;
+  for (int b = blockIdx.x * blockDim.x; b < nt; b += gridDim.x * blockDim.x) {
+    const int t = b + threadIdx.x;
+    bool any = false;
+    if (t < nt) {
+      const int e = tile_start[t + 1];
+      for (int k = tile_start[t]; k < e && !any; k++) any = filt[alist[k]] != 0;
+    }
+    const unsigned long long m = __ballot(any);
+    int off = 0;
+    if (lane == 0 && m) off = atomicAdd(nids, __popcll(m));
+    off = __shfl(off, 0);
+    if (any)
+      ids[off + __popcll(m & ((1ull << lane) - 1ull))] = t;
+  }
+}
+
 }  // namespace
 
 // --------------------------------------------------------------- launchers --
@@ -987,6 +1014,74 @@ hipError_t launch_queue(LoopKind kind, bool build_lists, const DevParts &d,
       return queue_any<ForceLoop>(d, la, ca, ta, nl, d.rf, CI, CJ,
                                   persistent, waves, max_tiles, rest,
                                     st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_select_tiles(const TileScratch &ts, const int *filt,
+                               int *ids, int *nids, hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, nids, 1);
+  hipLaunchKernelGGL(k_select_tiles, dim3(1024), dim3(256), 0, st,
+                     (const int *)ts.tile_start, (const int *)ts.alist,
+                     (const int *)ts.ntiles, filt, ids, nids);
+  return hipGetLastError();
+}
+
+template <class L>
+static int sparse_resident_blocks() {
+  static std::atomic<int> cached{0};
+  if (cached.load() == 0) {
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount,
+                              dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, k_sparse_queue<L>,
+                                                     64, 0) != hipSuccess ||
+        per <= 0)
+      return 0;
+    cached.store(ncu * per);
+  }
+  return cached.load();
+}
+
+template <class L>
+hipError_t sparse_l(const DevParts &d, const LoopArgs &la, const ChunkArgs &c1,
+                    const ChunkArgs &c2, const TileArgs &ta,
+                    const typename L::Rec *rec, bool persistent, int waves,
+                    int max_tiles, bool rest, hipStream_t st) {
+  const PosRec *pos = (const PosRec *)d.pos;
+  if (!persistent) {
+    const int nb = max_tiles > 0 ? xcd_blocks(max_tiles) : 0;
+    if (nb > 0)
+      hipLaunchKernelGGL((k_sparse_tiles<L>), dim3(nb), dim3(64), 0, st, d, la,
+                         c1, c2, rec, pos, ta);
+    if (rest)
+      hipLaunchKernelGGL((k_sparse_tiles_rest<L>), dim3(kRestBlocks), dim3(64),
+                         0, st, d, la, c1, c2, rec, pos, ta, nb);
+    return hipGetLastError();
+  }
+  if (waves <= 0) waves = sparse_resident_blocks<L>();
+  if (waves <= 0) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((k_sparse_queue<L>), dim3(waves), dim3(64), 0, st, d, la,
+                     c1, c2, rec, pos, ta);
+  return hipGetLastError();
+}
+
+hipError_t launch_sparse(LoopKind kind, const DevParts &d, const LoopArgs &la,
+                         const ChunkArgs &c1, const ChunkArgs &c2,
+                         const TileArgs &ta, bool persistent, int waves,
+                         int max_tiles, bool rest, hipStream_t st) {
+  if (c1.C != kSpC1 || c2.C != kSpC2) return hipErrorInvalidValue;
+  switch (kind) {
+    case LoopKind::Density:
+      return sparse_l<DensityLoop>(d, la, c1, c2, ta, d.rd, persistent, waves,
+                                   max_tiles, rest, st);
+    case LoopKind::Gradient:
+      return sparse_l<GradientLoop>(d, la, c1, c2, ta, d.rg, persistent, waves,
+                                    max_tiles, rest, st);
+    case LoopKind::Force:
+      return sparse_l<ForceLoop>(d, la, c1, c2, ta, d.rf, persistent, waves,
+                                 max_tiles, rest, st);
   }
   return hipErrorInvalidValue;
 }
@@ -1207,10 +1302,10 @@ hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
 
 hipError_t launch_init_density(const DevParts &d, int lo, int hi,
                                int max_active_bin, float h_max, hipStream_t st,
-                               const int *list, const int *cnt) {
+                               const int *list, const int *cnt, bool light) {
   if (hi <= lo) return hipSuccess;
   hipLaunchKernelGGL(k_init_density, dim3(run_blocks(hi - lo, 256)), dim3(256), 0, st,
-                     d, lo, hi, max_active_bin, h_max, list, cnt);
+                     d, lo, hi, max_active_bin, h_max, list, cnt, light ? 1 : 0);
   return hipGetLastError();
 }
 

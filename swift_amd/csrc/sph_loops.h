@@ -14,6 +14,10 @@ constexpr int kOccDensity = 7;
 constexpr int kOccGradient = 7;
 constexpr int kOccForce = 5;
 constexpr int kOccForceTiles = 6;
+// The sparse-tile kernels (sph_sparse.h): one tile per wave, and the
+// persistent form (its loop around the tile costs registers).
+constexpr int kOccForceSparse = 5;
+constexpr int kOccSparseQueue = 5;
 
 namespace sphl {
 using namespace sph;
@@ -119,6 +123,7 @@ struct DensityLoop {
   // occupancy target of the batched kernels (waves per SIMD)
   static constexpr int kWavesPerSimd = kOccDensity;
   static constexpr int kTileWavesPerSimd = kOccDensity;
+  static constexpr int kSparseWavesPerSimd = kOccDensity;
   using Rec = DensRec;
   using Acc = DensAcc;
   using I = DensI;
@@ -220,6 +225,7 @@ struct GradientLoop {
   // occupancy target of the batched kernels (waves per SIMD)
   static constexpr int kWavesPerSimd = kOccGradient;
   static constexpr int kTileWavesPerSimd = kOccGradient;
+  static constexpr int kSparseWavesPerSimd = kOccGradient;
   using Rec = GradRec;
   using Acc = GradAcc;
   using I = GradI;
@@ -302,6 +308,7 @@ struct ForceLoop {
   // occupancy target of the batched kernels (waves per SIMD)
   static constexpr int kWavesPerSimd = kOccForce;
   static constexpr int kTileWavesPerSimd = kOccForceTiles;
+  static constexpr int kSparseWavesPerSimd = kOccForceSparse;
   using Rec = ForceRec;
   using Acc = ForceAcc;
   using I = ForceI;

@@ -149,14 +149,16 @@ def test_chunk_shapes(orc, C):
 
 
 @pytest.mark.parametrize("shape", [(16, 8), (32, 16), (8, 16), (4, 16), (4, 32),
-                                   (16, 32)])
+                                   (16, 32), (4, 8)])
 @pytest.mark.parametrize("schedule", [1, 2])
 def test_tile_shapes(orc, shape, schedule):
     """Work-queue tiles against finer candidate chunks, and narrow tiles
     against coarser ones (4 and 8 particles with 16 and 8 candidate streams
-    each, FP32 pre-test for (4, 16), packed FP16 for the others), give the
-    oracle's neighbour sets and values, on a uniform and on a clustered box
-    (half of the particles active)."""
+    each, FP32 pre-test for (4, 16), packed FP16 for the others), and the
+    sparse tiles (4, 8) (chunks of 32 against the tile, their sub-chunks of
+    8 against each particle, per-particle hit lists), give the oracle's
+    neighbour sets and values, on a uniform and on a clustered box (half of
+    the particles active)."""
     from swift_amd import Engine, HydroProps, ics
     CI, CJ = shape
     for st, props in (
@@ -180,18 +182,20 @@ def test_tile_shapes(orc, shape, schedule):
         _check(orc, props, st, res)
 
 
-def test_narrow_shape_with_chunk_paths(orc):
+@pytest.mark.parametrize("cj", [32, 8])
+def test_narrow_shape_with_chunk_paths(orc, cj):
     """With a narrow tile shape set (tiles of 4 exist only in the work
     queue), the paths that run table chunks as i-chunks -- the per-task
     entry points of the plain pass and the grid schedule -- use the
-    candidate chunks of 32 and still give the oracle's results."""
+    candidate chunks (of 32, or of 8 for the sparse tiles) and still give
+    the oracle's results."""
     from swift_amd import Engine, HydroProps, ics
     st = ics.modulated_box(16, seed=17, amp=0.5)
     props = HydroProps(ncell=(4, 4, 4))
     for run in ("plain", "grid"):
         with Engine(props) as eng:
             for loop in ("density", "gradient", "force"):
-                eng.set_tile_shape(loop, 4, 32)
+                eng.set_tile_shape(loop, 4, cj)
             eng.set_particles(st)
             if run == "plain":
                 eng.pass_plain()
@@ -428,7 +432,8 @@ def test_graph_replay_matches_eager(orc, mode):
 def test_autotune_keeps_state_and_results(orc):
     """The autotuner times every schedule and chunk capacity on the current
     particles (then, under the work queue, half-size candidate chunks and
-    the narrow tiles (4, 32), (8, 16)), applies a valid choice, and leaves
+    the narrow tiles (4, 32), (8, 16) and the sparse tiles (4, 8)),
+    applies a valid choice, and leaves
     the pass state as it found it: the next pass gives the oracle's
     results."""
     from swift_amd import Engine, HydroProps, ics
@@ -438,9 +443,9 @@ def test_autotune_keeps_state_and_results(orc):
         eng.set_particles(st)
         h0 = eng.get("h")
         t = eng.autotune(repeats=1)
-        # 9 schedule x capacity trials, then up to 3 shape trials under the
+        # 9 schedule x capacity trials, then up to 4 shape trials under the
         # work queue
-        assert 9 <= t["trials"] <= 12
+        assert 9 <= t["trials"] <= 13
         assert t["schedule"] in (0, 1, 2)
         assert all(c in (4, 8, 16, 32) for c in t["chunk"])
         assert all(c in (8, 16, 32) for c in t["chunk_j"])
