@@ -97,8 +97,15 @@ struct swsph_engine {
     int *d_start = nullptr, *d_cell_chunk = nullptr, *d_chunk_cell = nullptr;
     double *d_bbox = nullptr;
     float *d_hg2max = nullptr;
+    // boxes of the pre-test records (the tables of the sparse tiles)
+    float4 *d_fbox = nullptr;
+    // positions the boxes were formed from (engine pos_gen; 0: unknown)
+    uint64_t pos_gen = 0;
   };
   std::vector<ChunkTable> tables;
+  // Generation of the particle positions (bumped by every binning, drift
+  // and grid change) and the one the pre-test records were formed from.
+  uint64_t pos_gen = 1, posrec_gen = 0;
   int loop_c[3] = {16, 16, 16};   // tile (i-chunk) capacity CI per loop
   int loop_cj[3] = {16, 16, 16};  // candidate chunk capacity CJ (work queue)
   unsigned int *d_hg2_all = nullptr;  // max (gamma h)^2 over all chunks
@@ -282,11 +289,11 @@ LoopArgs loop_args(const swsph_engine *e) {
 }
 
 // The batched loops (tiles, stencil grid, list walks) give each particle
-// its whole stencil in one wave: density and force sums start from the
-// identity (the gradient sums start from the ghost's v_sig and alpha).
-LoopArgs batched_args(const swsph_engine *e, LoopKind k) {
+// its whole stencil in one wave: the sums start from their initial values
+// in registers (L::fresh_acc) and the resets before them are skipped.
+LoopArgs batched_args(const swsph_engine *e, LoopKind) {
   LoopArgs la = loop_args(e);
-  la.fresh = k != LoopKind::Gradient ? 1 : 0;
+  la.fresh = 1;
   return la;
 }
 
@@ -306,6 +313,10 @@ GhostArgs ghost_args(const swsph_engine *e) {
   g.diff_alpha_max = e->props.diff_alpha_max;
   g.diff_alpha_min = e->props.diff_alpha_min;
   g.fresh_force = 0;
+  g.fresh_grad = 0;
+  g.lists = e->use_lists ? 1 : 0;
+  g.grad_rec = 0;
+  g.force_rec = 0;
   return g;
 }
 
@@ -355,7 +366,7 @@ void free_dev(swsph_engine *e) {
   e->saved_valid = false;
   for (auto &t : e->tables) {
     void *tp[] = {t.d_start, t.d_cell_chunk, t.d_chunk_cell, t.d_bbox,
-                  t.d_hg2max};
+                  t.d_hg2max, t.d_fbox};
     for (void *p : tp)
       if (p) (void)hipFree(p);
   }
@@ -376,13 +387,14 @@ swsph_engine::ChunkTable *chunk_table(swsph_engine *e, int C, int *rc) {
     if (tp->stale) {  // positions or radii changed while it was not in use
       const hipError_t err = sphk::launch_chunk_bounds(
           e->d, tp->d_start, 0, tp->nchunks, tp->C, tp->d_bbox, tp->d_hg2max,
-          e->d_hg2_all, e->st);
+          e->d_hg2_all, e->st, e->grid, pos_eps(e), tp->d_fbox);
       if (err != hipSuccess) {
         *rc = fail(2, "chunk bounds: %s", hipGetErrorString(err));
         return nullptr;
       }
       e->launches++;
       tp->stale = false;
+      tp->pos_gen = e->posrec_gen;
     }
     return tp;
   }
@@ -422,18 +434,21 @@ swsph_engine::ChunkTable *chunk_table(swsph_engine *e, int C, int *rc) {
       return nullptr;
     }
     void *tp2[] = {t.d_start, t.d_cell_chunk, t.d_chunk_cell, t.d_bbox,
-                   t.d_hg2max};
+                   t.d_hg2max, t.d_fbox};
     for (void *q : tp2)
       if (q) (void)hipFree(q);
     t.d_start = t.d_cell_chunk = t.d_chunk_cell = nullptr;
     t.d_bbox = nullptr;
     t.d_hg2max = nullptr;
+    t.d_fbox = nullptr;
     const size_t nk = (size_t)std::max(1, t.nchunks);
     if (dalloc(&t.d_start, nk + 1) != hipSuccess ||
         dalloc(&t.d_cell_chunk, (size_t)e->ncells + 1) != hipSuccess ||
         dalloc(&t.d_chunk_cell, nk) != hipSuccess ||
         dalloc(&t.d_bbox, 6 * nk) != hipSuccess ||
-        dalloc(&t.d_hg2max, nk) != hipSuccess) {
+        dalloc(&t.d_hg2max, nk) != hipSuccess ||
+        ((C == kSparseChunk || C == kSparseSub) &&
+         dalloc(&t.d_fbox, 2 * nk) != hipSuccess)) {
       *rc = fail(2, "allocation of chunk table failed");
       return nullptr;
     }
@@ -467,6 +482,7 @@ ChunkArgs chunk_args(const swsph_engine::ChunkTable *t) {
   ca.chunk_cell = t->d_chunk_cell;
   ca.bbox = t->d_bbox;
   ca.hg2max = t->d_hg2max;
+  ca.fbox = t->d_fbox;
   ca.nchunks = t->nchunks;
   ca.C = t->C;
   return ca;
@@ -524,9 +540,22 @@ int refresh_bounds(swsph_engine *e, int c_lo, int c_hi, bool pos,
   const int lo = e->cell_start[c_lo], hi = e->cell_start[c_hi];
   const bool all = c_lo == 0 && c_hi == e->ncells;
   if (all) HIPCHK(sphk::launch_zero_u32(e->d_hg2_all, 1, e->st));
-  if (pos) {
+  // Positions unchanged since the records and the boxes in use were formed
+  // (a pass moves no particle; only the ghost's radii change): radii only.
+  bool radius = all && pos && e->posrec_gen == e->pos_gen;
+  for (auto &t : e->tables) {
+    if (!radius) break;
+    bool u = &t == extra;
+    for (int k = 0; k < 3; k++)
+      u = u || t.C == table_c(e, k) || t.C == table2_c(e, k);
+    if (u && t.valid && !t.stale && t.pos_gen != e->pos_gen) radius = false;
+  }
+  bool records_done = !pos;
+  if (pos && !radius) {
     HIPCHK(sphk::launch_build_pos(e->d, e->grid, pos_eps(e), lo, hi, e->st));
     e->launches++;
+    e->posrec_gen = all ? e->pos_gen : 0;
+    records_done = true;
   }
   auto used = [&](const swsph_engine::ChunkTable &t) {
     bool u = &t == extra;
@@ -535,9 +564,18 @@ int refresh_bounds(swsph_engine *e, int c_lo, int c_hi, bool pos,
     return u;
   };
   auto refresh = [&](swsph_engine::ChunkTable &t) -> int {
-    HIPCHK(sphk::launch_chunk_bounds(e->d, t.d_start, t.cell_chunk_h[c_lo],
-                                     t.cell_chunk_h[c_hi], t.C, t.d_bbox,
-                                     t.d_hg2max, e->d_hg2_all, e->st));
+    if (radius) {
+      HIPCHK(sphk::launch_radius_bounds(
+          e->d, t.d_start, 0, t.nchunks, t.C, t.d_hg2max, e->d_hg2_all,
+          pos_eps(e), t.d_fbox, !records_done, e->st));
+      records_done = true;
+    } else {
+      HIPCHK(sphk::launch_chunk_bounds(e->d, t.d_start, t.cell_chunk_h[c_lo],
+                                       t.cell_chunk_h[c_hi], t.C, t.d_bbox,
+                                       t.d_hg2max, e->d_hg2_all, e->st,
+                                       e->grid, pos_eps(e), t.d_fbox));
+      t.pos_gen = all && e->posrec_gen == e->pos_gen ? e->pos_gen : 0;
+    }
     e->launches++;
     return 0;
   };
@@ -556,6 +594,10 @@ int refresh_bounds(swsph_engine *e, int c_lo, int c_hi, bool pos,
       if (int rc = refresh(t)) return rc;
       any = true;
     }
+  }
+  if (!records_done) {  // no table took the records along
+    HIPCHK(sphk::launch_build_pos(e->d, e->grid, pos_eps(e), lo, hi, e->st));
+    e->launches++;
   }
   return 0;
 }
@@ -1184,6 +1226,7 @@ int bin_sort(swsph_engine *e, const double *px, const double *py,
 // positions and the chunk tables of the loops.
 int finish_binning(swsph_engine *e) {
   int rc = 0;
+  e->pos_gen++;  // particles moved between slots
   e->cell_start.resize(e->ncells + 1);
   HIPCHK(hipMemcpyAsync(e->cell_start.data(), e->d_cell_start,
                         sizeof(int) * (e->ncells + 1), hipMemcpyDeviceToHost,
@@ -1295,6 +1338,7 @@ int drift_cells(swsph_engine *e, int ci, int64_t ti_old, int64_t ti_current) {
   a.lo = lo;
   a.hi = hi;
   HIPCHK(sphk::launch_drift(e->d, a, e->d_dmax, e->st));
+  e->pos_gen++;
   // every field a record holds may have changed
   for (LoopKind k : {LoopKind::Density, LoopKind::Gradient, LoopKind::Force})
     HIPCHK(sphk::launch_build_records(k, e->d, lo, hi, e->st));
@@ -1407,8 +1451,11 @@ const int *queue_list(const swsph_engine *e, const int **cnt) {
 int phase_ghost(swsph_engine *e) {
   const int *cnt;
   const int *list = queue_list(e, &cnt);
-  HIPCHK(sphk::launch_ghost(e->d, ghost_args(e), 0, (int)e->n, e->d_redo,
-                            e->st, list, cnt));
+  GhostArgs ga = ghost_args(e);
+  ga.fresh_grad = 1;  // the batched gradient loop starts its own sums
+  ga.grad_rec = 1;    // and reads the records the ghost writes
+  HIPCHK(sphk::launch_ghost(e->d, ga, 0, (int)e->n, e->d_redo, e->st, list,
+                            cnt));
   e->launches += 2;
   return 0;
 }
@@ -1439,12 +1486,9 @@ int phase_finish(swsph_engine *e) {
   int rc = 0;
   const int n = (int)e->n;
   const bool queue = e->schedule >= 1;
-  // the records of the inactive particles are current (ensure_records)
-  const int mab = e->props.max_active_bin;
-  // (the queue's list holds the particles of the last density round here,
-  // a subset of the active ones: the records sweep all slots)
-  HIPCHK(sphk::launch_build_records(LoopKind::Gradient, e->d, 0, n, e->st, mab));
-  e->launches++;
+  // the records of the inactive particles are current (ensure_records),
+  // the active ones' gradient records were written by the ghost as each
+  // particle converged
   if ((rc = refresh_bounds(e, 0, e->ncells))) return rc;
   if (e->use_lists) {
     if ((rc = list_loop(e, LoopKind::Gradient, false, 0))) return rc;
@@ -1461,10 +1505,9 @@ int phase_finish(swsph_engine *e) {
   const int *list = queue_list(e, &cnt);
   GhostArgs gx = ghost_args(e);
   gx.fresh_force = 1;
+  gx.force_rec = 1;  // the extra ghost writes the active force records
   HIPCHK(sphk::launch_extra_ghost(e->d, gx, 0, n, e->st, list, cnt));
-  HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, 0, n, e->st, mab,
-                                    list, cnt));
-  e->launches += 2;
+  e->launches++;
   if ((rc = queue ? queue_loop(e, LoopKind::Force, 0, nullptr, false)
                   : stencil(e, LoopKind::Force, false)))
     return rc;
@@ -2065,6 +2108,7 @@ int swsph_set_props(swsph_engine *e, const swsph_props *props) {
     grid = grid || props->ncell[k] != e->props.ncell[k] ||
            props->dim[k] != e->props.dim[k];
   e->props = *props;
+  e->pos_gen++;  // the pre-test records depend on the grid and pos_eps
   if (grid) {
     // the particles must be binned again (swsph_set_particles), which
     // reuses the device buffers when they are large enough

@@ -291,7 +291,9 @@ __device__ __forceinline__ void run_chunk(
     st = typename L::I{};
     st.x = st.y = st.z = kFar;
   }
-  if (act && s == 0 && !la.fresh)
+  if (act && s == 0 && la.fresh)
+    L::fresh_acc(st, acc);
+  else if (act && s == 0)
     L::load_acc(d, i, acc);
   else
     L::identity(acc);
@@ -742,7 +744,9 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_list(
   }
   if (!__any(act)) return;  // uniform
   typename L::Acc acc;
-  if (act && s == 0 && !la.fresh)
+  if (act && s == 0 && la.fresh)
+    L::fresh_acc(st, acc);
+  else if (act && s == 0)
     L::load_acc(d, i, acc);
   else
     L::identity(acc);

@@ -51,6 +51,10 @@ struct ChunkArgs {
   const int *chunk_cell;    // [nchunks] cell of each chunk
   const double *bbox;       // [6 nchunks] lo x,y,z, hi x,y,z
   const float *hg2max;      // [nchunks] max (gamma_K h)^2
+  // [2 nchunks] the box of the chunk's pre-test records (PosRec x, y, z
+  // relative to the cell corner) with their largest widened radius in the
+  // first .w, or null (tables the sparse tiles do not read)
+  const float4 *fbox;
   int nchunks;
   int C;                    // chunk capacity (8, 16, 32 or 64)
 };
@@ -161,8 +165,9 @@ struct LoopArgs {
   // (null: every cell) -- the cells of a bundle of cell tasks.
   const unsigned char *cmask;
   // 1: every particle a launch takes meets its whole 27-cell stencil in one
-  // wave, so its accumulators start from the identity and are stored, not
-  // loaded (the batched density and force loops; their resets are skipped).
+  // wave, so its accumulators start from their initial values in registers
+  // (L::fresh_acc) and are stored, not loaded (the batched loops; their
+  // resets are skipped).
   int fresh;
 };
 
@@ -201,6 +206,14 @@ struct GhostArgs {
   // 1: the force loop that follows starts its sums from the identity, so
   // the extra ghost leaves the force accumulators alone.
   int fresh_force;
+  // 1: the gradient loop that follows starts its sums from the particle's
+  // own v_sig and alpha, so the ghost leaves the gradient accumulators alone.
+  int fresh_grad;
+  // 1: neighbour lists are in use (the ghost keeps their fallback flags).
+  int lists;
+  // 1: the ghost writes the gradient record of each particle it finishes,
+  // the extra ghost the force record of each particle (no separate builds).
+  int grad_rec, force_rec;
 };
 
 enum class LoopKind { Density, Gradient, Force };
@@ -325,13 +338,23 @@ hipError_t launch_share_flags(int *dst, const int *src, int lo, int hi,
                               int off, int nt, hipStream_t st);
 hipError_t launch_zero_u64(unsigned long long *p, int n, hipStream_t st);
 // Bounding boxes and largest search radius of chunks [c0, c1); the largest
-// search radius over them is max-reduced into *hg2_all (float bits).
+// search radius over them is max-reduced into *hg2_all (float bits).  With
+// fbox, also the boxes of the chunks' pre-test records (grid g, pos_eps as
+// for launch_build_pos).
 hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
                                int c0, int c1, int C, double *bbox,
                                float *hg2max, unsigned int *hg2_all,
-                               hipStream_t st);
+                               hipStream_t st, const GridDesc &g, float pos_eps,
+                               float4 *fbox);
 // light: only the ghost's flags and brackets (the density loop that follows
 // starts its sums from the identity).
+// The radii of launch_chunk_bounds alone (positions unchanged since the
+// boxes were formed), and with `records` the radii of the candidates'
+// pre-test records (the table must cover every particle).
+hipError_t launch_radius_bounds(const DevParts &d, const int *chunk_start,
+                                int c0, int c1, int C, float *hg2max,
+                                unsigned int *hg2_all, float pos_eps,
+                                float4 *fbox, bool records, hipStream_t st);
 hipError_t launch_init_density(const DevParts &d, int lo, int hi,
                                int max_active_bin, float h_max, hipStream_t st,
                                const int *list = nullptr,

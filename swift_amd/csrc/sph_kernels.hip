@@ -37,14 +37,16 @@ __global__ void k_build_pos(DevParts d, GridDesc g, float pos_eps, int lo,
 constexpr int kBoundsBlock = 1024;
 __global__ __launch_bounds__(kBoundsBlock) void k_chunk_bounds(
     DevParts d, const int *chunk_start, int c0, int c1, int C, double *bbox,
-    float *hg2max, unsigned int *hg2_all) {
+    float *hg2max, unsigned int *hg2_all, GridDesc gd, float pos_eps,
+    float4 *fbox) {
   __shared__ float wmax[kBoundsBlock / 64];
   const int lane = threadIdx.x & 63;
   const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const int k = c0 + wave * (64 / C) + lane / C;
   const int g = lane % C;
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-  float hm = 0.f;
+  float flo[3] = {3e38f, 3e38f, 3e38f}, fhi[3] = {-3e38f, -3e38f, -3e38f};
+  float hm = 0.f, fh = -1.f;
   if (k < c1) {
     const int i = chunk_start[k] + g;
     if (i < chunk_start[k + 1] && d.tb[i] != kBinInhibited) {
@@ -52,6 +54,14 @@ __global__ __launch_bounds__(kBoundsBlock) void k_chunk_bounds(
       lo[1] = hi[1] = d.py[i];
       lo[2] = hi[2] = d.pz[i];
       hm = search_radius2(d.h[i]);
+      if (fbox) {
+        // the candidate's pre-test record, as k_build_pos forms it
+        const PosRec p = pos_rec(d, gd, pos_eps, i);
+        flo[0] = fhi[0] = p.x;
+        flo[1] = fhi[1] = p.y;
+        flo[2] = fhi[2] = p.z;
+        fh = p.hg2m;
+      }
     }
   }
   for (int off = 1; off < C; off <<= 1) {
@@ -60,6 +70,13 @@ __global__ __launch_bounds__(kBoundsBlock) void k_chunk_bounds(
       hi[a] = fmax(hi[a], __shfl_xor(hi[a], off));
     }
     hm = fmaxf(hm, __shfl_xor(hm, off));
+    if (fbox) {
+      for (int a = 0; a < 3; a++) {
+        flo[a] = fminf(flo[a], __shfl_xor(flo[a], off));
+        fhi[a] = fmaxf(fhi[a], __shfl_xor(fhi[a], off));
+      }
+      fh = fmaxf(fh, __shfl_xor(fh, off));
+    }
   }
   if (k < c1 && g == 0) {
     for (int a = 0; a < 3; a++) {
@@ -67,6 +84,10 @@ __global__ __launch_bounds__(kBoundsBlock) void k_chunk_bounds(
       bbox[6 * k + 3 + a] = hi[a];
     }
     hg2max[k] = hm;
+    if (fbox) {
+      fbox[2 * k] = make_float4(flo[0], flo[1], flo[2], fh);
+      fbox[2 * k + 1] = make_float4(fhi[0], fhi[1], fhi[2], 0.f);
+    }
   }
   for (int off = 32; off >= C; off >>= 1) hm = fmaxf(hm, __shfl_xor(hm, off));
   if (lane == 0) wmax[threadIdx.x >> 6] = hm;
@@ -74,6 +95,48 @@ __global__ __launch_bounds__(kBoundsBlock) void k_chunk_bounds(
   if (threadIdx.x == 0) {
     for (int k = 1; k < kBoundsBlock / 64; k++) hm = fmaxf(hm, wmax[k]);
     // positive floats order like their bit patterns
+    atomicMax(hg2_all, __float_as_uint(hm));
+  }
+}
+
+// The radius part of k_chunk_bounds for chunks whose positions have not
+// moved since their boxes were formed: the largest search radius (and, with
+// fbox, the largest widened radius in fbox[2 k].w) of chunks [c0, c1), and
+// with pos the widened radius of each member's pre-test record, as
+// k_build_pos forms it.
+__global__ __launch_bounds__(kBoundsBlock) void k_radius_bounds(
+    DevParts d, const int *chunk_start, int c0, int c1, int C, float *hg2max,
+    unsigned int *hg2_all, float pos_eps, float4 *fbox, PosRec *pos) {
+  __shared__ float wmax[kBoundsBlock / 64];
+  const int lane = threadIdx.x & 63;
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+  const int k = c0 + wave * (64 / C) + lane / C;
+  const int g = lane % C;
+  float hm = 0.f, fh = -1.f;
+  if (k < c1) {
+    const int i = chunk_start[k] + g;
+    if (i < chunk_start[k + 1]) {
+      const bool inh = d.tb[i] == kBinInhibited;
+      if (!inh) {
+        hm = search_radius2(d.h[i]);
+        fh = pretest_radius2(hm, pos_eps);
+      }
+      if (pos) pos[i].hg2m = inh ? -1.f : fh;
+    }
+  }
+  for (int off = 1; off < C; off <<= 1) {
+    hm = fmaxf(hm, __shfl_xor(hm, off));
+    fh = fmaxf(fh, __shfl_xor(fh, off));
+  }
+  if (k < c1 && g == 0) {
+    hg2max[k] = hm;
+    if (fbox) fbox[2 * k].w = fh;
+  }
+  for (int off = 32; off >= C; off >>= 1) hm = fmaxf(hm, __shfl_xor(hm, off));
+  if (lane == 0) wmax[threadIdx.x >> 6] = hm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k2 = 1; k2 < kBoundsBlock / 64; k2++) hm = fmaxf(hm, wmax[k2]);
     atomicMax(hg2_all, __float_as_uint(hm));
   }
 }
@@ -226,7 +289,8 @@ __device__ void prepare_gradient(const DevParts &d, int i, float h, float rho,
                                  const GhostArgs &ga) {
   const float curl_v = sqrtf(rx * rx + ry * ry + rz * rz);
   const float abs_div_v = fabsf(divv);
-  const float P = kGasGm1 * d.u[i] * rho;
+  const float u = d.u[i];
+  const float P = kGasGm1 * u * rho;
   const float c = sqrtf(kGasGamma * P / rho);
   const float bal = abs_div_v / (abs_div_v + curl_v + 0.0001f * c / h);
   const float common = h * kDimInv / wc;
@@ -241,8 +305,12 @@ __device__ void prepare_gradient(const DevParts &d, int i, float h, float rho,
   d.c[i] = c;
   d.f[i] = fgh;
   d.bal[i] = bal;
+  const float alpha = d.alpha[i];
+  // the gradient record, built here from the values at hand
+  if (ga.grad_rec) d.rg[i] = grad_rec_of(d, i, rho, u, c, alpha);
+  if (ga.fresh_grad) return;  // the gradient loop starts from these itself
   d.vsig[i] = 2.f * c;
-  d.avmax[i] = d.alpha[i];
+  d.avmax[i] = alpha;
   d.lapu[i] = 0.f;
   d.ng[i] = 0;
 }
@@ -323,7 +391,7 @@ __device__ bool ghost_one(const DevParts &d, const GhostArgs &ga, int i,
       d.roty[i] = 0.f;
       d.rotz[i] = 0.f;
       d.nd[i] = 0;
-      relist = !(search_radius2(hn) <= d.r2b[i]);
+      relist = !ga.lists || !(search_radius2(hn) <= d.r2b[i]);
       return true;
     }
     if (hn <= ga.h_min) {
@@ -346,8 +414,8 @@ __device__ bool ghost_one(const DevParts &d, const GhostArgs &ga, int i,
   // Converged.
   d.h[i] = h_fin;
   d.redo[i] = 0;
-  relist = !(search_radius2(h_fin) <= d.r2b[i]);
-  d.lfb[i] = relist ? 1 : 0;
+  relist = !ga.lists || !(search_radius2(h_fin) <= d.r2b[i]);
+  if (ga.lists) d.lfb[i] = relist ? 1 : 0;
   d.rho[i] = rho;
   d.rho_dh[i] = rho_dh;
   d.wc[i] = wc;
@@ -502,6 +570,10 @@ __global__ void k_extra_ghost(DevParts d, GhostArgs ga, int lo, int hi,
     na = fmaxf(na, ga.diff_alpha_min);
     const float lim = ga.diff_alpha_max * (1.f - d.avmax[i] / ga.alpha_max);
     d.adiff[i] = fminf(na, lim);
+    // the force record, built here from the values at hand
+    if (ga.force_rec)
+      d.rf[i] = force_rec_of(d, i, tb, h, rho, P, c, alpha, fminf(na, lim),
+                             d.u[i]);
     if (ga.fresh_force) return;
     d.ax[i] = 0.f;
     d.ay[i] = 0.f;
@@ -1267,13 +1339,28 @@ hipError_t launch_build_pos(const DevParts &d, const GridDesc &g, float pos_eps,
 hipError_t launch_chunk_bounds(const DevParts &d, const int *chunk_start,
                                int c0, int c1, int C, double *bbox,
                                float *hg2max, unsigned int *hg2_all,
-                               hipStream_t st) {
+                               hipStream_t st, const GridDesc &g, float pos_eps,
+                               float4 *fbox) {
   if (c1 <= c0) return hipSuccess;
   if (C != 8 && C != 16 && C != 32 && C != 64) return hipErrorInvalidValue;
   const int waves = nblk(c1 - c0, 64 / C);
   hipLaunchKernelGGL(k_chunk_bounds, dim3(nblk(waves, kBoundsBlock / 64)),
                      dim3(kBoundsBlock), 0, st, d, chunk_start, c0, c1, C, bbox,
-                     hg2max, hg2_all);
+                     hg2max, hg2_all, g, pos_eps, fbox);
+  return hipGetLastError();
+}
+
+hipError_t launch_radius_bounds(const DevParts &d, const int *chunk_start,
+                                int c0, int c1, int C, float *hg2max,
+                                unsigned int *hg2_all, float pos_eps,
+                                float4 *fbox, bool records, hipStream_t st) {
+  if (c1 <= c0) return hipSuccess;
+  if (C != 8 && C != 16 && C != 32 && C != 64) return hipErrorInvalidValue;
+  const int waves = nblk(c1 - c0, 64 / C);
+  hipLaunchKernelGGL(k_radius_bounds, dim3(nblk(waves, kBoundsBlock / 64)),
+                     dim3(kBoundsBlock), 0, st, d, chunk_start, c0, c1, C,
+                     hg2max, hg2_all, pos_eps, fbox,
+                     records ? (PosRec *)d.pos : nullptr);
   return hipGetLastError();
 }
 

@@ -16,6 +16,7 @@ constexpr int kOccForce = 5;
 constexpr int kOccForceTiles = 6;
 // The sparse-tile kernels (sph_sparse.h): one tile per wave, and the
 // persistent form (its loop around the tile costs registers).
+constexpr int kOccSparse = 6;  // density, gradient (their LDS allows 6)
 constexpr int kOccForceSparse = 5;
 constexpr int kOccSparseQueue = 5;
 
@@ -45,50 +46,71 @@ __device__ __forceinline__ DensRec dens_rec(const DevParts &d, int i) {
   return r;
 }
 
-__device__ __forceinline__ GradRec grad_rec(const DevParts &d, int i) {
-  const bool inh = d.tb[i] == kBinInhibited;
+// The gradient record of a particle (not inhibited) whose rho, u, c and
+// alpha the caller holds (the ghost, as it finishes the particle).
+__device__ __forceinline__ GradRec grad_rec_of(const DevParts &d, int i,
+                                               float rho, float u, float c,
+                                               float alpha) {
   GradRec r;
-  r.x = inh ? kFar : d.px[i];
-  r.y = inh ? kFar : d.py[i];
-  r.z = inh ? kFar : d.pz[i];
+  r.x = d.px[i];
+  r.y = d.py[i];
+  r.z = d.pz[i];
   r.vx = d.vx[i];
   r.vy = d.vy[i];
   r.vz = d.vz[i];
-  const float rho = d.rho[i];
   r.mrho = rho > 0.f ? d.m[i] / rho : 0.f;
-  r.u = d.u[i];
-  r.c = d.c[i];
-  r.alpha = d.alpha[i];
+  r.u = u;
+  r.c = c;
+  r.alpha = alpha;
   r.pad0 = 0.f;
   return r;
 }
 
-__device__ __forceinline__ ForceRec force_rec(const DevParts &d, int i) {
-  const int8_t tb = d.tb[i];
-  const bool inh = tb == kBinInhibited;
+__device__ __forceinline__ GradRec grad_rec(const DevParts &d, int i) {
+  GradRec r = grad_rec_of(d, i, d.rho[i], d.u[i], d.c[i], d.alpha[i]);
+  if (d.tb[i] == kBinInhibited) r.x = r.y = r.z = kFar;
+  return r;
+}
+
+// The force record of a particle (not inhibited) whose h, rho, P, c,
+// alpha, alpha_diff and u the caller holds (the extra ghost).
+__device__ __forceinline__ ForceRec force_rec_of(const DevParts &d, int i,
+                                                 int8_t tb, float h, float rho,
+                                                 float P, float c, float alpha,
+                                                 float adiff, float u) {
   ForceRec r;
-  r.x = inh ? kFar : d.px[i];
-  r.y = inh ? kFar : d.py[i];
-  r.z = inh ? kFar : d.pz[i];
-  const float h = d.h[i];
-  r.hg2 = inh ? -1.f : search_radius2(h);
+  r.x = d.px[i];
+  r.y = d.py[i];
+  r.z = d.pz[i];
+  r.hg2 = search_radius2(h);
   r.vx = d.vx[i];
   r.vy = d.vy[i];
   r.vz = d.vz[i];
   r.m = d.m[i];
   r.inv_m = 1.f / r.m;
   r.h_inv = 1.f / h;
-  r.rho = d.rho[i];
-  r.inv_rho = 1.f / r.rho;
-  r.P = d.P[i];
-  r.pr2 = r.P / (r.rho * r.rho);
+  r.rho = rho;
+  r.inv_rho = 1.f / rho;
+  r.P = P;
+  r.pr2 = P / (rho * rho);
   r.f = d.f[i];
   r.bal = d.bal[i];
-  r.c = d.c[i];
-  r.alpha = d.alpha[i];
-  r.Pad = r.P * d.adiff[i];
-  r.u = d.u[i];
+  r.c = c;
+  r.alpha = alpha;
+  r.Pad = P * adiff;
+  r.u = u;
   r.tb = (float)tb;
+  return r;
+}
+
+__device__ __forceinline__ ForceRec force_rec(const DevParts &d, int i) {
+  const int8_t tb = d.tb[i];
+  ForceRec r = force_rec_of(d, i, tb, d.h[i], d.rho[i], d.P[i], d.c[i],
+                            d.alpha[i], d.adiff[i], d.u[i]);
+  if (tb == kBinInhibited) {
+    r.x = r.y = r.z = kFar;
+    r.hg2 = -1.f;
+  }
   return r;
 }
 
@@ -123,7 +145,7 @@ struct DensityLoop {
   // occupancy target of the batched kernels (waves per SIMD)
   static constexpr int kWavesPerSimd = kOccDensity;
   static constexpr int kTileWavesPerSimd = kOccDensity;
-  static constexpr int kSparseWavesPerSimd = kOccDensity;
+  static constexpr int kSparseWavesPerSimd = kOccSparse;
   using Rec = DensRec;
   using Acc = DensAcc;
   using I = DensI;
@@ -164,6 +186,10 @@ struct DensityLoop {
   static __device__ __forceinline__ void identity(Acc &a) {
     a.rho = a.rho_dh = a.wc = a.wc_dh = a.divv = a.rx = a.ry = a.rz = 0.f;
     a.n = 0;
+  }
+  // hydro_init_part's zero sums (hydro.h:564-585)
+  static __device__ __forceinline__ void fresh_acc(const I &, Acc &a) {
+    identity(a);
   }
   // Sum the partial sums of the S streams of a particle.
   template <int S>
@@ -218,14 +244,14 @@ struct GradAcc {
 };
 struct GradI {
   double x, y, z;
-  float vx, vy, vz, h_inv, hg2, u, c;
+  float vx, vy, vz, h_inv, hg2, u, c, alpha;
 };
 
 struct GradientLoop {
   // occupancy target of the batched kernels (waves per SIMD)
   static constexpr int kWavesPerSimd = kOccGradient;
   static constexpr int kTileWavesPerSimd = kOccGradient;
-  static constexpr int kSparseWavesPerSimd = kOccGradient;
+  static constexpr int kSparseWavesPerSimd = kOccSparse;
   using Rec = GradRec;
   using Acc = GradAcc;
   using I = GradI;
@@ -241,6 +267,16 @@ struct GradientLoop {
     s.hg2 = search_radius2(h);
     s.u = d.u[i];
     s.c = d.c[i];
+    s.alpha = d.alpha[i];
+  }
+  // The sums of a particle whose stencil one wave covers: hydro_prepare_-
+  // gradient's initial v_sig = 2 c and alpha_visc_max_ngb = alpha
+  // (hydro.h:671-750), the rest zero.
+  static __device__ __forceinline__ void fresh_acc(const I &s, Acc &a) {
+    a.vsig = 2.f * s.c;
+    a.lapu = 0.f;
+    a.avmax = s.alpha;
+    a.n = 0;
   }
   static __device__ void load_acc(const DevParts &d, int i, Acc &a) {
     a.vsig = d.vsig[i];
@@ -358,6 +394,11 @@ struct ForceLoop {
     a.ax = a.ay = a.az = a.udt = a.hdt = 0.f;
     a.n = 0;
     a.mintb = kNumTimeBins + 1;
+  }
+  // hydro_reset_acceleration's sums and timestep_limiter_prepare_force's
+  // bin (hydro.h:897-912)
+  static __device__ __forceinline__ void fresh_acc(const I &, Acc &a) {
+    identity(a);
   }
   template <int S>
   static __device__ __forceinline__ void combine(Acc &a) {

@@ -20,6 +20,15 @@
 //      positions, radius widened by pretest_radius2), the hits appended to
 //      the particle's hit list in LDS in row order.
 //
+// Steps 1 to 3 run in FP32 on the candidates' pre-test records (positions
+// relative to their cell's corner): each chunk table carries the box of its
+// chunks' records and their largest widened radius, and each particle its
+// position relative to every neighbour image's corner, formed once per tile.
+// A box gap formed with the same roundings is never larger than the
+// pre-test distance of any member (float subtraction, products and FMAs of
+// non-negative terms are monotone), so a box test with the pre-test's radii
+// never rejects a candidate that the pre-test would pass.
+//
 // The 16 lanes of a particle then walk its hit list (lane s takes entries
 // s, s + 16, ...), evaluate the pair contract in the FP64 frame of the
 // hit's neighbour image and run the interaction body; the 16 partial sums
@@ -40,7 +49,7 @@ constexpr int kSpC2 = kSparseSub;      // sub-chunk capacity
 constexpr int kSpSub = kSpC1 / kSpC2;  // sub-chunks per chunk
 constexpr int kSpGroup = 64 / (kSpSub * kSpCI);  // chunks per pair test
 constexpr int kSpPairCap = 64;         // pairs per particle between drains
-constexpr int kSpHitCap = 96;          // hits per particle between walks
+constexpr int kSpHitCap = 80;          // hits per particle between walks
 static_assert(kSpGroup * kSpSub * kSpCI == 64, "one pair test per lane");
 // a row appends at most 64 hits to one list, a pair test at most
 // kSpGroup * kSpSub pairs per particle (two of each per step)
@@ -54,17 +63,29 @@ static_assert(kSpPairCap >= 2 * kSpGroup * kSpSub, "a step's pairs fit");
 template <class I>
 struct SparseLds {
   I ist[kSpCI];
-  double ttab[27 * 3];  // translation of each neighbour's image
-  double qorg[27 * 3];  // corner of each neighbour's image
   double fo[27 * 6];    // pair-contract frame {oi, oj} of each neighbour
+  // particle g relative to the corner of neighbour image q (FP32 pre-test
+  // frame), entry 4 q + g; and the tile's box in it (active particles)
+  float frx[27 * kSpCI], fry[27 * kSpCI], frz[27 * kSpCI];
+  float tlo[27 * 3], thi[27 * 3];
   int qk0[27], qk8[27], qpre[28], qj0[27], qj1[27];
   unsigned int plist[64];
   unsigned int pairs[kSpCI][kSpPairCap];
   unsigned int hits[kSpCI][kSpHitCap];
-  float irad[kSpCI];  // squared radius of the per-particle box test
-  float ipre[kSpCI];  // squared radius of the FP32 pre-test
+  float ipre[kSpCI];  // squared radius of the FP32 pre-test (-1: inactive)
   int ii[kSpCI];      // particle index
 };
+
+// Squared gap between two FP32 boxes (0 when they overlap), with the
+// roundings of the pre-test distance.
+__device__ __forceinline__ float box_gap2(float alx, float aly, float alz,
+                                          float ahx, float ahy, float ahz,
+                                          float4 blo, float4 bhi) {
+  const float gx = fmaxf(0.f, fmaxf(blo.x - ahx, alx - bhi.x));
+  const float gy = fmaxf(0.f, fmaxf(blo.y - ahy, aly - bhi.y));
+  const float gz = fmaxf(0.f, fmaxf(blo.z - ahz, alz - bhi.z));
+  return fmaf(gz, gz, fmaf(gy, gy, gx * gx));
+}
 
 // Count field g (8 bits) of a packed per-particle counter.
 __device__ __forceinline__ int pk_get(unsigned int p, int g) {
@@ -101,33 +122,31 @@ __device__ __forceinline__ void run_sparse(
   if (!__any(act)) return;  // uniform
   const int cell_i = uniform(tq.tile_cell[tl]);
   typename L::Acc acc;
-  if (act && s == 0 && !la.fresh)
-    L::load_acc(d, i, acc);
-  else
-    L::identity(acc);
+  L::identity(acc);
   // the particle's state: loaded by its stream-0 lane, kept in LDS
-  double px = kFar, py = kFar, pz = kFar;
-  float rt2 = 0.f;
+  float pre = -1.f;
   if (s == 0) {
     typename L::I st;
     if (act) {
       L::load_i(d, i, st);
+      if (la.fresh)
+        L::fresh_acc(st, acc);
+      else
+        L::load_acc(d, i, acc);
     } else {
       st = typename L::I{};
       st.x = st.y = st.z = kFar;
     }
     sh.ist[g] = st;
-    px = st.x;
-    py = st.y;
-    pz = st.z;
-    rt2 = act ? st.hg2 : 0.f;
-    sh.irad[g] = act ? rt2 : -1.f;
-    sh.ipre[g] = act ? pretest_radius2(rt2, la.pos_eps) : -1.f;
+    pre = act ? pretest_radius2(st.hg2, la.pos_eps) : -1.f;
+    sh.ipre[g] = pre;
     sh.ii[g] = act ? i : -1;
   }
-  const float hg2i = wave_max(rt2);
+  const float maxpre = wave_max(pre);
+  wave_sync();
 
-  // Neighbour table: lane q < 27 describes stencil neighbour q.
+  // Neighbour table: lane q < 27 describes stencil neighbour q, and forms
+  // the particles' pre-test frames in its image and the tile's box there.
   const double w[3] = {la.grid.dim[0] / la.grid.nc[0],
                        la.grid.dim[1] / la.grid.nc[1],
                        la.grid.dim[2] / la.grid.nc[2]};
@@ -143,9 +162,28 @@ __device__ __forceinline__ void run_sparse(
     int cj3[3];
     cell_coords(cj, la.grid, cj3[0], cj3[1], cj3[2]);
     const int o3[3] = {o / 9 - 1, (o / 3) % 3 - 1, o % 3 - 1};
+    double org[3];
+    for (int k = 0; k < 3; k++) org[k] = (double)cj3[k] * w[k] + t[k];
+    float lo[3] = {3e38f, 3e38f, 3e38f}, hi[3] = {-3e38f, -3e38f, -3e38f};
+    for (int gg = 0; gg < kSpCI; gg++) {
+      const float fx = (float)(sh.ist[gg].x - org[0]);
+      const float fy = (float)(sh.ist[gg].y - org[1]);
+      const float fz = (float)(sh.ist[gg].z - org[2]);
+      sh.frx[kSpCI * lane + gg] = fx;
+      sh.fry[kSpCI * lane + gg] = fy;
+      sh.frz[kSpCI * lane + gg] = fz;
+      if (sh.ipre[gg] >= 0.f) {
+        lo[0] = fminf(lo[0], fx);
+        lo[1] = fminf(lo[1], fy);
+        lo[2] = fminf(lo[2], fz);
+        hi[0] = fmaxf(hi[0], fx);
+        hi[1] = fmaxf(hi[1], fy);
+        hi[2] = fmaxf(hi[2], fz);
+      }
+    }
     for (int k = 0; k < 3; k++) {
-      sh.qorg[3 * lane + k] = (double)cj3[k] * w[k] + t[k];
-      sh.ttab[3 * lane + k] = t[k];
+      sh.tlo[3 * lane + k] = lo[k];
+      sh.thi[3 * lane + k] = hi[k];
     }
     double fo[6];
     qself = pair_frame(o3, ci3, cj3, t, w, fo);
@@ -167,27 +205,6 @@ __device__ __forceinline__ void run_sparse(
   }
   if (lane < 27) sh.qpre[lane] = incl - nchq;
   const int total = uniform(__shfl(incl, 26));
-  // the tile's box
-  // (lanes 0..3 hold the particles; the reduction over them reaches lane 0)
-  const bool pa = act && s == 0;
-  double ilx = pa ? px : 1e300, ily = pa ? py : 1e300, ilz = pa ? pz : 1e300;
-  double ihx = pa ? px : -1e300, ihy = pa ? py : -1e300,
-         ihz = pa ? pz : -1e300;
-#pragma unroll
-  for (int off = 1; off < kSpCI; off <<= 1) {
-    ilx = fmin(ilx, __shfl_xor(ilx, off));
-    ily = fmin(ily, __shfl_xor(ily, off));
-    ilz = fmin(ilz, __shfl_xor(ilz, off));
-    ihx = fmax(ihx, __shfl_xor(ihx, off));
-    ihy = fmax(ihy, __shfl_xor(ihy, off));
-    ihz = fmax(ihz, __shfl_xor(ihz, off));
-  }
-  ilx = uniform(ilx);
-  ily = uniform(ily);
-  ilz = uniform(ilz);
-  ihx = uniform(ihx);
-  ihy = uniform(ihy);
-  ihz = uniform(ihz);
   wave_sync();
 
   // The walk over the hit lists (all lanes; hcp holds the list lengths).
@@ -234,8 +251,6 @@ __device__ __forceinline__ void run_sparse(
     for (int gg = 0; gg < kSpCI; gg++) {
       const int np = pk_get(pcp, gg);
       if (np == 0) continue;  // uniform
-      const double xg = uniform(sh.ist[gg].x), yg = uniform(sh.ist[gg].y),
-                   zg = uniform(sh.ist[gg].z);
       const float pre = __int_as_float(
           __builtin_amdgcn_readfirstlane(__float_as_int(sh.ipre[gg])));
       const int ig = uniform(sh.ii[gg]);
@@ -250,9 +265,9 @@ __device__ __forceinline__ void run_sparse(
           const int j = c0 + (int)(e & kRowMask) + lane % kSpC2;
           if (j < sh.qj1[q]) {
             const PosRec pj = pos[j];
-            const float xs = (float)(xg - sh.qorg[3 * q]);
-            const float ys = (float)(yg - sh.qorg[3 * q + 1]);
-            const float zs = (float)(zg - sh.qorg[3 * q + 2]);
+            const float xs = sh.frx[kSpCI * q + gg];
+            const float ys = sh.fry[kSpCI * q + gg];
+            const float zs = sh.frz[kSpCI * q + gg];
             const float dx = xs - pj.x, dy = ys - pj.y, dz = zs - pj.z;
             const float r2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
             pass = L::kSymmetric ? (r2 < pre || r2 < pj.hg2m) : r2 < pre;
@@ -305,15 +320,12 @@ __device__ __forceinline__ void run_sparse(
       for (int b = 16; b > 0; b >>= 1)
         if (q + b < 27 && sh.qpre[q + b] <= f) q += b;
       const int kc = sh.qk0[q] + (f - sh.qpre[q]);
-      const double tx = sh.ttab[3 * q], ty = sh.ttab[3 * q + 1],
-                   tz = sh.ttab[3 * q + 2];
-      const double *jb = c1.bbox + 6 * kc;
-      const double gx = fmax(0.0, fmax(jb[0] + tx - ihx, ilx - (jb[3] + tx)));
-      const double gy = fmax(0.0, fmax(jb[1] + ty - ihy, ily - (jb[4] + ty)));
-      const double gz = fmax(0.0, fmax(jb[2] + tz - ihz, ilz - (jb[5] + tz)));
-      const float gap2 = (float)(gx * gx + gy * gy + gz * gz);
-      const float rad2 = L::kSymmetric ? fmaxf(hg2i, c1.hg2max[kc]) : hg2i;
-      pass = gap2 <= rad2 * 1.0001f;
+      const float4 blo = c1.fbox[2 * kc], bhi = c1.fbox[2 * kc + 1];
+      const float gap2 =
+          box_gap2(sh.tlo[3 * q], sh.tlo[3 * q + 1], sh.tlo[3 * q + 2],
+                   sh.thi[3 * q], sh.thi[3 * q + 1], sh.thi[3 * q + 2], blo,
+                   bhi);
+      pass = gap2 < (L::kSymmetric ? fmaxf(maxpre, blo.w) : maxpre);
       pe = ((unsigned int)q << 27) | (unsigned int)(kc - sh.qk0[q]);
     }
     const unsigned long long pm = __ballot(pass);
@@ -332,19 +344,14 @@ __device__ __forceinline__ void run_sparse(
         const int q = (int)(e >> 27);
         const int kin = (int)(e & kRowMask);  // chunk of 32 within the cell
         const int js = kin * kSpC1 + u * kSpC2;  // first particle, in-cell
-        const float rad = sh.irad[g];
-        if (sh.qj0[q] + js < sh.qj1[q] && rad >= 0.f) {
+        const float pg = sh.ipre[g];
+        if (sh.qj0[q] + js < sh.qj1[q] && pg >= 0.f) {
           const int k8 = sh.qk8[q] + kin * kSpSub + u;
-          const double tx = sh.ttab[3 * q], ty = sh.ttab[3 * q + 1],
-                       tz = sh.ttab[3 * q + 2];
-          const double *jb = c2.bbox + 6 * k8;
-          const double x = sh.ist[g].x, y = sh.ist[g].y, z = sh.ist[g].z;
-          const double gx = fmax(0.0, fmax(jb[0] + tx - x, x - (jb[3] + tx)));
-          const double gy = fmax(0.0, fmax(jb[1] + ty - y, y - (jb[4] + ty)));
-          const double gz = fmax(0.0, fmax(jb[2] + tz - z, z - (jb[5] + tz)));
-          const float gap2 = (float)(gx * gx + gy * gy + gz * gz);
-          const float rad2 = L::kSymmetric ? fmaxf(rad, c2.hg2max[k8]) : rad;
-          ok = gap2 <= rad2 * 1.0001f;
+          const float4 blo = c2.fbox[2 * k8], bhi = c2.fbox[2 * k8 + 1];
+          const float x = sh.frx[kSpCI * q + g], y = sh.fry[kSpCI * q + g],
+                      z = sh.frz[kSpCI * q + g];
+          const float gap2 = box_gap2(x, y, z, x, y, z, blo, bhi);
+          ok = gap2 < (L::kSymmetric ? fmaxf(pg, blo.w) : pg);
           ent = ((unsigned int)q << 27) | (unsigned int)js;
         }
       }
