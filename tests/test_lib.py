@@ -95,3 +95,42 @@ def test_c_abi_program_builds(tmp_path):
                     "-L" + os.path.join(root, "oracle"), "-lswsph", "-lorc",
                     "-lm"], check=True)
     assert exe.exists()
+
+
+def _header_int(path, name):
+    import re
+    m = re.search(r"constexpr int %s = (\d+);" % name, open(path).read())
+    assert m, name
+    return int(m.group(1))
+
+
+def test_tile_tables_hold_every_shape():
+    """The tile tables are sized n / kMinTile + ncells + 1: a cell of k
+    particles cuts into ceil(k / C) tiles, at most k / C + 1, so every
+    accepted tile capacity C >= kMinTile fits (the sparse tiles' list of
+    flagged tiles reuses an n-entry buffer: tiles are never empty).  Checked
+    against the shapes swsph_set_tile_shape accepts and, by brute force, on
+    skewed random cell occupancies."""
+    import os
+    import re
+    import numpy as np
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    kh = os.path.join(root, "swift_amd", "csrc", "sph_kernels.h")
+    kmin = _header_int(kh, "kMinTile")
+    src = open(os.path.join(root, "swift_amd", "csrc", "sph_engine.cpp")).read()
+    body = src[src.index("int swsph_set_tile_shape("):]
+    body = body[:body.index("if (!ok)")]
+    caps = {int(c) for c in re.findall(r"CI == (\d+)", body)}
+    assert caps and min(caps) >= kmin, caps
+    rng = np.random.default_rng(0)
+    for ncells in (1, 27, 1000):
+        for _ in range(20):
+            w = rng.pareto(0.7, ncells) + (rng.random(ncells) < 0.3) * 50
+            n = int(rng.integers(1, 20000))
+            k = np.floor(w / w.sum() * n).astype(np.int64)
+            k[0] += n - k.sum()
+            cap = n // kmin + ncells + 1
+            for C in sorted(caps):
+                tiles = int(((k + C - 1) // C).sum())
+                assert tiles + 1 <= cap, (ncells, n, C, tiles, cap)
+                assert tiles <= max(n, 1)
