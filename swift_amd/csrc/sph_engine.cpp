@@ -983,13 +983,16 @@ void all_neighbours(const swsph_engine *e, int ci, std::vector<int> &out) {
 // needing a new neighbour list, [2] converged without a usable list.
 int read_redo(swsph_engine *e, int *count, int *relist = nullptr,
               int *nolist = nullptr) {
-  HIPCHK(hipMemcpyAsync(e->h_redo, e->d_redo, 4 * sizeof(unsigned long long),
+  HIPCHK(hipMemcpyAsync(e->h_redo, e->d_redo,
+                        kGhostCounters * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost, e->st));
   HIPCHK(hipStreamSynchronize(e->st));
   *count = (int)e->h_redo[0];
   if (relist) *relist = (int)e->h_redo[1];
   if (nolist) *nolist = (int)e->h_redo[2];
-  e->density_work = (int64_t)e->h_redo[3];  // accumulated on the device
+  int64_t work = 0;  // accumulated on the device, over the slots
+  for (int k = 0; k < kGhostWorkSlots; k++) work += (int64_t)e->h_redo[3 + k];
+  e->density_work = work;
   return 0;
 }
 
@@ -1423,7 +1426,7 @@ int phase_start(swsph_engine *e) {
   HIPCHK(sphk::launch_init_density(e->d, 0, (int)e->n, e->props.max_active_bin,
                                    e->props.h_max, e->st, nullptr, nullptr,
                                    true));
-  HIPCHK(sphk::launch_zero_u64(e->d_redo + 3, 1, e->st));
+  HIPCHK(sphk::launch_zero_u64(e->d_redo + 3, kGhostWorkSlots, e->st));
   e->launches += 2;
   e->tiles_c = 0;  // activity may have changed since the last pass
   if (e->schedule >= 1)
@@ -1992,7 +1995,7 @@ int sched_pass_locked(swsph_engine *e, swsph_sched_stats *ss,
   e->density_work = 0;
   if ((rc = ensure_records(e))) return rc;
   e->last_pass_mab = e->props.max_active_bin;
-  HIPCHK(sphk::launch_zero_u64(e->d_redo + 3, 1, e->st));
+  HIPCHK(sphk::launch_zero_u64(e->d_redo + 3, kGhostWorkSlots, e->st));
   e->launches++;
   EngineTasks dev(e);
   int rounds = 0;
@@ -2037,9 +2040,11 @@ int swsph_engine_create(const swsph_props *props, swsph_engine **out) {
     delete e;
     return fail(2, "hipStreamCreate failed");
   }
-  if (hipMalloc((void **)&e->d_redo, 4 * sizeof(unsigned long long)) !=
+  if (hipMalloc((void **)&e->d_redo,
+                kGhostCounters * sizeof(unsigned long long)) !=
           hipSuccess ||
-      hipHostMalloc((void **)&e->h_redo, 4 * sizeof(unsigned long long), 0) !=
+      hipHostMalloc((void **)&e->h_redo,
+                    kGhostCounters * sizeof(unsigned long long), 0) !=
           hipSuccess ||
       hipMalloc((void **)&e->d_cnt, 3 * sizeof(unsigned long long)) != hipSuccess ||
       hipHostMalloc((void **)&e->h_cnt, 3 * sizeof(unsigned long long), 0) !=
@@ -2548,7 +2553,7 @@ int swsph_pass_plain(swsph_engine *e, swsph_pass_stats *stats) {
   e->density_work = 0;
   if ((rc = ensure_records(e))) return rc;
   e->last_pass_mab = e->props.max_active_bin;
-  HIPCHK(sphk::launch_zero_u64(e->d_redo + 3, 1, e->st));
+  HIPCHK(sphk::launch_zero_u64(e->d_redo + 3, kGhostWorkSlots, e->st));
   HIPCHK(sphk::launch_zero_u32(e->d_hg2_all, 1, e->st));
   for (int c = 0; c < e->ncells; c++) {
     HIPCHK(sphk::launch_init_density(e->d, e->cell_start[c], e->cell_start[c + 1],

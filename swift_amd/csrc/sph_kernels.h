@@ -131,6 +131,10 @@ constexpr int kListChunk = 64 / kListStreams;
 // Smallest tile (i-particles per wave) of the work-queue schedules; the
 // tile tables are sized for it.
 constexpr int kMinTile = 4;
+// Slots of the ghost's density-work counter (launch_ghost): the counter
+// block holds 3 + kGhostWorkSlots 64-bit counters.
+constexpr int kGhostWorkSlots = 32;
+constexpr int kGhostCounters = 3 + kGhostWorkSlots;
 // Sparse tiles (sph_sparse.h): tiles of kMinTile particles, candidates
 // culled by the chunks of kSparseChunk, then by their sub-chunks of
 // kSparseSub against each particle.
@@ -362,9 +366,12 @@ hipError_t launch_init_density(const DevParts &d, int lo, int hi,
 // counters[0]: particles sent back for another density round (their
 // indices are appended to d.sel); [1]: of those, the ones whose list no
 // longer covers the new radius; [2]: converged particles whose list does
-// not cover their final radius (flagged in d.lfb); [3]: density
-// interactions evaluated for the processed particles, accumulated over the
-// rounds (the pass zeroes it).  [0..2] are zeroed by each launch.  Over a
+// not cover their final radius (flagged in d.lfb; with lists only);
+// [3, 3 + kGhostWorkSlots): density interactions evaluated for the
+// processed particles, spread over the slots (one atomic per block on one
+// address serialised the launch) and accumulated over the rounds: their
+// sum is the count (the pass zeroes them).  [0..2] are zeroed by each
+// launch.  Over a
 // range, only the slots i with (i - lo) % nt == off are processed (one of
 // nt ghost tasks of a cell, runner_do_ghost's offset / ntasks).
 hipError_t launch_ghost(const DevParts &d, const GhostArgs &ga, int lo, int hi,

@@ -95,7 +95,11 @@ __global__ __launch_bounds__(kBoundsBlock) void k_chunk_bounds(
   if (threadIdx.x == 0) {
     for (int k = 1; k < kBoundsBlock / 64; k++) hm = fmaxf(hm, wmax[k]);
     // positive floats order like their bit patterns
-    atomicMax(hg2_all, __float_as_uint(hm));
+    // a plain read first: only a block that raises the maximum pays an
+    // atomic (one per block on one address serialised the whole launch)
+    if (__float_as_uint(hm) >
+        __hip_atomic_load(hg2_all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(hg2_all, __float_as_uint(hm));
   }
 }
 
@@ -137,7 +141,11 @@ __global__ __launch_bounds__(kBoundsBlock) void k_radius_bounds(
   __syncthreads();
   if (threadIdx.x == 0) {
     for (int k2 = 1; k2 < kBoundsBlock / 64; k2++) hm = fmaxf(hm, wmax[k2]);
-    atomicMax(hg2_all, __float_as_uint(hm));
+    // a plain read first: only a block that raises the maximum pays an
+    // atomic (one per block on one address serialised the whole launch)
+    if (__float_as_uint(hm) >
+        __hip_atomic_load(hg2_all, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(hg2_all, __float_as_uint(hm));
   }
 }
 
@@ -481,7 +489,7 @@ __global__ __launch_bounds__(kGhostBlock) void k_ghost(
     const bool ag = (again >> u) & 1u, rl = (relist >> u) & 1u;
     na += __popcll(__ballot(ag));
     nb += __popcll(__ballot(ag && rl));
-    nc += __popcll(__ballot(!ag && rl));
+    nc += ga.lists ? __popcll(__ballot(!ag && rl)) : 0;
   }
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   if (lane == 0) {
@@ -500,7 +508,7 @@ __global__ __launch_bounds__(kGhostBlock) void k_ghost(
   } else if (threadIdx.x == 3) {
     unsigned long long t = 0;
     for (int k = 0; k < W; k++) t += wwork[k];
-    if (t) atomicAdd(&counters[3], t);
+    if (t) atomicAdd(&counters[3 + blockIdx.x % kGhostWorkSlots], t);
   }
   __syncthreads();
   if (__any(again != 0u)) {

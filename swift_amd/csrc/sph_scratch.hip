@@ -33,7 +33,7 @@ constexpr size_t scratch_bytes(size_t n, size_t nc) {
          + 2 * align256(4 * tcap)          // tile_start, tile_cell
          + align256(4)                     // ntiles
          + align256(4 * 8 * 64)            // heads
-         + align256(8 * 4)                 // ghost counters
+         + align256(8 * kGhostCounters)    // ghost counters
          + align256(8 * 3);                // interaction counters
 }
 
@@ -101,7 +101,7 @@ hipError_t noalloc_scratch(int device, int64_t n, int ncells, TileScratch *ts,
   ts->tile_cell = (int *)take(4 * tcap);
   ts->ntiles = (int *)take(4);
   ts->heads = (int *)take(4 * 8 * 64);
-  *redo = (unsigned long long *)take(8 * 4);
+  *redo = (unsigned long long *)take(8 * kGhostCounters);
   *cnt = (unsigned long long *)take(8 * 3);
   if (off != scratch_bytes(nn, nc) || off > kScratchBytes)
     return hipErrorInvalidValue;

@@ -274,7 +274,12 @@ __global__ __launch_bounds__(kBlock) void k_drift(DevParts d, DriftArgs a,
   if (threadIdx.x == 0) {
     float m = red[0];
     for (int k = 1; k < kBlock / 64; k++) m = fmaxf(m, red[k]);
-    if (m > 0.f) atomicMax(dmax, __float_as_uint(m));
+    // a plain read first: only a block that raises the maximum pays an
+    // atomic (one per block on one address serialised large drifts)
+    if (m > 0.f &&
+        __float_as_uint(m) >
+            __hip_atomic_load(dmax, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+      atomicMax(dmax, __float_as_uint(m));
   }
 }
 
