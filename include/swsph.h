@@ -356,6 +356,16 @@ int swsph_set_step_sched(swsph_engine *e, int on);
  * CJ (default (4, 32) below 16 per cell; per_cell = 0 turns it off).  No
  * counterpart in the reference; speed only, as swsph_set_tile_shape. */
 int swsph_set_sparse_shape(swsph_engine *e, int CI, int CJ, int per_cell);
+
+/* In-cell particle order: 0 (default) Morton order inside each top-level
+ * cell; 1 by time bin first (shortest steps first), then Morton order, so
+ * that the particles active at any step form a prefix of every cell and the
+ * per-particle kernels touch contiguous runs of their fields.  Resident
+ * particles are re-binned at once (saved states and captured graphs are
+ * dropped); later binnings keep the order, which time-step changes between
+ * binnings only loosen.  No counterpart in the reference; speed only (the
+ * results differ by FP32 summation order at most). */
+int swsph_set_bin_order(swsph_engine *e, int on);
 int swsph_get_tile_shape(swsph_engine *e, int which, int *CI, int *CJ);
 
 /* Schedule of the batched pass's neighbour loops.  mode 0: one wave per

@@ -159,6 +159,10 @@ struct swsph_engine {
   int *d_val[2] = {nullptr, nullptr};
   void *d_sort_tmp = nullptr;
   size_t sort_tmp_bytes = 0;
+  // In-cell order by time bin, then Morton (swsph_set_bin_order), and the
+  // key layout of the current binning.
+  bool bin_order = false;
+  int key_bin_bits = 0;
   double *d_stage = nullptr;
   // Time integration: parameters, the integer time line, and the largest
   // distance by which a particle lies outside its cell since the last
@@ -1204,22 +1208,29 @@ int fitting(swsph_engine *e, swsph_pass_stats *stats, F pass) {
 // Keys, stable radix sort and cell bounds of the current n positions
 // (component k of particle i at p_k[stride * i]); the sort's values are the
 // positions' indices, written to vals_out in the new order.
+// tb: the particles' time bins in the positions' index space (used when the
+// engine orders cells by time bin).
 int bin_sort(swsph_engine *e, const double *px, const double *py,
-             const double *pz, int stride, int *vals_out) {
+             const double *pz, int stride, int *vals_out, const int8_t *tb) {
   const int ni = (int)e->n;
   double inv_w[3];
   // iwidth as space_regrid sets it (space_regrid.c:221-222)
   for (int k = 0; k < 3; k++)
     inv_w[k] = 1.0 / (e->grid.dim[k] / e->grid.nc[k]);
+  const bool by_bin = e->bin_order && tb;
+  e->key_bin_bits = by_bin ? sphk::kBinBits : 0;
   HIPCHK(sphk::launch_bin_keys(px, py, pz, stride, e->grid, inv_w, ni,
-                               e->d_key[0], e->d_val[0], e->st));
-  int bits = 30;
-  while ((1LL << (bits - 30)) < (long long)e->ncells) bits++;
+                               e->d_key[0], e->d_val[0], e->st,
+                               by_bin ? tb : nullptr));
+  const int low = 30 + e->key_bin_bits;
+  int bits = low;
+  while ((1LL << (bits - low)) < (long long)e->ncells) bits++;
   size_t bytes = e->sort_tmp_bytes;
   HIPCHK(sphk::sort_pairs(e->d_sort_tmp, bytes, e->d_key[0], e->d_key[1],
                           e->d_val[0], vals_out, ni, bits, e->st));
   HIPCHK(sphk::launch_cell_bounds(e->d_key[1], ni, e->ncells,
-                                  e->d_cell_start, e->d.cell, e->st));
+                                  e->d_cell_start, e->d.cell, e->st,
+                                  e->key_bin_bits));
   e->launches += 4;
   return 0;
 }
@@ -1260,7 +1271,7 @@ int rebin(swsph_engine *e) {
   int rc = 0;
   HIPCHK(sphk::launch_wrap(d, e->grid, ni, e->st));
   int *idx = e->d_val[1];
-  if ((rc = bin_sort(e, d.px, d.py, d.pz, 1, idx))) return rc;
+  if ((rc = bin_sort(e, d.px, d.py, d.pz, 1, idx, d.tb))) return rc;
   auto carry = [&](auto *field) -> int {
     using T = typename std::remove_pointer<decltype(field)>::type;
     T *tmp = (T *)e->d_stage;
@@ -2167,8 +2178,13 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
     // caller's indices, i.e. the permutation sorted slot -> caller index.
     HIPCHK(hipMemcpyAsync(e->d_stage, p->x, sizeof(double) * 3 * n,
                           hipMemcpyHostToDevice, e->st));
+    // the time bins in the caller's order for the keys (d.tb is written in
+    // slot order from the staging buffer below)
+    if (e->bin_order)
+      HIPCHK(hipMemcpyAsync(d.tb, p->time_bin, n, hipMemcpyHostToDevice,
+                            e->st));
     if ((rc = bin_sort(e, e->d_stage, e->d_stage + 1, e->d_stage + 2, 3,
-                       d.perm)))
+                       d.perm, e->bin_order ? d.tb : nullptr)))
       return rc;
     double *pd[3] = {d.px, d.py, d.pz};
     for (int k = 0; k < 3; k++)
@@ -2222,7 +2238,7 @@ int swsph_set_particles(swsph_engine *e, const swsph_host_parts *p) {
                                e->st));
   } else {
     HIPCHK(sphk::launch_cell_bounds(e->d_key[1], 0, e->ncells,
-                                    e->d_cell_start, d.cell, e->st));
+                                    e->d_cell_start, d.cell, e->st, 0));
   }
   if ((rc = finish_binning(e))) return rc;
   return check_fits(e);
@@ -2809,6 +2825,20 @@ int swsph_set_tile_shape(swsph_engine *e, int which, int CI, int CJ) {
   e->loop_cj[which] = CJ;
   prune_tables(e);
   if (!e->cell_start.empty()) return ensure_tables(e, which);
+  return 0;
+}
+
+int swsph_set_bin_order(swsph_engine *e, int on) {
+  if (!e) return fail(1, "null argument");
+  std::lock_guard<std::mutex> lk(e->mtx);
+  if (e->bin_order == (on != 0)) return 0;
+  e->bin_order = on != 0;
+  e->gen++;  // captured graphs no longer match
+  // resident particles are re-binned in the new order now
+  if (e->n > 0 && !e->need_particles) {
+    e->tiles_c = 0;
+    return rebin(e);
+  }
   return 0;
 }
 

@@ -412,11 +412,13 @@ hipError_t launch_build_records(LoopKind kind, const DevParts &d, int lo,
 
 // ---- binning and time integration (sph_step.hip) ----
 // Keys (cell << 30 | in-cell Morton code) of n positions, component k of
-// particle i at p_k[stride * i]; val[i] = i.
+// particle i at p_k[stride * i]; val[i] = i.  With tb (the particles' time
+// bins, same index), (cell << 30 + kBinBits) | (bin << 30) | Morton.
+constexpr int kBinBits = 6;
 hipError_t launch_bin_keys(const double *px, const double *py, const double *pz,
                            int stride, const GridDesc &g, const double inv_w[3],
                            int n, unsigned long long *key, int *val,
-                           hipStream_t st);
+                           hipStream_t st, const int8_t *tb = nullptr);
 // Stable radix sort of (key, value) pairs on the low `bits` key bits; with
 // temp == nullptr only sets `bytes` (rocprim::radix_sort_pairs).
 hipError_t sort_pairs(void *temp, size_t &bytes, const unsigned long long *kin,
@@ -424,9 +426,11 @@ hipError_t sort_pairs(void *temp, size_t &bytes, const unsigned long long *kin,
                       int n, int bits, hipStream_t st);
 hipError_t launch_wrap(const DevParts &d, const GridDesc &g, int n,
                        hipStream_t st);
-// cell[k] and cell_start[0..ncells] from sorted keys.
+// cell[k] and cell_start[0..ncells] from sorted keys (bin_bits: 0, or
+// kBinBits for keys with the time bin).
 hipError_t launch_cell_bounds(const unsigned long long *key, int n, int ncells,
-                              int *cell_start, int *cell, hipStream_t st);
+                              int *cell_start, int *cell, hipStream_t st,
+                              int bin_bits = 0);
 // dst[k] = src[stride * idx[k] + comp]
 template <class T>
 hipError_t launch_gather(T *dst, const T *src, int stride, int comp,

@@ -8,7 +8,11 @@
 // the highest bit of each triple).  A stable LSD radix sort (rocPRIM) of the
 // keys with the current slot as value gives the new order; cells start
 // where their key range starts.  With the caller's index as the value this
-// is exactly the host rule "sort by (cell, morton, caller index)".
+// is exactly the host rule "sort by (cell, morton, caller index)".  With
+// the time-bin order (kBinBits > 0) the key is (cell << 36) | (time bin <<
+// 30) | morton: inside a cell the particles of the shortest steps come
+// first, so the particles active at any step form a prefix of each cell
+// (their fields are contiguous runs instead of one line per particle).
 //
 // Time integration restates, for gas without gravity and without
 // cosmology:
@@ -64,7 +68,8 @@ __device__ __forceinline__ unsigned long long spread10(unsigned int v) {
 __global__ void k_bin_keys(const double *px, const double *py,
                            const double *pz, int stride, GridDesc g,
                            double iw0, double iw1, double iw2, int n,
-                           unsigned long long *key, int *val) {
+                           unsigned long long *key, int *val,
+                           const int8_t *tb, int bin_bits) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -85,7 +90,12 @@ __global__ void k_bin_keys(const double *px, const double *py,
   }
   const unsigned long long cid =
       (unsigned long long)((c[0] * g.nc[1] + c[1]) * g.nc[2] + c[2]);
-  key[i] = (cid << 30) | m;
+  unsigned long long b = 0ull;
+  if (bin_bits) {  // inhibited particles (and any bin past the field) last
+    const int t = tb[i];
+    b = (unsigned long long)(t < 0 ? 0 : (t > 63 ? 63 : t));
+  }
+  key[i] = (cid << (30 + bin_bits)) | (b << 30) | m;
   val[i] = i;
 }
 
@@ -110,16 +120,17 @@ __global__ void k_wrap(DevParts d, GridDesc g, int n) {
 
 // Cell of every sorted slot, and the first slot of every cell (cells
 // without particles start where the next non-empty one does).
-__global__ void k_cell_of(const unsigned long long *key, int n, int *cell) {
+__global__ void k_cell_of(const unsigned long long *key, int n, int *cell,
+                          int shift) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) cell[k] = (int)(key[k] >> 30);
+  if (k < n) cell[k] = (int)(key[k] >> shift);
 }
 
 __global__ void k_cell_start(const unsigned long long *key, int n, int ncells,
-                             int *cs) {
+                             int *cs, int shift) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c > ncells) return;
-  const unsigned long long t = (unsigned long long)c << 30;
+  const unsigned long long t = (unsigned long long)c << shift;
   int lo = 0, hi = n;  // first slot with key >= t
   while (lo < hi) {
     const int mid = (lo + hi) >> 1;
@@ -456,10 +467,11 @@ namespace sphk {
 hipError_t launch_bin_keys(const double *px, const double *py, const double *pz,
                            int stride, const GridDesc &g, const double inv_w[3],
                            int n, unsigned long long *key, int *val,
-                           hipStream_t st) {
+                           hipStream_t st, const int8_t *tb) {
   if (n <= 0) return hipSuccess;
   k_bin_keys<<<blocks(n), kBlock, 0, st>>>(px, py, pz, stride, g, inv_w[0],
-                                           inv_w[1], inv_w[2], n, key, val);
+                                           inv_w[1], inv_w[2], n, key, val, tb,
+                                           tb ? kBinBits : 0);
   return hipGetLastError();
 }
 
@@ -478,10 +490,12 @@ hipError_t launch_wrap(const DevParts &d, const GridDesc &g, int n,
 }
 
 hipError_t launch_cell_bounds(const unsigned long long *key, int n, int ncells,
-                              int *cell_start, int *cell, hipStream_t st) {
-  if (n > 0) k_cell_of<<<blocks(n), kBlock, 0, st>>>(key, n, cell);
+                              int *cell_start, int *cell, hipStream_t st,
+                              int bin_bits) {
+  const int shift = 30 + bin_bits;
+  if (n > 0) k_cell_of<<<blocks(n), kBlock, 0, st>>>(key, n, cell, shift);
   k_cell_start<<<blocks(ncells + 1), kBlock, 0, st>>>(key, n, ncells,
-                                                      cell_start);
+                                                      cell_start, shift);
   return hipGetLastError();
 }
 

@@ -394,6 +394,12 @@ class Engine:
         check(lib().swsph_set_tile_shape(self._h, self.LOOPS[loop], int(CI),
                                          int(CJ)), "swsph_set_tile_shape")
 
+    def set_bin_order(self, on: bool) -> None:
+        """Order particles inside each cell by time bin, then Morton code
+        (the active particles of a step become a prefix of every cell)."""
+        check(lib().swsph_set_bin_order(self._h, int(bool(on))),
+              "swsph_set_bin_order")
+
     def set_sparse_shape(self, CI: int, CJ: int, per_cell: int) -> None:
         """Tile shape of step() passes with fewer than per_cell active
         particles per top-level cell (0 turns it off)."""
