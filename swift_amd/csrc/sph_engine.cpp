@@ -800,11 +800,15 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
   if (sparse && !t1) return rc;
   const LoopArgs la = batched_args(e, k);
   TileArgs ta = tile_args(e);
-  // The all-active table is kept for the later loops of the pass, and the
-  // flagged rounds (mode 1) take the subset of its tiles that hold a
-  // flagged particle, with the others masked out, instead of a new table
-  // (a build sweeps and scans every slot; this sweeps the tiles).
-  const bool subset = mode == 1 && e->tiles_c == C && !build && !la.cmask;
+  // The all-active table is kept for the later loops of the pass.  With
+  // the narrowest tiles the flagged rounds (mode 1) take the subset of its
+  // tiles that hold a flagged particle, with the others masked out, instead
+  // of a new table (a build sweeps and scans every slot; this sweeps the
+  // tiles).  Wider tiles rebuild: the flagged particles of a dense box are
+  // scattered over nearly every wide tile, which would run mostly masked
+  // (+20 % on the 128^3 box's re-run rounds).
+  const bool subset = mode == 1 && e->tiles_c == C && C == kMinTile &&
+                      !build && !la.cmask;
   if (subset) {
     ta.ids = e->ts.flag;    // free until the next build (tiles <= particles)
     ta.ntiles = e->ts.blk;  // the subset's count
@@ -1519,9 +1523,20 @@ int phase_finish(swsph_engine *e) {
   const int *list = queue_list(e, &cnt);
   GhostArgs gx = ghost_args(e);
   gx.fresh_force = 1;
-  gx.force_rec = 1;  // the extra ghost writes the active force records
+  // With the narrow tiles of sparse activity the extra ghost writes the
+  // active force records as it goes; otherwise (most particles active) the
+  // builder's wave-staged contiguous stores are faster than one scattered
+  // 96-byte record per thread.
+  const bool narrow = e->schedule >= 1 &&
+                      e->loop_c[(int)LoopKind::Force] == kMinTile;
+  gx.force_rec = narrow ? 1 : 0;
   HIPCHK(sphk::launch_extra_ghost(e->d, gx, 0, n, e->st, list, cnt));
   e->launches++;
+  if (!narrow) {
+    HIPCHK(sphk::launch_build_records(LoopKind::Force, e->d, 0, n, e->st,
+                                      e->props.max_active_bin, list, cnt));
+    e->launches++;
+  }
   if ((rc = queue ? queue_loop(e, LoopKind::Force, 0, nullptr, false)
                   : stencil(e, LoopKind::Force, false)))
     return rc;

@@ -642,7 +642,7 @@ __global__ __launch_bounds__(64) void k_loop_task(
 // dry, so every wave exits.
 constexpr int kGrab = 2;
 template <class L, int CI, int CJ, bool BUILD>
-__global__ SWSPH_LOOP_BOUNDS(L) void k_loop_queue(
+__global__ __launch_bounds__(64, kOccSparseQueue) void k_loop_queue(
     DevParts d, LoopArgs la, ChunkArgs ca,
     const typename L::Rec *__restrict__ rec, const PosRec *__restrict__ pos,
     NbrList nl, TileArgs tq) {
@@ -653,11 +653,15 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_queue(
     const int hi = min(nt, (q + 1) * per);
     int base = 0;
     if ((threadIdx.x & 63) == 0) {
-      // a plain read first: an exhausted queue costs no atomic (the tail
-      // of a drain, and every wave of a grid for a nearly empty queue)
-      base = __hip_atomic_load(&tq.heads[64 * q], __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-      if (q * per + base < hi) base = atomicAdd(&tq.heads[64 * q], kGrab);
+      // Once its own queue is exhausted, a wave reads the others' heads
+      // before it tries to steal: an exhausted queue then costs no atomic
+      // (the tail of a drain, and every wave of a grid for a nearly empty
+      // queue); its own queue takes the atomic directly.
+      base = tries > 0 ? __hip_atomic_load(&tq.heads[64 * q], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_AGENT)
+                       : 0;
+      if (tries == 0 || q * per + base < hi)
+        base = atomicAdd(&tq.heads[64 * q], kGrab);
     }
     base = uniform(__shfl(base, 0));
     const int lo = q * per + base;

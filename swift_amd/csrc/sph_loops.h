@@ -14,10 +14,11 @@ constexpr int kOccDensity = 7;
 constexpr int kOccGradient = 7;
 constexpr int kOccForce = 5;
 constexpr int kOccForceTiles = 6;
-// The sparse-tile kernels (sph_sparse.h): one tile per wave, and the
-// persistent form (its loop around the tile costs registers).
+// The sparse-tile kernels (sph_sparse.h): one tile per wave.
 constexpr int kOccSparse = 6;  // density, gradient (their LDS allows 6)
 constexpr int kOccForceSparse = 5;
+// The persistent forms of both engines: their loop around the tile costs
+// registers (at 7 waves per SIMD the density form spilled 34 VGPRs).
 constexpr int kOccSparseQueue = 5;
 
 namespace sphl {
