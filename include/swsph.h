@@ -353,7 +353,7 @@ int swsph_set_step_sched(swsph_engine *e, int on);
 /* Tile shape of the passes of swsph_engine_step when its active particles
  * number fewer than per_cell per top-level cell on average (work-queue
  * schedules only): all three loops then use tiles of CI against chunks of
- * CJ (default (4, 32) below 16 per cell; per_cell = 0 turns it off).  No
+ * CJ (default (4, 8) below 16 per cell; per_cell = 0 turns it off).  No
  * counterpart in the reference; speed only, as swsph_set_tile_shape. */
 int swsph_set_sparse_shape(swsph_engine *e, int CI, int CJ, int per_cell);
 

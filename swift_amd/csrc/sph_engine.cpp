@@ -129,7 +129,7 @@ struct swsph_engine {
   // Steps with few active particles per cell (fewer than sparse_per_cell on
   // average) run their pass with tiles of sparse_ci against chunks of
   // sparse_cj in all three loops; 0 disables.
-  int sparse_ci = 4, sparse_cj = 32, sparse_per_cell = 16;
+  int sparse_ci = 4, sparse_cj = 8, sparse_per_cell = 16;
   // the ghost / extra ghost / force records / end force of a pass sweep the
   // work queue's list instead of every slot (set by engine_step when under
   // half of the particles are active; an all-active pass sweeps slots,
