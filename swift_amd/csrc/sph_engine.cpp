@@ -163,6 +163,9 @@ struct swsph_engine {
   // key layout of the current binning.
   bool bin_order = false;
   int key_bin_bits = 0;
+  // record slot of each field slot and back, with the time-bin order (set
+  // into d.cand / d.soa only while the binning uses that order)
+  int *d_cand_buf = nullptr, *d_soa_buf = nullptr;
   double *d_stage = nullptr;
   // Time integration: parameters, the integer time line, and the largest
   // distance by which a particle lies outside its cell since the last
@@ -341,9 +344,11 @@ void free_dev(swsph_engine *e) {
                   e->d.ufull, e->d.perm, e->d_key[0], e->d_key[1],
                   e->d_val[0], e->d_val[1], e->d_sort_tmp, e->d_stage,
                   e->d_subflag, e->d_inv, e->d_subind, e->d_suberr,
-                  e->d_cmask, e->d_cact, e->d_clist};
+                  e->d_cmask, e->d_cact, e->d_clist, e->d_cand_buf,
+                  e->d_soa_buf};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
+  e->d_cand_buf = e->d_soa_buf = nullptr;
   e->d_subflag = e->d_inv = e->d_subind = nullptr;
   e->d_suberr = nullptr;
   e->sub_cap = e->subind_cap = 0;
@@ -630,6 +635,8 @@ int alloc_dev(swsph_engine *e, size_t n) {
   HIPCHK(dalloc(&d.rg, n));
   HIPCHK(dalloc(&d.rf, n));
   HIPCHK(dalloc(&d.pos, n));
+  HIPCHK(dalloc(&e->d_cand_buf, n));
+  HIPCHK(dalloc(&e->d_soa_buf, n));
   {
     TileScratch &t = e->ts;
     const size_t nc = (size_t)e->ncells;
@@ -1245,6 +1252,28 @@ int bin_sort(swsph_engine *e, const double *px, const double *py,
 int finish_binning(swsph_engine *e) {
   int rc = 0;
   e->pos_gen++;  // particles moved between slots
+  e->d.cand = e->d.soa = nullptr;
+  if (e->key_bin_bits && e->n > 0) {
+    // Fields are in (cell, time bin, Morton) order; the records and chunk
+    // tables take (cell, Morton) order: a stable sort of the field slots by
+    // the key without the time bin gives the field slot of every record
+    // slot, and its inverse the record slot of every field slot.
+    const int ni = (int)e->n;
+    double inv_w[3];
+    for (int k = 0; k < 3; k++)
+      inv_w[k] = 1.0 / (e->grid.dim[k] / e->grid.nc[k]);
+    HIPCHK(sphk::launch_bin_keys(e->d.px, e->d.py, e->d.pz, 1, e->grid, inv_w,
+                                 ni, e->d_key[0], e->d_val[0], e->st));
+    int bits = 30;
+    while ((1LL << (bits - 30)) < (long long)e->ncells) bits++;
+    size_t bytes = e->sort_tmp_bytes;
+    HIPCHK(sphk::sort_pairs(e->d_sort_tmp, bytes, e->d_key[0], e->d_key[1],
+                            e->d_val[0], e->d_soa_buf, ni, bits, e->st));
+    HIPCHK(sphk::launch_invert_perm(e->d_soa_buf, e->d_cand_buf, ni, e->st));
+    e->d.cand = e->d_cand_buf;
+    e->d.soa = e->d_soa_buf;
+    e->launches += 3;
+  }
   e->cell_start.resize(e->ncells + 1);
   HIPCHK(hipMemcpyAsync(e->cell_start.data(), e->d_cell_start,
                         sizeof(int) * (e->ncells + 1), hipMemcpyDeviceToHost,

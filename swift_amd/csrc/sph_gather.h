@@ -261,7 +261,9 @@ __device__ __forceinline__ void run_chunk(
   const int lane = threadIdx.x & 63;
   const int g = lane % C;
   const int s = lane / C;
-  int i, cell_i;
+  // i: the lane's particle (field slot); ic: its record slot, the index
+  // space of the chunk tables (the two differ with the time-bin order)
+  int i, ic, cell_i;
   bool act;
   if (TILE) {
     // the tile builder applied the loop's predicate (and filt narrows it)
@@ -269,11 +271,13 @@ __device__ __forceinline__ void run_chunk(
     const int ts = uniform(tq->tile_start[tl]);
     act = g < uniform(tq->tile_start[tl + 1]) - ts;
     i = act ? tq->alist[ts + g] : 0;
+    ic = act ? rec_slot(d, i) : 0;
     if (tq->filt && act) act = tq->filt[i] != 0;
     cell_i = uniform(tq->tile_cell[tl]);
   } else {
-    i = ca.chunk_start[ich] + g;
-    act = i < ca.chunk_start[ich + 1];
+    ic = ca.chunk_start[ich] + g;
+    act = ic < ca.chunk_start[ich + 1];
+    i = act ? field_slot(d, ic) : 0;
     cell_i = ca.chunk_cell[ich];
     if (act) {
       act = is_active(d.tb[i], la.max_active_bin);
@@ -512,7 +516,7 @@ __device__ __forceinline__ void run_chunk(
         const int nj = min(CJ, uniform(sh.qj1[q]) - js);
         // chunk slot of the lane's own particle (a particle is not its own
         // neighbour); out of the chunk's range unless the chunk holds it
-        const int selfl = uniform(sh.qcell[q]) == cell_i ? i - js : -1;
+        const int selfl = uniform(sh.qcell[q]) == cell_i ? ic - js : -1;
         // i relative to the neighbour image's corner
         const float xs = (float)(st.x - sh.qorg[3 * q]);
         const float ys = (float)(st.y - sh.qorg[3 * q + 1]);
@@ -737,8 +741,9 @@ __global__ SWSPH_LOOP_BOUNDS(L) void k_loop_list(
   } else {
     const int ich = xcd_item(blockIdx.x, threadIdx.x >> 6, ca.nchunks);
     if (ich >= ca.nchunks) return;
-    i = ca.chunk_start[ich] + g;
-    act = i < ca.chunk_start[ich + 1];
+    const int j = ca.chunk_start[ich] + g;  // record slot
+    act = j < ca.chunk_start[ich + 1];
+    if (act) i = field_slot(d, j);
   }
   if (act) act = is_active(d.tb[i], la.max_active_bin);
   typename L::I st;

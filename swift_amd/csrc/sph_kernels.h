@@ -116,7 +116,22 @@ struct DevParts {
   // of every slot
   float *vfx, *vfy, *vfz, *ufull;
   int *perm;
+  // With the time-bin order the fields are in (time bin, Morton) order
+  // inside each cell while the records (rd, rg, rf, pos) and the chunk
+  // tables keep Morton order (the loops' candidates stay spatially
+  // compact): cand[i] is the record slot of field slot i, soa[j] the field
+  // slot of record slot j.  Both null: one order.  Cells cover the same
+  // slot ranges in both.
+  int *cand, *soa;
 };
+
+// Record slot of field slot i, and field slot of record slot j.
+__device__ __forceinline__ int rec_slot(const DevParts &d, int i) {
+  return d.cand ? d.cand[i] : i;
+}
+__device__ __forceinline__ int field_slot(const DevParts &d, int j) {
+  return d.soa ? d.soa[j] : j;
+}
 
 // Per-particle neighbour lists (Verlet lists with a skin): the candidates
 // j with r_ij^2 < lam2 * (gamma_K h_i)^2 found by the chunk loop, kept for

@@ -26,7 +26,7 @@ namespace {
 __global__ void k_build_pos(DevParts d, GridDesc g, float pos_eps, int lo,
                             int hi) {
   const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < hi) d.pos[i] = pos_rec(d, g, pos_eps, i);
+  if (i < hi) d.pos[rec_slot(d, i)] = pos_rec(d, g, pos_eps, i);
 }
 
 // Bounding box (non-inhibited particles) and largest search radius of each
@@ -48,8 +48,9 @@ __global__ __launch_bounds__(kBoundsBlock) void k_chunk_bounds(
   float flo[3] = {3e38f, 3e38f, 3e38f}, fhi[3] = {-3e38f, -3e38f, -3e38f};
   float hm = 0.f, fh = -1.f;
   if (k < c1) {
-    const int i = chunk_start[k] + g;
-    if (i < chunk_start[k + 1] && d.tb[i] != kBinInhibited) {
+    const int j = chunk_start[k] + g;
+    const int i = j < chunk_start[k + 1] ? field_slot(d, j) : 0;
+    if (j < chunk_start[k + 1] && d.tb[i] != kBinInhibited) {
       lo[0] = hi[0] = d.px[i];
       lo[1] = hi[1] = d.py[i];
       lo[2] = hi[2] = d.pz[i];
@@ -118,14 +119,15 @@ __global__ __launch_bounds__(kBoundsBlock) void k_radius_bounds(
   const int g = lane % C;
   float hm = 0.f, fh = -1.f;
   if (k < c1) {
-    const int i = chunk_start[k] + g;
-    if (i < chunk_start[k + 1]) {
+    const int j = chunk_start[k] + g;
+    if (j < chunk_start[k + 1]) {
+      const int i = field_slot(d, j);
       const bool inh = d.tb[i] == kBinInhibited;
       if (!inh) {
         hm = search_radius2(d.h[i]);
         fh = pretest_radius2(hm, pos_eps);
       }
-      if (pos) pos[i].hg2m = inh ? -1.f : fh;
+      if (pos) pos[j].hg2m = inh ? -1.f : fh;
     }
   }
   for (int off = 1; off < C; off <<= 1) {
@@ -196,10 +198,14 @@ __device__ __forceinline__ void build_records(const DevParts &d, R *dst,
   constexpr int Q = (int)(sizeof(R) / sizeof(W));
   __shared__ W buf[kRecBlock * sizeof(R) / sizeof(W)];
   if (mab >= 0) {  // uniform; launched with run_blocks(hi - lo, kRecBlock)
-    if (list) {
-      for_runs_list(list, cnt, d.tb, [&](int i, int8_t tb) {
-        if (is_active(tb, mab)) dst[i] = make(i);
-      });
+    if (list || d.cand) {  // record slots not contiguous: one by one
+      auto one = [&](int i, int8_t tb) {
+        if (is_active(tb, mab)) dst[rec_slot(d, i)] = make(i);
+      };
+      if (list)
+        for_runs_list(list, cnt, d.tb, one);
+      else
+        for_runs(lo, hi, d.tb, one);
       return;
     }
     // kRuns runs of the block's width; a run whose 64 particles of a wave
@@ -233,6 +239,10 @@ __device__ __forceinline__ void build_records(const DevParts &d, R *dst,
     return;
   }
   const int i = lo + blockIdx.x * kRecBlock + threadIdx.x;
+  if (d.cand) {  // record slots not contiguous: one by one
+    if (i < hi) dst[d.cand[i]] = make(i);
+    return;
+  }
   R r{};
   if (i < hi) r = make(i);
   const int w = threadIdx.x >> 6;
@@ -315,7 +325,7 @@ __device__ void prepare_gradient(const DevParts &d, int i, float h, float rho,
   d.bal[i] = bal;
   const float alpha = d.alpha[i];
   // the gradient record, built here from the values at hand
-  if (ga.grad_rec) d.rg[i] = grad_rec_of(d, i, rho, u, c, alpha);
+  if (ga.grad_rec) d.rg[rec_slot(d, i)] = grad_rec_of(d, i, rho, u, c, alpha);
   if (ga.fresh_grad) return;  // the gradient loop starts from these itself
   d.vsig[i] = 2.f * c;
   d.avmax[i] = alpha;
@@ -580,8 +590,8 @@ __global__ void k_extra_ghost(DevParts d, GhostArgs ga, int lo, int hi,
     d.adiff[i] = fminf(na, lim);
     // the force record, built here from the values at hand
     if (ga.force_rec)
-      d.rf[i] = force_rec_of(d, i, tb, h, rho, P, c, alpha, fminf(na, lim),
-                             d.u[i]);
+      d.rf[rec_slot(d, i)] = force_rec_of(d, i, tb, h, rho, P, c, alpha,
+                                          fminf(na, lim), d.u[i]);
     if (ga.fresh_force) return;
     d.ax[i] = 0.f;
     d.ay[i] = 0.f;

@@ -350,7 +350,7 @@ struct ForceLoop {
   using Acc = ForceAcc;
   using I = ForceI;
   static __device__ void load_i(const DevParts &d, int i, I &s) {
-    const ForceRec &r = d.rf[i];
+    const ForceRec &r = d.rf[rec_slot(d, i)];
     s.x = r.x;
     s.y = r.y;
     s.z = r.z;

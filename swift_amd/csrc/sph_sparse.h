@@ -73,7 +73,7 @@ struct SparseLds {
   unsigned int pairs[kSpCI][kSpPairCap];
   unsigned int hits[kSpCI][kSpHitCap];
   float ipre[kSpCI];  // squared radius of the FP32 pre-test (-1: inactive)
-  int ii[kSpCI];      // particle index
+  int ii[kSpCI];      // record slot of each particle (-1: inactive)
 };
 
 // Squared gap between two FP32 boxes (0 when they overlap), with the
@@ -140,7 +140,7 @@ __device__ __forceinline__ void run_sparse(
     sh.ist[g] = st;
     pre = act ? pretest_radius2(st.hg2, la.pos_eps) : -1.f;
     sh.ipre[g] = pre;
-    sh.ii[g] = act ? i : -1;
+    sh.ii[g] = act ? rec_slot(d, i) : -1;  // the self pair, in record slots
   }
   const float maxpre = wave_max(pre);
   wave_sync();
