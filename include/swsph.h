@@ -357,14 +357,17 @@ int swsph_set_step_sched(swsph_engine *e, int on);
  * counterpart in the reference; speed only, as swsph_set_tile_shape. */
 int swsph_set_sparse_shape(swsph_engine *e, int CI, int CJ, int per_cell);
 
-/* In-cell particle order: 0 (default) Morton order inside each top-level
- * cell; 1 by time bin first (shortest steps first), then Morton order, so
- * that the particles active at any step form a prefix of every cell and the
- * per-particle kernels touch contiguous runs of their fields.  Resident
- * particles are re-binned at once (saved states and captured graphs are
- * dropped); later binnings keep the order, which time-step changes between
- * binnings only loosen.  No counterpart in the reference; speed only (the
- * results differ by FP32 summation order at most). */
+/* In-cell particle order: 0 Morton order inside each top-level cell; 1
+ * (default) the fields by time bin first (shortest steps first), then Morton
+ * order, so that the particles active at any step form a prefix of every
+ * cell and the per-particle kernels touch contiguous runs of their fields,
+ * while the loops' candidate records and chunk tables keep the Morton order
+ * of each cell (through a permutation pair set at every binning; none when
+ * all particles share one time bin).  Resident particles are re-binned at
+ * once (saved states and captured graphs are dropped); later binnings keep
+ * the order, which time-step changes between binnings only loosen.  No
+ * counterpart in the reference; speed only (the results differ by FP32
+ * summation order at most). */
 int swsph_set_bin_order(swsph_engine *e, int on);
 int swsph_get_tile_shape(swsph_engine *e, int which, int *CI, int *CJ);
 

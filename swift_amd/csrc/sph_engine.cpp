@@ -161,11 +161,12 @@ struct swsph_engine {
   size_t sort_tmp_bytes = 0;
   // In-cell order by time bin, then Morton (swsph_set_bin_order), and the
   // key layout of the current binning.
-  bool bin_order = false;
+  bool bin_order = true;
   int key_bin_bits = 0;
   // record slot of each field slot and back, with the time-bin order (set
   // into d.cand / d.soa only while the binning uses that order)
   int *d_cand_buf = nullptr, *d_soa_buf = nullptr;
+  unsigned int *d_moved = nullptr;  // set when the two orders differ
   double *d_stage = nullptr;
   // Time integration: parameters, the integer time line, and the largest
   // distance by which a particle lies outside its cell since the last
@@ -345,10 +346,11 @@ void free_dev(swsph_engine *e) {
                   e->d_val[0], e->d_val[1], e->d_sort_tmp, e->d_stage,
                   e->d_subflag, e->d_inv, e->d_subind, e->d_suberr,
                   e->d_cmask, e->d_cact, e->d_clist, e->d_cand_buf,
-                  e->d_soa_buf};
+                  e->d_soa_buf, e->d_moved};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   e->d_cand_buf = e->d_soa_buf = nullptr;
+  e->d_moved = nullptr;
   e->d_subflag = e->d_inv = e->d_subind = nullptr;
   e->d_suberr = nullptr;
   e->sub_cap = e->subind_cap = 0;
@@ -421,18 +423,16 @@ swsph_engine::ChunkTable *chunk_table(swsph_engine *e, int C, int *rc) {
     tp->C = C;
   }
   swsph_engine::ChunkTable &t = *tp;
-  std::vector<int> start, cell_of;
+  // first chunk of every cell on the host (ncells + 1 entries); the chunks'
+  // starts and cells are filled on the device
   t.cell_chunk_h.assign(e->ncells + 1, 0);
+  long long nch = 0;
   for (int c = 0; c < e->ncells; c++) {
-    t.cell_chunk_h[c] = (int)start.size();
-    for (int f = e->cell_start[c]; f < e->cell_start[c + 1]; f += C) {
-      start.push_back(f);
-      cell_of.push_back(c);
-    }
+    t.cell_chunk_h[c] = (int)nch;
+    nch += (e->cell_start[c + 1] - e->cell_start[c] + C - 1) / C;
   }
-  t.nchunks = (int)start.size();
+  t.nchunks = (int)nch;
   t.cell_chunk_h[e->ncells] = t.nchunks;
-  start.push_back((int)e->n);
   // (re)allocate only when the capacity does not suffice
   if (t.nchunks > t.cap_chunks || e->ncells > t.cap_cells) {
     if (e->no_alloc) {
@@ -464,16 +464,17 @@ swsph_engine::ChunkTable *chunk_table(swsph_engine *e, int C, int *rc) {
     t.cap_chunks = t.nchunks;
     t.cap_cells = e->ncells;
   }
-  auto up = [&](int *dst, const std::vector<int> &v) -> int {
-    if (!v.empty())
-      HIPCHK(hipMemcpyAsync(dst, v.data(), sizeof(int) * v.size(),
-                            hipMemcpyHostToDevice, e->st));
-    return 0;
-  };
-  if ((*rc = up(t.d_start, start)) || (*rc = up(t.d_cell_chunk, t.cell_chunk_h)) ||
-      (*rc = up(t.d_chunk_cell, cell_of)))
+  if (hipMemcpyAsync(t.d_cell_chunk, t.cell_chunk_h.data(),
+                     sizeof(int) * t.cell_chunk_h.size(),
+                     hipMemcpyHostToDevice, e->st) != hipSuccess ||
+      sphk::launch_chunk_fill(e->d_cell_start, t.d_cell_chunk, e->ncells, C,
+                              t.nchunks, (int)e->n, t.d_start, t.d_chunk_cell,
+                              e->st) != hipSuccess) {
+    *rc = fail(2, "chunk table fill failed");
     return nullptr;
-  // the host vectors must outlive the asynchronous copies
+  }
+  e->launches++;
+  // the host array is rewritten by the next build of this table
   if (hipStreamSynchronize(e->st) != hipSuccess) {
     *rc = fail(2, "chunk table upload failed");
     return nullptr;
@@ -637,6 +638,7 @@ int alloc_dev(swsph_engine *e, size_t n) {
   HIPCHK(dalloc(&d.pos, n));
   HIPCHK(dalloc(&e->d_cand_buf, n));
   HIPCHK(dalloc(&e->d_soa_buf, n));
+  HIPCHK(dalloc(&e->d_moved, 1));
   {
     TileScratch &t = e->ts;
     const size_t nc = (size_t)e->ncells;
@@ -1270,15 +1272,22 @@ int finish_binning(swsph_engine *e) {
     HIPCHK(sphk::sort_pairs(e->d_sort_tmp, bytes, e->d_key[0], e->d_key[1],
                             e->d_val[0], e->d_soa_buf, ni, bits, e->st));
     HIPCHK(sphk::launch_invert_perm(e->d_soa_buf, e->d_cand_buf, ni, e->st));
-    e->d.cand = e->d_cand_buf;
-    e->d.soa = e->d_soa_buf;
-    e->launches += 3;
+    HIPCHK(sphk::launch_perm_moved(e->d_soa_buf, ni, e->d_moved, e->st));
+    e->launches += 4;
   }
+  unsigned int moved = 0;
+  if (e->key_bin_bits && e->n > 0)
+    HIPCHK(hipMemcpyAsync(&moved, e->d_moved, sizeof(moved),
+                          hipMemcpyDeviceToHost, e->st));
   e->cell_start.resize(e->ncells + 1);
   HIPCHK(hipMemcpyAsync(e->cell_start.data(), e->d_cell_start,
                         sizeof(int) * (e->ncells + 1), hipMemcpyDeviceToHost,
                         e->st));
   HIPCHK(hipStreamSynchronize(e->st));
+  if (moved) {  // otherwise every slot is its own (e.g. a single time bin)
+    e->d.cand = e->d_cand_buf;
+    e->d.soa = e->d_soa_buf;
+  }
   for (LoopKind k : {LoopKind::Density, LoopKind::Gradient, LoopKind::Force})
     HIPCHK(sphk::launch_build_records(k, e->d, 0, (int)e->n, e->st));
   HIPCHK(sphk::launch_build_pos(e->d, e->grid, pos_eps(e), 0, (int)e->n,

@@ -335,6 +335,16 @@ hipError_t launch_bundle(LoopKind kind, const DevParts &d, const LoopArgs &la,
 // Positions / search radii of the candidate test for particles [lo, hi).
 hipError_t launch_build_pos(const DevParts &d, const GridDesc &g, float pos_eps,
                             int lo, int hi, hipStream_t st);
+// *moved = 1 if perm[s] != s for some s < n, else 0.
+hipError_t launch_perm_moved(const int *perm, int n, unsigned int *moved,
+                             hipStream_t st);
+// Fills the per-chunk starts ([nchunks + 1]) and cells ([nchunks]) of a
+// chunk table of capacity C from the cells' slot ranges (cell_start) and
+// their first chunks (cell_chunk, [ncells + 1]); every cell holds
+// ceil(count / C) chunks.
+hipError_t launch_chunk_fill(const int *cell_start, const int *cell_chunk,
+                             int ncells, int C, int nchunks, int n, int *start,
+                             int *chunk_cell, hipStream_t st);
 hipError_t launch_zero_u32(unsigned int *p, int n, hipStream_t st);
 // mask[c] = 1 for the n listed cells, 0 for the other ncells - n.
 hipError_t launch_cell_mask(unsigned char *mask, int ncells, const int *cells,

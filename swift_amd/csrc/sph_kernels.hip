@@ -654,6 +654,37 @@ __global__ void k_invert_perm(const int *perm, int *inv, int n) {
   if (s < n) inv[perm[s]] = s;
 }
 
+// *moved = 1 when perm is not the identity (flag zeroed first).
+__global__ void k_perm_moved(const int *perm, int n, unsigned int *moved) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n && perm[s] != s) *moved = 1u;
+}
+
+// Chunk table of capacity C from the cells' slot ranges: chunk ch of cell c
+// (cell_chunk[c] <= ch < cell_chunk[c + 1]) starts at slot cell_start[c] +
+// C * (ch - cell_chunk[c]); start[nchunks] = n closes the last one.  The
+// cell of a chunk is found by bisection over cell_chunk.
+__global__ void k_chunk_fill(const int *cell_start, const int *cell_chunk,
+                             int ncells, int C, int nchunks, int n,
+                             int *start, int *chunk_cell) {
+  const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ch > nchunks) return;
+  if (ch == nchunks) {
+    start[ch] = n;
+    return;
+  }
+  int lo = 0, hi = ncells;  // cell_chunk[lo] <= ch < cell_chunk[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (cell_chunk[mid] <= ch)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  start[ch] = cell_start[lo] + C * (ch - cell_chunk[lo]);
+  chunk_cell[ch] = lo;
+}
+
 // flag[slot of caller index first + ind[k]] = 1 for k < count; a particle
 // whose slot lies outside [lo, hi) (not in the task's cell) is counted in
 // *err instead.  The caller checked first + ind[k] against the particle
@@ -808,8 +839,31 @@ __device__ __forceinline__ bool tile_pred(const DevParts &d, const LoopArgs &la,
   return true;
 }
 
+// f[s] = 1 for the particles the tiles take, in slot order s.  With two
+// slot orders the slots are record slots (f[j] for the particle in field
+// slot soa[j]), so that the particles of a cell that several active time
+// bins hold still form tiles in the cell's Morton order; with one active
+// bin both orders list them alike.
 __global__ void k_sel_flag(DevParts d, LoopArgs la, int mode, const int *flag,
                            int n, int *f) {
+  if (d.soa) {
+    const int base = blockIdx.x * blockDim.x * kRuns + threadIdx.x;
+    int ii[kRuns];
+#pragma unroll
+    for (int u = 0; u < kRuns; u++) {
+      const int j = base + u * (int)blockDim.x;
+      ii[u] = j < n ? d.soa[j] : -1;
+    }
+    int8_t tb[kRuns];
+#pragma unroll
+    for (int u = 0; u < kRuns; u++) tb[u] = ii[u] >= 0 ? d.tb[ii[u]] : 0;
+#pragma unroll
+    for (int u = 0; u < kRuns; u++)
+      if (ii[u] >= 0)
+        f[base + u * (int)blockDim.x] =
+            tile_pred(d, la, mode, flag, ii[u], tb[u]) ? 1 : 0;
+    return;
+  }
   for_runs(0, n, d.tb, [&](int i, int8_t tb) {
     f[i] = tile_pred(d, la, mode, flag, i, tb) ? 1 : 0;
   });
@@ -852,9 +906,11 @@ __global__ __launch_bounds__(kScanBlock) void k_scan_down(const int *in, int n,
   if (i < n) out[i] = blk[blockIdx.x] + ex;
 }
 
-__global__ void k_sel_scatter(const int *f, const int *S, int n, int *alist) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && f[i]) alist[S[i]] = i;
+// alist[S[s]] = the field slot of slot s (soa[s] with two slot orders).
+__global__ void k_sel_scatter(const int *f, const int *S, int n,
+                              const int *soa, int *alist) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n && f[s]) alist[S[s]] = soa ? soa[s] : s;
 }
 
 __global__ void k_tile_count(const int *cell_start, const int *S, int ncells,
@@ -994,7 +1050,8 @@ hipError_t launch_build_tiles(const DevParts &d, const LoopArgs &la, int n,
   hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(kScanBlock), 0, st,
                      (const int *)ts.flag, n, (const int *)ts.blk, ts.scan);
   hipLaunchKernelGGL(k_sel_scatter, dim3(nblk(n, 256)), dim3(256), 0, st,
-                     (const int *)ts.flag, (const int *)ts.scan, n, ts.alist);
+                     (const int *)ts.flag, (const int *)ts.scan, n,
+                     (const int *)d.soa, ts.alist);
   hipLaunchKernelGGL(k_tile_count, dim3(nblk(ncells, 256)), dim3(256), 0, st,
                      la.cell_start, (const int *)ts.scan, ncells, C, ts.cnt);
   hipLaunchKernelGGL(k_scan_up, dim3(ncb), dim3(kScanBlock), 0, st,
@@ -1337,6 +1394,24 @@ hipError_t launch_share_flags(int *dst, const int *src, int lo, int hi,
   if (hi <= lo) return hipSuccess;
   hipLaunchKernelGGL(k_share_flags, dim3(nblk(hi - lo, 256)), dim3(256), 0, st,
                      dst, src, lo, hi, off, nt);
+  return hipGetLastError();
+}
+
+hipError_t launch_perm_moved(const int *perm, int n, unsigned int *moved,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, (int *)moved, 1);
+  if (n > 0)
+    hipLaunchKernelGGL(k_perm_moved, dim3(nblk(n, 256)), dim3(256), 0, st,
+                       perm, n, moved);
+  return hipGetLastError();
+}
+
+hipError_t launch_chunk_fill(const int *cell_start, const int *cell_chunk,
+                             int ncells, int C, int nchunks, int n, int *start,
+                             int *chunk_cell, hipStream_t st) {
+  hipLaunchKernelGGL(k_chunk_fill, dim3(nblk(nchunks + 1, 256)), dim3(256), 0,
+                     st, cell_start, cell_chunk, ncells, C, nchunks, n, start,
+                     chunk_cell);
   return hipGetLastError();
 }
 

@@ -395,8 +395,9 @@ class Engine:
                                          int(CJ)), "swsph_set_tile_shape")
 
     def set_bin_order(self, on: bool) -> None:
-        """Order particles inside each cell by time bin, then Morton code
-        (the active particles of a step become a prefix of every cell)."""
+        """Order the fields inside each cell by time bin, then Morton code
+        (the active particles of a step become a prefix of every cell; the
+        loops' records keep the Morton order).  On by default."""
         check(lib().swsph_set_bin_order(self._h, int(bool(on))),
               "swsph_set_bin_order")
 
