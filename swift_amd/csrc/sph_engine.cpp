@@ -846,15 +846,16 @@ int queue_loop(swsph_engine *e, LoopKind k, int mode, const int *flag,
   } else if (e->capturing && e->schedule == 1) {
     // Captured: the all-active tiles are counted before the capture (the
     // count holds for every replay of this configuration), with a small
-    // grid behind for any tiles beyond it; the re-run rounds, whose counts
-    // vary, are drained by the persistent grid.
-    if (mode == 0) {
-      const auto it = e->cap_tiles.find(C);
-      if (it != e->cap_tiles.end()) {
-        max_tiles = std::min(max_tiles, it->second);
-        rest = true;
-      }
-    } else {
+    // grid behind for any tiles beyond it.  The re-run rounds' counts vary
+    // but never exceed it (their particles are active ones): with wide
+    // tiles, whose re-runs of a dense box hold most tiles again, they take
+    // the same grid (surplus waves exit at once); narrow tiles' re-runs,
+    // often a few tiles, are drained by the persistent grid.
+    const auto it = e->cap_tiles.find(C);
+    if (it != e->cap_tiles.end() && (mode == 0 || C > kMinTile)) {
+      max_tiles = std::min(max_tiles, it->second);
+      rest = true;
+    } else if (mode != 0) {
       persistent = true;
     }
   }
