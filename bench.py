@@ -507,7 +507,9 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256,
-                    help="particles per axis of the clustered headline box")
+                    help="particles per axis of the clustered headline box "
+                         "(0: skip it, e.g. to profile the other workloads; "
+                         "the line then carries no value)")
     ap.add_argument("--ncell", type=int, default=48,
                     help="top-level cells per axis of the headline box")
     ap.add_argument("--modulated-n", type=int, default=128,
@@ -553,10 +555,17 @@ def main() -> int:
         return 2
 
     shape = tuple(int(c) for c in args.shape.split(":")) if args.shape else None
-    props, st, workload = clustered_state(args.n, args.ncell)
-    head = measure(props, st, workload, args.steps, args.warmup,
-                   not args.no_tune, bool(args.graph), not args.no_plain,
-                   f"clustered{args.n}", shape)
+    if args.n > 0:
+        props, st, workload = clustered_state(args.n, args.ncell)
+        head = measure(props, st, workload, args.steps, args.warmup,
+                       not args.no_tune, bool(args.graph), not args.no_plain,
+                       f"clustered{args.n}", shape)
+    else:
+        st = None
+        workload = "none (headline skipped with --n 0)"
+        head = {"value": None, "ms_per_step": None, "p50_step_ms": None,
+                "p99_step_ms": None, "particles": 0, "active": 0,
+                "roofline": None}
     out = {
         "metric": "SPH pair interactions/s (density+force) batched vs plain on GPU; p50/p99 step ms",
         "value": head["value"],
@@ -583,7 +592,7 @@ def main() -> int:
         if k in head:
             out[k] = head[k]
 
-    if not args.no_cpu:
+    if not args.no_cpu and st is not None:
         threads = (args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", 0))
                    or len(os.sched_getaffinity(0)))
         # the single-thread sample: the same construction at 64^3
