@@ -23,10 +23,12 @@ using namespace sphl;
 namespace {
 
 // ------------------------------------------------------ neighbour records --
+// Thread = record slot j of a cell-aligned slot range (its particle is the
+// field slot soa[j] with two slot orders, in the same cells).
 __global__ void k_build_pos(DevParts d, GridDesc g, float pos_eps, int lo,
                             int hi) {
-  const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < hi) d.pos[rec_slot(d, i)] = pos_rec(d, g, pos_eps, i);
+  const int j = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < hi) d.pos[j] = pos_rec(d, g, pos_eps, field_slot(d, j));
 }
 
 // Bounding box (non-inhibited particles) and largest search radius of each
@@ -238,13 +240,11 @@ __device__ __forceinline__ void build_records(const DevParts &d, R *dst,
     }
     return;
   }
-  const int i = lo + blockIdx.x * kRecBlock + threadIdx.x;
-  if (d.cand) {  // record slots not contiguous: one by one
-    if (i < hi) dst[d.cand[i]] = make(i);
-    return;
-  }
+  // thread = record slot; with two slot orders its particle is the field
+  // slot soa[j] (the same cells' slot range), so the stores stay contiguous
+  const int j = lo + blockIdx.x * kRecBlock + threadIdx.x;
   R r{};
-  if (i < hi) r = make(i);
+  if (j < hi) r = make(field_slot(d, j));
   const int w = threadIdx.x >> 6;
   const int i0 = lo + blockIdx.x * kRecBlock + (w << 6);
   store_wave_records(dst, i0, max(0, min(64, hi - i0)), r, buf + w * 64 * Q);
