@@ -147,6 +147,11 @@ struct swsph_engine {
   uint64_t gen = 1;  // bumped by every change of particles or settings
   uint64_t graph_gen = 0;
   int graph_reruns = 0;
+  // redo count after each captured ghost round (device, pinned readback),
+  // and how many replays in a row needed fewer rounds than captured
+  static constexpr int kHistRounds = 64;
+  unsigned long long *d_hist = nullptr, *h_hist = nullptr;
+  int shrink_votes = 0;
   int last_rounds = 4;  // ghost rounds of the last pass
   bool capturing = false;
   hipGraphExec_t g_density = nullptr, g_finish = nullptr;
@@ -2108,6 +2113,12 @@ int swsph_engine_create(const swsph_props *props, swsph_engine **out) {
   if (hipMalloc((void **)&e->d_redo,
                 kGhostCounters * sizeof(unsigned long long)) !=
           hipSuccess ||
+      hipMalloc((void **)&e->d_hist, swsph_engine::kHistRounds *
+                                         sizeof(unsigned long long)) !=
+          hipSuccess ||
+      hipHostMalloc((void **)&e->h_hist, swsph_engine::kHistRounds *
+                                             sizeof(unsigned long long),
+                    0) != hipSuccess ||
       hipHostMalloc((void **)&e->h_redo,
                     kGhostCounters * sizeof(unsigned long long), 0) !=
           hipSuccess ||
@@ -2149,6 +2160,8 @@ int swsph_engine_destroy(swsph_engine *e) {
   if (e->g_finish) (void)hipGraphExecDestroy(e->g_finish);
   if (e->d_redo) (void)hipFree(e->d_redo);
   if (e->h_redo) (void)hipHostFree(e->h_redo);
+  if (e->d_hist) (void)hipFree(e->d_hist);
+  if (e->h_hist) (void)hipHostFree(e->h_hist);
   if (e->d_cnt) (void)hipFree(e->d_cnt);
   if (e->d_hg2_all) (void)hipFree(e->d_hg2_all);
   if (e->h_hg2_all) (void)hipHostFree(e->h_hg2_all);
@@ -2746,7 +2759,10 @@ static int pass_graph_locked(swsph_engine *e, swsph_pass_stats *stats) {
   if ((rc = ensure_records(e))) return rc;
   e->last_pass_mab = e->props.max_active_bin;
   // Re-runs captured: as many as the last pass needed.
-  const int reruns = std::max(1, std::min(e->props.max_iter, e->last_rounds) - 1);
+  // (at most kHistRounds - 1: one redo count per captured ghost round)
+  const int reruns = std::min(
+      swsph_engine::kHistRounds - 1,
+      std::max(1, std::min(e->props.max_iter, e->last_rounds) - 1));
   if (!e->g_density || e->graph_gen != e->gen || e->graph_reruns != reruns) {
     drop_graphs(e);
     // chunk tables must exist before capture (they allocate)
@@ -2768,13 +2784,22 @@ static int pass_graph_locked(swsph_engine *e, swsph_pass_stats *stats) {
       }
       e->tiles_c = 0;
     }
+    // after each ghost round its redo count is copied aside, so that a
+    // replay tells how many of the captured rounds had work
+    auto ghost_round = [&](int k) -> int {
+      if (int r = phase_ghost(e)) return r;
+      HIPCHK(hipMemcpyAsync(e->d_hist + k, e->d_redo,
+                            sizeof(unsigned long long),
+                            hipMemcpyDeviceToDevice, e->st));
+      return 0;
+    };
     if ((rc = capture(e, &e->g_density, [&]() -> int {
            int r = phase_start(e);
            for (int k = 0; k < reruns && !r; k++) {
-             if ((r = phase_ghost(e))) break;
+             if ((r = ghost_round(k))) break;
              r = phase_rerun(e, 0, 0);
            }
-           return r ? r : phase_ghost(e);
+           return r ? r : ghost_round(reruns);
          })))
       return rc;
     if ((rc = capture(e, &e->g_finish, [&]() -> int { return phase_finish(e); })))
@@ -2783,8 +2808,18 @@ static int pass_graph_locked(swsph_engine *e, swsph_pass_stats *stats) {
     e->graph_reruns = reruns;
   }
   HIPCHK(hipGraphLaunch(e->g_density, e->st));
+  HIPCHK(hipMemcpyAsync(e->h_hist, e->d_hist,
+                        (reruns + 1) * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost, e->st));
   int redo = 0;
-  if ((rc = read_redo(e, &redo))) return rc;
+  if ((rc = read_redo(e, &redo))) return rc;  // waits for the copy too
+  // rounds this replay needed: up to the first ghost that flagged none
+  int needed = reruns + 1;
+  for (int k = 0; k <= reruns; k++)
+    if (e->h_hist[k] == 0) {
+      needed = k + 1;
+      break;
+    }
   int rounds = reruns + 1;
   // more rounds than captured: continue eagerly (and capture more next time)
   while (redo > 0 && rounds < e->props.max_iter) {
@@ -2795,6 +2830,17 @@ static int pass_graph_locked(swsph_engine *e, swsph_pass_stats *stats) {
   }
   if (redo > 0)
     return fail(3, "Smoothing length failed to converge on %d particles", redo);
+  // More rounds than needed were captured: after 3 such replays in a row
+  // the next pass captures fewer (a count that changes from pass to pass
+  // does not recapture every time).
+  if (rounds == reruns + 1 && needed < rounds) {
+    if (++e->shrink_votes >= 3) {
+      rounds = needed;
+      e->shrink_votes = 0;
+    }
+  } else {
+    e->shrink_votes = 0;
+  }
   e->last_rounds = rounds;
   HIPCHK(hipGraphLaunch(e->g_finish, e->st));
   return finish_stats(e, stats, rounds, l0);
